@@ -1,0 +1,204 @@
+/*
+ * fcc.h — C ABI of libfcc_hip.so, the MI355X (gfx950) time step of the
+ * Fourier contour cell model (FourierContourCells.jl).
+ *
+ * Conventions follow the reference's Julia arrays so that a Julia package
+ * extension can pass its `Bodies` fields unchanged:
+ *   - all sizes are int64_t, body counts N, mode count K = maxmode - 1,
+ *     ncoef = 2K + 1, contour nodes n_theta;
+ *   - matrices are column-major with one column per body (coef is ncoef x N,
+ *     the neighbour list is capacity x N);
+ *   - body types and parent ids are 1-based, as in `Bodies.type` /
+ *     `Bodies.parent` (src/bodies.jl:53-72);
+ *   - host arrays are owned by the caller and only copied in or out; the
+ *     library owns every device buffer of a simulation (src/simulation.jl:91-101).
+ * Every entry point returns FCC_OK (0) or a negative error code, with a
+ * message in fcc_last_error() (thread-local). Nothing here takes torch or
+ * C++ types.
+ */
+#ifndef FCC_H
+#define FCC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FCC_ABI_VERSION 1
+
+#define FCC_MAX_TYPES 4        /* body types per simulation */
+#define FCC_MAX_TERMS 8        /* energy terms per simulation */
+#define FCC_MAX_MODES 15       /* K = maxmode - 1 */
+#define FCC_MAX_THETA 64       /* contour nodes handled on the device */
+
+/* Error codes (the reference throws ArgumentError / DimensionMismatch on the
+ * host, src/simulation.jl:71-118, src/basis.jl:36-38, src/bodies.jl:76-77). */
+enum fcc_status {
+    FCC_OK = 0,
+    FCC_ERR_ARGUMENT = -1,      /* ArgumentError in the reference */
+    FCC_ERR_DIMENSION = -2,     /* DimensionMismatch in the reference */
+    FCC_ERR_UNSUPPORTED = -3,   /* valid for the reference, not native here:
+                                   the caller falls back to the reference path */
+    FCC_ERR_DEVICE = -4,        /* HIP runtime error */
+    FCC_ERR_STATE = -5          /* call not valid in the current state */
+};
+
+/* Arithmetic of the time step. */
+enum fcc_precision {
+    FCC_FP64 = 0,        /* everything in FP64 (parity mode, T = Float64) */
+    FCC_FP32 = 1,        /* plain FP32 (T = Float32); positions stored in FP64 */
+    FCC_MIXED_FP16 = 2,  /* contour contractions on MFMA in FP16 hi+lo with FP32
+                            accumulation, contact FP32, a0 and wrap FP64 */
+    FCC_MIXED_BF16 = 3   /* as FCC_MIXED_FP16 with BF16 hi+lo */
+};
+
+/* Built-in energy terms (src/energies/body_terms.jl, src/energies/contact.jl;
+ * Bending as defined in test/test_energies.jl:6-10). */
+enum fcc_term_kind {
+    FCC_TERM_TENSION = 1,               /* p[0] = eta */
+    FCC_TERM_PERIMETER_ELASTICITY = 2,  /* p[0] = lambda, p[1] = p0 */
+    FCC_TERM_BENDING = 3,               /* p[0] = kappa */
+    FCC_TERM_OVERLAP = 16,              /* J, Jrel tables */
+    FCC_TERM_LINEAR_REPULSION = 17      /* p[0] = k (oracle only) */
+};
+
+/* Simulation flags. */
+enum fcc_flags {
+    FCC_FLAG_DEBUG_BUFFERS = 1  /* keep P, GR, GD, GDD and gcoef for inspection */
+};
+
+/* BodyType (src/bodies.jl:30-45). */
+typedef struct fcc_body_type {
+    double v0, Dr, Dt, mur, mutheta, muab;
+    int32_t parent_drag;   /* drag = :parent */
+    int32_t reserved;
+} fcc_body_type;
+
+/* One energy term; `n_on` > 0 restricts it to the listed (1-based) body
+ * types like OnTypes (src/energies/interface.jl:278-284). Overlap couplings
+ * are indexed J[(ti-1)*FCC_MAX_TYPES + (tj-1)] (src/energies/contact.jl:12-28). */
+typedef struct fcc_term {
+    int32_t kind;
+    int32_t n_on;
+    int32_t on_types[FCC_MAX_TYPES];
+    double p[2];
+    double J[FCC_MAX_TYPES * FCC_MAX_TYPES];
+    double Jrel[FCC_MAX_TYPES * FCC_MAX_TYPES];
+} fcc_term;
+
+/* Arguments of Simulation(bodies, box; types, energies, dt, nθ, T, seed, skin)
+ * (src/simulation.jl:66-68). */
+typedef struct fcc_config {
+    int64_t n_bodies;
+    int32_t n_modes;       /* K */
+    int32_t n_theta;
+    int32_t n_types;
+    int32_t n_terms;
+    fcc_body_type types[FCC_MAX_TYPES];
+    fcc_term terms[FCC_MAX_TERMS];
+    double dt, Lx, Ly, skin;
+    uint64_t seed;
+    int32_t precision;     /* enum fcc_precision */
+    int32_t flags;         /* enum fcc_flags */
+} fcc_config;
+
+/* Host view of `Bodies` (src/bodies.jl:62-72). Any pointer may be NULL in
+ * fcc_get_bodies to skip that field. */
+typedef struct fcc_bodies {
+    double *x, *y, *theta;
+    int32_t *imgx, *imgy;
+    double *coef;          /* ncoef x N */
+    double *R0;
+    int32_t *type;         /* 1-based */
+    int32_t *parent;       /* 1-based */
+} fcc_bodies;
+
+typedef struct fcc_sim fcc_sim;
+
+int fcc_abi_version(void);
+const char *fcc_last_error(void);
+
+/* Simulation(...) (src/simulation.jl:66-105): validates like the reference
+ * (check_parents :108-118), enforces the shape constraint and computes the
+ * lab-frame geometry. FCC_ERR_UNSUPPORTED for terms or sizes without a
+ * native kernel. */
+int fcc_create(const fcc_config *cfg, const fcc_bodies *bodies, fcc_sim **out);
+int fcc_destroy(fcc_sim *sim);
+
+/* Replace the body state, then re-enforce the shape, recompute the geometry
+ * and invalidate the neighbour list (the test helper refresh!,
+ * test/runtests.jl:9-15, minus the gradient evaluation). */
+int fcc_set_bodies(fcc_sim *sim, const fcc_bodies *bodies);
+int fcc_get_bodies(fcc_sim *sim, fcc_bodies *bodies);
+int fcc_set_dt(fcc_sim *sim, double dt);
+
+/* step!(sim, n) (src/simulation.jl:165-173): eager steps, one host
+ * synchronisation per step for the skin check. */
+int fcc_step(fcc_sim *sim, int64_t n);
+
+/* run_steps!(backend, sim, n) in graph mode (ext/FourierContourCellsCUDAExt.jl:144-180):
+ * the step is captured in a hipGraph and replayed in chunks with one host
+ * synchronisation per chunk; results are bitwise those of fcc_step. */
+int fcc_run(fcc_sim *sim, int64_t n);
+
+/* update_neighbors!(sim; force) (src/simulation.jl:132-142). */
+int fcc_update_neighbors(fcc_sim *sim, int32_t force);
+
+/* compute_gradients!(sim; frame) (src/simulation.jl:150-158). */
+int fcc_compute_gradients(fcc_sim *sim, int32_t frame);
+
+/* Gradients (src/gradients.jl:15-27): gx, gy, gth, Ebody [N]; glab, gcoef
+ * [ncoef x N]. NULL pointers are skipped. gcoef needs frame = 1 in the last
+ * fcc_compute_gradients or FCC_FLAG_DEBUG_BUFFERS during steps. */
+int fcc_get_gradients(fcc_sim *sim, double *gx, double *gy, double *gth,
+                      double *glab, double *gcoef, double *Ebody);
+
+/* Contour gradients and contact density [n_theta x N] (needs
+ * FCC_FLAG_DEBUG_BUFFERS). */
+int fcc_get_contour_gradients(fcc_sim *sim, double *P, double *GR, double *GD,
+                              double *GDD);
+
+/* Geometry (src/geometry.jl:13-32): lab [ncoef x N], rmax [N], and the
+ * contours R, dR, ddR [n_theta x N] synthesised on demand. */
+int fcc_get_geometry(fcc_sim *sim, double *lab, double *rmax, double *R,
+                     double *dR, double *ddR);
+
+/* perimeters / areas / shape_index of all bodies (src/observables.jl:6-18). */
+int fcc_shape_index(fcc_sim *sim, double *perimeters, double *areas,
+                    double *shape_index);
+
+/* Cell list (src/neighbors.jl:16-33) in the reference's 1-based convention:
+ * cellid, perm [N]; start, stop [ncx*ncy] (empty cells 1/0). */
+int fcc_get_celllist(fcc_sim *sim, int32_t *ncx, int32_t *ncy, int32_t *cellid,
+                     int32_t *perm, int32_t *start, int32_t *stop);
+
+/* Neighbour list (src/neighbors.jl:110-127): capacity and count [N]; idx
+ * [capacity x N] (1-based body ids) if not NULL. */
+int fcc_get_neighbors(fcc_sim *sim, int32_t *capacity, int32_t *count,
+                      int32_t *idx);
+
+/* Shrink or grow the neighbour list matrix and invalidate it, like assigning
+ * sim.neighbors.idx in test/test_contact.jl:81 and test/test_cuda.jl:72. */
+int fcc_set_neighbor_capacity(fcc_sim *sim, int32_t capacity);
+
+/* Counters: step, t (host), device clock (Philox draw index), builds. */
+int fcc_get_counters(fcc_sim *sim, int64_t *step, double *t, int64_t *clock,
+                     int64_t *builds);
+
+/* philox_noise!(ξ, philox_key(seed), draw) (src/random.jl:113-130): 3 x n
+ * standard normals, float (FCC_FP32) or double (FCC_FP64), computed on the
+ * device into host memory `out`. */
+int fcc_philox_noise(uint64_t seed, uint64_t draw, int64_t n, int32_t precision,
+                     void *out);
+
+/* Raw Philox4x32-10 blocks on the device (src/random.jl:25-36): ctr [4 x n],
+ * key [2 x n] -> out [4 x n]. */
+int fcc_philox4x32(const uint32_t *ctr, const uint32_t *key, int64_t n,
+                   uint32_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FCC_H */
