@@ -1,0 +1,197 @@
+#!/usr/bin/env python3
+"""Body-steps/s of the FourierContourCells time step on one MI355X.
+
+Workload (the north-star configuration): hexagonal_monolayer(512, 512;
+packing = 1.0), N = 262144 circles, shape modes n = 2..8 (ncoef 15), nθ = 40,
+BodyType(v0 = 0.1, Dr = 0.01, μab = 0.1), Tension(0.01) + Overlap(1.0),
+dt = 0.005, Philox seed 1234, Verlet skin 0.3. A step is one pass of the time
+step over all bodies; steps run in graph mode (captured hipGraph replayed in
+chunks, one host sync per chunk) with the state resident in HBM.
+
+Prints one JSON line: the mixed-precision path's throughput (`value`), the
+plain FP32 path's for comparison, the roofline of the dominant kernel (forces,
+FP32 VALU) from HIP-event timings taken here, and the FP64 CPU baseline (the
+oracle, OpenMP over bodies) on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "fouriercontourcells.jl_amd"))
+
+import ctypes  # noqa: E402
+
+import numpy as np  # noqa: E402
+
+import fcc_hip as F  # noqa: E402
+
+PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 (vendor figure, dense matrix / packed vector)
+PEAK_HBM_GBS = 8000.0      # MI355X HBM3E
+
+
+def make_sim(n, precision, seed=1234):
+    b, box = F.hexagonal_monolayer(n, n, packing=1.0, maxmode=8, seed=seed)
+    return F.Simulation(b, box, types=(F.BodyType(v0=0.1, Dr=0.01, muab=0.1),),
+                        energies=(F.Tension(0.01), F.Overlap(1.0)), dt=0.005, ntheta=40,
+                        precision=precision, seed=seed, skin=0.3)
+
+
+def sync(sim):
+    import torch
+    torch.cuda.synchronize()
+    sim.counters()  # synchronises the simulation's own stream
+
+
+def timed_run(sim, steps):
+    sync(sim)
+    t0 = time.perf_counter()
+    F.run(sim, steps)
+    sync(sim)
+    return time.perf_counter() - t0
+
+
+def kernel_times(sim, steps):
+    f, u = ctypes.c_double(), ctypes.c_double()
+    rc = sim.engine.time_step_kernels(sim._h, int(steps), ctypes.byref(f), ctypes.byref(u))
+    if rc != 0:
+        raise RuntimeError(sim.engine.last_error())
+    return f.value, u.value
+
+
+def contact_pairs(sim):
+    """(tested, in-range) (node, neighbour) pairs of the current state: the
+    reference tests every listed neighbour at every node and evaluates R_j(ψ)
+    for those within rmax_j (src/energies/contact.jl:67-79)."""
+    geo, bod, nl = sim.geom, sim.bodies, sim.neighbors
+    nth, N = sim.ntheta, sim.N
+    phi = 2 * math.pi * np.arange(nth) / nth
+    ux = bod.x[None, :] + geo.R * np.cos(phi)[:, None]
+    uy = bod.y[None, :] + geo.R * np.sin(phi)[:, None]
+    Lx, Ly = sim.box.Lx, sim.box.Ly
+    tested = int(nl.count.sum()) * nth
+    inr = 0
+    for q in range(int(nl.count.max())):
+        sel = np.flatnonzero(nl.count > q)
+        j = nl.idx[q, sel] - 1
+        dx = ux[:, sel] - bod.x[j][None, :]
+        dx -= Lx * np.round(dx / Lx)
+        dy = uy[:, sel] - bod.y[j][None, :]
+        dy -= Ly * np.round(dy / Ly)
+        inr += int(np.count_nonzero(dx * dx + dy * dy <= (geo.rmax[j] ** 2)[None, :]))
+    return tested, inr
+
+
+def forces_flops(sim, tested, inr):
+    """Algorithmic FLOPs of one forces launch: contour contractions
+    2·[2·(3nθ)·ncoef] per body (synthesis + pullback, src/geometry.jl:114-120,
+    src/gradients.jl:376-382), contact 12 per tested and 7 + 8K per in-range
+    (node, neighbour) pair, and 10 per node for the Tension integrand and its
+    gradient."""
+    N, nth, nc, K = sim.N, sim.ntheta, sim.ncoef, (sim.ncoef - 1) // 2
+    return N * 2 * (2 * 3 * nth * nc) + 12 * tested + (7 + 8 * K) * inr + 10 * nth * N
+
+
+def cpu_baseline(n, seconds=15.0):
+    """FP64 oracle (C restatement of the reference CPU path, OpenMP over
+    bodies) on the same workload: 1 warm-up step, then steps until about
+    `seconds` have elapsed."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    cores = max(1, min(cores, len(os.sched_getaffinity(0))))
+    oracle.set_threads(cores)
+    b, box = F.hexagonal_monolayer(n, n, packing=1.0, maxmode=8, seed=1234)
+    sim = F.Simulation(b, box, types=(F.BodyType(v0=0.1, Dr=0.01, muab=0.1),),
+                       energies=(F.Tension(0.01), F.Overlap(1.0)), dt=0.005, ntheta=40, seed=1234,
+                       skin=0.3, engine=oracle.engine())
+    F.step(sim, 1)
+    steps, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        F.step(sim, 1)
+        steps += 1
+    dt = time.perf_counter() - t0
+    return {"value": sim.N * steps / dt, "unit": "body-steps/s", "cores": cores, "kind": "port",
+            "sample": f"N={sim.N}, FP64 oracle, {steps} steps after 1 warm-up step, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--n", type=int, default=512, help="bodies per lattice row (N = n²)")
+    ap.add_argument("--precision", default="mixed")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the time step has no multi-GPU path in this build "
+              "(see DESIGN.md for how it would shard); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    print(f"[bench] workload N={args.n ** 2} precision={args.precision}", file=sys.stderr, flush=True)
+    sim = make_sim(args.n, args.precision)
+    F.run(sim, args.warmup)
+    sec = timed_run(sim, args.steps)
+    c = sim.counters()
+    value = sim.N * args.steps / sec
+    print(f"[bench] {args.precision}: {value / 1e6:.1f} M body-steps/s ({sec / args.steps * 1e3:.4f} ms/step), "
+          f"builds {c.builds}", file=sys.stderr, flush=True)
+
+    plain = None
+    if not args.no_plain and args.precision != "fp32":
+        ps = make_sim(args.n, "fp32")
+        F.run(ps, args.warmup)
+        psec = timed_run(ps, args.steps)
+        plain = ps.N * args.steps / psec
+        print(f"[bench] fp32: {plain / 1e6:.1f} M body-steps/s", file=sys.stderr, flush=True)
+        ps.close()
+
+    fms, ums = kernel_times(sim, 50)
+    tested, inr = contact_pairs(sim)
+    flops = forces_flops(sim, tested, inr)
+    achieved = flops / (fms * 1e-3) / 1e12
+    upd_bytes = 340 * sim.N  # per body-step, 188 B read + 152 B written
+    print(f"[bench] forces {fms * 1e3:.1f} us, update {ums * 1e3:.1f} us, pairs tested {tested}, "
+          f"in range {inr}", file=sys.stderr, flush=True)
+
+    cpu = None if args.no_cpu else cpu_baseline(args.n, args.cpu_seconds)
+
+    line = {
+        "metric": "body-steps/sec at N=262144 (mixed vs plain FP32, 1 GPU) @ force err ≤1e-4",
+        "value": value,
+        "unit": "body-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sec / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.precision != "fp64" else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"hexagonal_monolayer({args.n},{args.n}; packing=1.0), N={sim.N}, "
+                               f"modes 2..8, ntheta=40, Tension(0.01)+Overlap(1.0), v0=0.1, Dr=0.01, "
+                               f"dt=0.005, skin=0.3, graph mode",
+                   "precision": args.precision, "N": sim.N},
+        "plain_fp32_value": plain,
+        "mixed_over_plain": (value / plain) if plain else None,
+        "roofline": {"kernel": "k_forces", "bound": "mfma", "achieved": achieved,
+                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+                     "traffic": None, "launch_ms": fms, "flops_per_launch": flops},
+        "roofline_update": {"kernel": "k_update", "bound": "hbm",
+                            "achieved": upd_bytes / (ums * 1e-3) / 1e9, "peak": PEAK_HBM_GBS,
+                            "unit": "GB/s", "frac": upd_bytes / (ums * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                            "launch_ms": ums},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,919 @@
+// fcc_api.hip — host orchestration of the gfx950 time step and the C ABI of
+// include/fcc.h (libfcc_hip.so).
+//
+// One simulation = one HIP stream and device buffers it owns. A step is two
+// kernels (forces, update; three with parent drag), eager (fcc_step, one
+// host read of the rebuild flag per step, like step! in src/simulation.jl:165-173)
+// or replayed from a captured hipGraph in chunks (fcc_run, like
+// run_steps! in ext/FourierContourCellsCUDAExt.jl:144-180).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "fcc_kernels.hpp"
+#include "../../include/fcc_bench.h"
+
+using namespace fcc;
+
+static thread_local char g_err[512];
+
+static int fail(int code, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIPCHK(x)                                                                         \
+    do {                                                                                  \
+        hipError_t e_ = (x);                                                              \
+        if (e_ != hipSuccess) return fail(FCC_ERR_DEVICE, "%s: %s", #x, hipGetErrorString(e_)); \
+    } while (0)
+
+static inline unsigned blocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+// Graph capture is serialised (ext/FourierContourCellsCUDAExt.jl:80-84).
+static std::mutex g_capture_lock;
+
+// ------------------------------------------------------------- validation
+
+// Simulation(...) checks (src/simulation.jl:71-118, src/basis.jl:36-38,
+// src/neighbors.jl:334) and the native-kernel limits.
+static int validate(const fcc_config *cfg, const fcc_bodies *b) {
+    if (!cfg || !b) return fail(FCC_ERR_ARGUMENT, "null argument");
+    if (cfg->n_bodies < 1) return fail(FCC_ERR_ARGUMENT, "need at least one body");
+    if (cfg->n_modes < 1) return fail(FCC_ERR_ARGUMENT, "maxmode must be ≥ 2");
+    if (cfg->n_theta <= 2 * (cfg->n_modes + 1)) return fail(FCC_ERR_ARGUMENT, "nθ must exceed 2·maxmode");
+    if (cfg->skin < 0.0) return fail(FCC_ERR_ARGUMENT, "skin must be non-negative");
+    if (cfg->n_types < 1 || cfg->n_types > FCC_MAX_TYPES)
+        return fail(FCC_ERR_UNSUPPORTED, "1..%d body types are native", FCC_MAX_TYPES);
+    if (cfg->n_terms < 0 || cfg->n_terms > FCC_MAX_TERMS)
+        return fail(FCC_ERR_UNSUPPORTED, "at most %d energy terms", FCC_MAX_TERMS);
+    if (cfg->n_modes > FCC_MAX_MODES) return fail(FCC_ERR_UNSUPPORTED, "at most %d shape modes are native", FCC_MAX_MODES);
+    if (cfg->n_theta > FCC_MAX_THETA) return fail(FCC_ERR_UNSUPPORTED, "nθ > %d has no native kernel", FCC_MAX_THETA);
+    if (cfg->n_bodies > (int64_t)INT32_MAX) return fail(FCC_ERR_UNSUPPORTED, "too many bodies");
+    if (cfg->precision < FCC_FP64 || cfg->precision > FCC_MIXED_BF16) return fail(FCC_ERR_ARGUMENT, "unknown precision");
+    if (!b->x || !b->y || !b->theta || !b->coef || !b->R0) return fail(FCC_ERR_DIMENSION, "missing body arrays");
+    const int64_t N = cfg->n_bodies;
+    for (int64_t i = 0; i < N; i++) {
+        int32_t t = b->type ? b->type[i] : 1;
+        if (t < 1 || t > cfg->n_types) return fail(FCC_ERR_ARGUMENT, "body type without BodyType entry");
+    }
+    for (int64_t i = 0; i < N; i++) {
+        int32_t p = b->parent ? b->parent[i] : (int32_t)(i + 1);
+        if (p < 1 || p > N)
+            return fail(FCC_ERR_ARGUMENT, "body %lld has parent %d outside 1:%lld", (long long)(i + 1), p, (long long)N);
+        int32_t t = b->type ? b->type[i] : 1;
+        if (!cfg->types[t - 1].parent_drag) continue;
+        if (p == i + 1)
+            return fail(FCC_ERR_ARGUMENT, "body %lld has drag = :parent but is its own parent", (long long)(i + 1));
+        int32_t tp = b->type ? b->type[p - 1] : 1;
+        if (cfg->types[tp - 1].parent_drag)
+            return fail(FCC_ERR_ARGUMENT, "body %lld has drag = :parent, but so has its parent %d (one level only)",
+                        (long long)(i + 1), p);
+    }
+    int nbody = 0;
+    for (int q = 0; q < cfg->n_terms; q++) {
+        const int k = cfg->terms[q].kind;
+        if (k == FCC_TERM_TENSION || k == FCC_TERM_PERIMETER_ELASTICITY || k == FCC_TERM_BENDING)
+            nbody++;
+        else if (k == FCC_TERM_LINEAR_REPULSION)
+            return fail(FCC_ERR_UNSUPPORTED, "LinearRepulsion has no native kernel");
+        else if (k != FCC_TERM_OVERLAP)
+            return fail(FCC_ERR_ARGUMENT, "energy terms must be BodyTerms, ContactTerms or BondTerms");
+    }
+    if (nbody > MAXB) return fail(FCC_ERR_UNSUPPORTED, "at most %d native body terms", MAXB);
+    return FCC_OK;
+}
+
+static uint32_t type_mask(const fcc_term &t) {
+    if (t.n_on <= 0) return 0xFFFFFFFFu;
+    uint32_t m = 0;
+    for (int q = 0; q < t.n_on && q < FCC_MAX_TYPES; q++)
+        if (t.on_types[q] >= 1 && t.on_types[q] <= 32) m |= 1u << (t.on_types[q] - 1);
+    return m;
+}
+
+// ------------------------------------------------------------- simulation
+
+struct fcc_sim {
+    virtual ~fcc_sim() {}
+    virtual int set_bodies(const fcc_bodies *b) = 0;
+    virtual int get_bodies(fcc_bodies *b) = 0;
+    virtual int set_dt(double dt) = 0;
+    virtual int step(int64_t n) = 0;
+    virtual int run(int64_t n) = 0;
+    virtual int update_neighbors(int force) = 0;
+    virtual int compute_gradients(int frame) = 0;
+    virtual int get_gradients(double *, double *, double *, double *, double *, double *) = 0;
+    virtual int get_contour_gradients(double *, double *, double *, double *) = 0;
+    virtual int get_geometry(double *, double *, double *, double *, double *) = 0;
+    virtual int shape_index(double *, double *, double *) = 0;
+    virtual int get_celllist(int32_t *, int32_t *, int32_t *, int32_t *, int32_t *, int32_t *) = 0;
+    virtual int get_neighbors(int32_t *, int32_t *, int32_t *) = 0;
+    virtual int set_neighbor_capacity(int32_t) = 0;
+    virtual int get_counters(int64_t *, double *, int64_t *, int64_t *) = 0;
+    virtual int time_step_kernels(int64_t, double *, double *) = 0;
+};
+
+template <typename T>
+struct Sim final : fcc_sim {
+    fcc_config cfg;
+    Params<T> p{};
+    Ptrs<T> d{};
+    hipStream_t stream = nullptr;
+    int64_t N = 0;
+    int K = 0, nc = 0, ldc = 0, nth = 0, KM = 0;
+    bool mixed = false, debug = false, has_contact = false, has_pdrag = false;
+    double skin = 0.0, range = 0.0;
+    int32_t ncx = 1, ncy = 1;
+    double wx = 0, wy = 0;
+    int64_t nce = 1;
+    int32_t *cellid = nullptr, *perm = nullptr, *cstart = nullptr, *ccnt = nullptr, *cfill = nullptr;
+    bool valid = false;
+    int64_t builds = 0, nstep = 0;
+    double t = 0.0;
+    std::vector<void *> allocs;
+    // graph mode (ext/FourierContourCellsCUDAExt.jl:52-71)
+    hipGraphExec_t gexec = nullptr;
+    int gen = 0, gexec_gen = -1;
+    double interval = 16.0;
+    int64_t since_rebuild = 0, gbuilds = 0;
+
+    ~Sim() override {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        for (void *q : allocs) (void)hipFree(q);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    template <typename U>
+    int alloc(U **out, size_t n) {
+        void *q = nullptr;
+        HIPCHK(hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(U)));
+        allocs.push_back(q);
+        *out = (U *)q;
+        return FCC_OK;
+    }
+    template <typename U>
+    void release(U *q) {
+        if (!q) return;
+        auto it = std::find(allocs.begin(), allocs.end(), (void *)q);
+        if (it != allocs.end()) allocs.erase(it);
+        (void)hipFree(q);
+    }
+
+    int init(const fcc_config *c, const fcc_bodies *b) {
+        cfg = *c;
+        N = c->n_bodies;
+        K = c->n_modes;
+        nc = 2 * K + 1;
+        ldc = nc <= 16 ? 16 : 32;
+        nth = c->n_theta;
+        KM = K <= 5 ? 5 : (K <= 7 ? 7 : 15);
+        mixed = c->precision == FCC_MIXED_FP16 || c->precision == FCC_MIXED_BF16;
+        debug = (c->flags & FCC_FLAG_DEBUG_BUFFERS) != 0;
+        skin = c->skin;
+        HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+
+        p.N = N;
+        p.K = K;
+        p.nc = nc;
+        p.ldc = ldc;
+        p.nth = nth;
+        const double dphi = 2.0 * 3.141592653589793 / (double)nth;
+        p.dphi = (T)dphi;
+        p.Lx = c->Lx;
+        p.Ly = c->Ly;
+        // terms (src/simulation.jl:73-76)
+        int nb = 0;
+        int overlaps[FCC_MAX_TERMS];
+        int nov = 0;
+        for (int q = 0; q < c->n_terms; q++) {
+            const fcc_term &tm = c->terms[q];
+            if (tm.kind == FCC_TERM_OVERLAP) {
+                overlaps[nov++] = q;
+                continue;
+            }
+            p.bkind[nb] = tm.kind;
+            p.bmask[nb] = type_mask(tm);
+            p.bp0[nb] = (T)tm.p[0];
+            p.bp1[nb] = (T)tm.p[1];
+            if (tm.kind == FCC_TERM_BENDING) p.need_dd = 1;
+            nb++;
+        }
+        p.nbody = nb;
+        has_contact = nov > 0;
+        p.ncontact = has_contact ? 1 : 0;
+        // combined Overlap table: right fold over the terms (contact.jl:46-51)
+        for (int rel = 0; rel < 2; rel++)
+            for (int ti = 0; ti < FCC_MAX_TYPES; ti++)
+                for (int tj = 0; tj < FCC_MAX_TYPES; tj++) {
+                    double acc = 0.0;
+                    for (int q = nov - 1; q >= 0; q--) {
+                        const fcc_term &tm = c->terms[overlaps[q]];
+                        double g = 0.0;
+                        if (type_mask(tm) & (1u << ti))
+                            g = rel ? tm.Jrel[ti * FCC_MAX_TYPES + tj] : tm.J[ti * FCC_MAX_TYPES + tj];
+                        acc = g + acc;
+                    }
+                    p.Jc[rel][ti][tj] = (T)acc;
+                }
+        range = 0.0;  // interaction_range(Overlap) = 0
+        p.range = (T)range;
+        for (int q = 0; q < FCC_MAX_TYPES; q++) {
+            const fcc_body_type &bt = c->types[q];
+            p.v0[q] = (T)bt.v0;
+            p.Dr[q] = (T)bt.Dr;
+            p.Dt[q] = (T)bt.Dt;
+            p.mur[q] = (T)bt.mur;
+            p.muth[q] = (T)bt.mutheta;
+            p.muab[q] = (T)bt.muab;
+            p.pdrag[q] = q < c->n_types ? bt.parent_drag : 0;
+            if (q < c->n_types && bt.parent_drag) has_pdrag = true;
+        }
+        p.dt = (T)c->dt;
+        p.key0 = (uint32_t)c->seed;
+        p.key1 = (uint32_t)(c->seed >> 32);
+        p.skin = (T)skin;
+
+        // buffers
+        int rc = 0;
+        if ((rc = alloc(&d.x, N)) || (rc = alloc(&d.y, N)) || (rc = alloc(&d.imgx, N)) || (rc = alloc(&d.imgy, N)) ||
+            (rc = alloc(&d.th, N)) || (rc = alloc(&d.coef, N * ldc)) || (rc = alloc(&d.R0, N)) ||
+            (rc = alloc(&d.type, N)) || (rc = alloc(&d.parent, N)) || (rc = alloc(&d.lab, N * ldc)) ||
+            (rc = alloc(&d.rmax, N)) || (rc = alloc(&d.glab, N * ldc)) || (rc = alloc(&d.gx, N)) ||
+            (rc = alloc(&d.gy, N)) || (rc = alloc(&d.gth, N)) || (rc = alloc(&d.Ebody, N)) ||
+            (rc = alloc(&d.gcoef, N * ldc)) || (rc = alloc(&d.st, 1)))
+            return rc;
+        HIPCHK(hipMemsetAsync(d.coef, 0, sizeof(T) * N * ldc, stream));
+        HIPCHK(hipMemsetAsync(d.lab, 0, sizeof(T) * N * ldc, stream));
+        HIPCHK(hipMemsetAsync(d.glab, 0, sizeof(T) * N * ldc, stream));
+        HIPCHK(hipMemsetAsync(d.gcoef, 0, sizeof(T) * N * ldc, stream));
+        HIPCHK(hipMemsetAsync(d.gx, 0, sizeof(T) * N, stream));
+        HIPCHK(hipMemsetAsync(d.gy, 0, sizeof(T) * N, stream));
+        HIPCHK(hipMemsetAsync(d.gth, 0, sizeof(T) * N, stream));
+        HIPCHK(hipMemsetAsync(d.Ebody, 0, sizeof(T) * N, stream));
+        HIPCHK(hipMemsetAsync(d.st, 0, sizeof(DevStatus), stream));
+        if (debug) {
+            if ((rc = alloc(&d.P, N * nth)) || (rc = alloc(&d.GR, N * nth)) || (rc = alloc(&d.GD, N * nth)) ||
+                (rc = alloc(&d.GDD, N * nth)))
+                return rc;
+            HIPCHK(hipMemsetAsync(d.P, 0, sizeof(T) * N * nth, stream));
+        }
+        // basis (src/basis.jl:35-63) in the device row layout
+        std::vector<T> S((size_t)3 * nth * SB, T(0)), cs((size_t)2 * nth);
+        for (int k = 0; k < nth; k++) {
+            const double phi = (double)k * dphi;
+            cs[2 * k] = (T)std::cos(phi);
+            cs[2 * k + 1] = (T)std::sin(phi);
+            S[(size_t)k * SB] = T(1);
+            for (int m = 1; m <= K; m++) {
+                const int n = m + 1;
+                const double sn = std::sin((double)n * phi), cn = std::cos((double)n * phi);
+                S[(size_t)k * SB + m] = (T)cn;
+                S[(size_t)k * SB + 16 + m] = (T)sn;
+                S[(size_t)(nth + k) * SB + m] = (T)((double)(-n) * sn);
+                S[(size_t)(nth + k) * SB + 16 + m] = (T)((double)n * cn);
+                S[(size_t)(2 * nth + k) * SB + m] = (T)((double)(-n * n) * cn);
+                S[(size_t)(2 * nth + k) * SB + 16 + m] = (T)((double)(-n * n) * sn);
+            }
+        }
+        T *dS, *dcs;
+        if ((rc = alloc(&dS, S.size())) || (rc = alloc(&dcs, cs.size()))) return rc;
+        HIPCHK(hipMemcpyAsync(dS, S.data(), sizeof(T) * S.size(), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(dcs, cs.data(), sizeof(T) * cs.size(), hipMemcpyHostToDevice, stream));
+        d.S = dS;
+        d.cs = dcs;
+
+        // cell grid (src/simulation.jl:90, src/neighbors.jl:238-244)
+        double maxR0 = 0.0;
+        for (int64_t i = 0; i < N; i++) maxR0 = std::max(maxR0, b->R0[i]);
+        const double cutoff = 2.0 * maxR0 * (1.0 + std::sqrt(2.0 / 3.0)) + range + skin;
+        auto ncells = [&](double L) {
+            int n = (int)std::floor(L / cutoff);
+            return n >= 3 ? n : 1;
+        };
+        ncx = ncells(c->Lx);
+        ncy = ncells(c->Ly);
+        wx = c->Lx / ncx;
+        wy = c->Ly / ncy;
+        nce = (int64_t)ncx * ncy;
+        if ((rc = alloc(&cellid, N)) || (rc = alloc(&perm, N)) || (rc = alloc(&cstart, nce)) ||
+            (rc = alloc(&ccnt, nce)) || (rc = alloc(&cfill, nce)))
+            return rc;
+        HIPCHK(hipMemsetAsync(cellid, 0, sizeof(int32_t) * N, stream));
+        HIPCHK(hipMemsetAsync(perm, 0, sizeof(int32_t) * N, stream));
+        HIPCHK(hipMemsetAsync(cstart, 0, sizeof(int32_t) * nce, stream));
+        HIPCHK(hipMemsetAsync(ccnt, 0, sizeof(int32_t) * nce, stream));
+        // neighbour list (capacity 32, src/neighbors.jl:332-338)
+        d.cap = 32;
+        if ((rc = alloc(&d.nidx, (size_t)d.cap * N)) || (rc = alloc(&d.ncount, N)) || (rc = alloc(&d.x0, N)) ||
+            (rc = alloc(&d.y0, N)) || (rc = alloc(&d.rmax0, N)))
+            return rc;
+        HIPCHK(hipMemsetAsync(d.nidx, 0, sizeof(int32_t) * d.cap * N, stream));
+        HIPCHK(hipMemsetAsync(d.ncount, 0, sizeof(int32_t) * N, stream));
+        // children (src/bodies.jl:121-135), slot-major
+        std::vector<int32_t> cnt(N, 0);
+        for (int64_t k = 0; k < N; k++) {
+            int32_t pr = (b->parent ? b->parent[k] : (int32_t)(k + 1)) - 1;
+            if (pr != k) cnt[pr]++;
+        }
+        int32_t cmax = 1;
+        for (int64_t k = 0; k < N; k++) cmax = std::max(cmax, cnt[k]);
+        d.ccap = cmax;
+        std::vector<int32_t> cidx((size_t)cmax * N, 0);
+        std::fill(cnt.begin(), cnt.end(), 0);
+        for (int64_t k = 0; k < N; k++) {
+            int32_t pr = (b->parent ? b->parent[k] : (int32_t)(k + 1)) - 1;
+            if (pr == k) continue;
+            cidx[(size_t)cnt[pr] * N + pr] = (int32_t)k;
+            cnt[pr]++;
+        }
+        if ((rc = alloc(&d.cidx, cidx.size())) || (rc = alloc(&d.ccount, N))) return rc;
+        HIPCHK(hipMemcpy(d.cidx, cidx.data(), sizeof(int32_t) * cidx.size(), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d.ccount, cnt.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice));
+        if (has_pdrag) {
+            if ((rc = alloc(&d.disp, 3 * N))) return rc;
+            HIPCHK(hipMemsetAsync(d.disp, 0, sizeof(T) * 3 * N, stream));
+        }
+        return set_bodies(b);
+    }
+
+    int launch_init() {
+        const unsigned g = blocks(N, 256);
+        switch (KM) {
+        case 5:
+            if (mixed) hipLaunchKernelGGL((k_init<T, 5, true>), g, 256, 0, stream, p, d);
+            else hipLaunchKernelGGL((k_init<T, 5, false>), g, 256, 0, stream, p, d);
+            break;
+        case 7:
+            if (mixed) hipLaunchKernelGGL((k_init<T, 7, true>), g, 256, 0, stream, p, d);
+            else hipLaunchKernelGGL((k_init<T, 7, false>), g, 256, 0, stream, p, d);
+            break;
+        default:
+            if (mixed) hipLaunchKernelGGL((k_init<T, 15, true>), g, 256, 0, stream, p, d);
+            else hipLaunchKernelGGL((k_init<T, 15, false>), g, 256, 0, stream, p, d);
+        }
+        HIPCHK(hipGetLastError());
+        return FCC_OK;
+    }
+
+    int launch_forces(int use_status) {
+        const unsigned g = blocks(N, 64);
+        const int dbg = debug ? 1 : 0;
+#define FCC_FORCES(KMV)                                                                                  \
+    if (nth == 40) hipLaunchKernelGGL((k_forces<T, 40, KMV>), g, 64, 0, stream, p, d, use_status, dbg);   \
+    else hipLaunchKernelGGL((k_forces<T, 0, KMV>), g, 64, 0, stream, p, d, use_status, dbg);
+        switch (KM) {
+        case 5: FCC_FORCES(5) break;
+        case 7: FCC_FORCES(7) break;
+        default: FCC_FORCES(15)
+        }
+#undef FCC_FORCES
+        HIPCHK(hipGetLastError());
+        return FCC_OK;
+    }
+
+    int launch_update() {
+        const unsigned g = blocks(N, 256);
+        const int ud = has_pdrag ? 1 : 0, dbg = debug ? 1 : 0;
+        if (has_pdrag) hipLaunchKernelGGL((k_displacement<T>), g, 256, 0, stream, p, d);
+#define FCC_UPD(KMV)                                                                                   \
+    if (mixed) hipLaunchKernelGGL((k_update<T, KMV, true>), g, 256, 0, stream, p, d, ud, dbg);          \
+    else hipLaunchKernelGGL((k_update<T, KMV, false>), g, 256, 0, stream, p, d, ud, dbg);
+        switch (KM) {
+        case 5: FCC_UPD(5) break;
+        case 7: FCC_UPD(7) break;
+        default: FCC_UPD(15)
+        }
+#undef FCC_UPD
+        HIPCHK(hipGetLastError());
+        return FCC_OK;
+    }
+
+    // advance! (src/simulation.jl:181-191) as device work on the stream.
+    int device_step() {
+        p.skin_check = (has_contact && valid && skin > 0.0) ? 1 : 0;
+        int rc = launch_forces(1);
+        if (rc) return rc;
+        return launch_update();
+    }
+
+    // update_celllist! + update_neighbors!(nl) (src/neighbors.jl:81-92,223-242).
+    int rebuild() {
+        const unsigned gN = blocks(N, 256), gC = blocks(nce, 256);
+        hipLaunchKernelGGL(k_fill_i32, gC, 256, 0, stream, nce, ccnt, 0);
+        hipLaunchKernelGGL(k_cell_index, gN, 256, 0, stream, N, d.x, d.y, ncx, ncy, wx, wy, cellid, ccnt);
+        hipLaunchKernelGGL(k_scan, 1, 1024, 0, stream, nce, ccnt, cstart, cfill);
+        hipLaunchKernelGGL(k_scatter, gN, 256, 0, stream, N, cellid, cfill, perm);
+        hipLaunchKernelGGL(k_cell_sort, gC, 256, 0, stream, nce, cstart, ccnt, perm);
+        HIPCHK(hipGetLastError());
+        for (;;) {
+            hipLaunchKernelGGL(k_clear_status, 1, 1, 0, stream, d.st);
+            hipLaunchKernelGGL((k_neighbors<T>), gN, 256, 0, stream, N, d.x, d.y, d.rmax, perm, cstart, ccnt,
+                               p.Lx, p.Ly, ncx, ncy, wx, wy, range + skin, d.cap, d.nidx, d.ncount, d.st);
+            HIPCHK(hipGetLastError());
+            DevStatus st;
+            HIPCHK(hipMemcpyAsync(&st, d.st, sizeof st, hipMemcpyDeviceToHost, stream));
+            HIPCHK(hipStreamSynchronize(stream));
+            if (st.overflow <= d.cap) break;
+            const int32_t cap = 2 * st.overflow;
+            int32_t *ni = nullptr;
+            int rc = alloc(&ni, (size_t)cap * N);
+            if (rc) return rc;
+            HIPCHK(hipMemsetAsync(ni, 0, sizeof(int32_t) * cap * N, stream));
+            release(d.nidx);
+            d.nidx = ni;
+            d.cap = cap;
+            gen++;
+        }
+        if (skin > 0.0)
+            hipLaunchKernelGGL((k_copy_skin_ref<T>), gN, 256, 0, stream, N, d.x, d.y, d.rmax, d.x0, d.y0, d.rmax0);
+        hipLaunchKernelGGL(k_clear_status, 1, 1, 0, stream, d.st);
+        HIPCHK(hipGetLastError());
+        valid = true;
+        builds++;
+        return FCC_OK;
+    }
+
+    // update_neighbors!(sim; force) (src/simulation.jl:132-142). The skin
+    // check of the current state is the flag raised by the fused check of the
+    // last update (the same test on the same state).
+    int update_neighbors(int force) override {
+        if (!force && valid && skin > 0.0) {
+            DevStatus st;
+            HIPCHK(hipMemcpyAsync(&st, d.st, sizeof st, hipMemcpyDeviceToHost, stream));
+            HIPCHK(hipStreamSynchronize(stream));
+            if (st.rebuild == 0) return FCC_OK;
+        }
+        return rebuild();
+    }
+
+    int step(int64_t n) override {
+        for (int64_t q = 0; q < n; q++) {
+            int rc;
+            if (has_contact && (rc = update_neighbors(0))) return rc;
+            if ((rc = device_step())) return rc;
+            t += cfg.dt;
+            nstep++;
+        }
+        HIPCHK(hipStreamSynchronize(stream));
+        return FCC_OK;
+    }
+
+    // run_steps! in graph mode (ext/FourierContourCellsCUDAExt.jl:65-71,133-180).
+    int capture() {
+        std::lock_guard<std::mutex> lk(g_capture_lock);
+        if (gexec) {
+            HIPCHK(hipGraphExecDestroy(gexec));
+            gexec = nullptr;
+        }
+        p.skin_check = (has_contact && skin > 0.0) ? 1 : 0;
+        hipGraph_t graph;
+        HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        int rc = launch_forces(1);
+        if (!rc) rc = launch_update();
+        hipError_t e = hipStreamEndCapture(stream, &graph);
+        if (rc) return rc;
+        HIPCHK(e);
+        HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+        HIPCHK(hipGraphDestroy(graph));
+        gexec_gen = gen;
+        return FCC_OK;
+    }
+
+    int run(int64_t n) override {
+        if (n <= 0) return FCC_OK;
+        if (has_contact && skin <= 0.0) return step(n);  // graph_capable (src/simulation.jl:206-207)
+        int rc;
+        if (has_contact && (rc = update_neighbors(valid ? 0 : 1))) return rc;
+        DevStatus st;
+        HIPCHK(hipMemcpyAsync(&st, d.st, sizeof st, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        int64_t clock = st.clock, done = 0;
+        while (done < n) {
+            if (!gexec || gexec_gen != gen) {
+                if ((rc = capture())) return rc;
+            }
+            if (builds != gbuilds) {
+                gbuilds = builds;
+                since_rebuild = 0;
+            }
+            const int64_t chunk = has_contact ? std::min<int64_t>(256, std::max<int64_t>(4, (int64_t)std::llround(interval / 4))) : 256;
+            const int64_t k = std::min(chunk, n - done);
+            for (int64_t q = 0; q < k; q++) HIPCHK(hipGraphLaunch(gexec, stream));
+            HIPCHK(hipMemcpyAsync(&st, d.st, sizeof st, hipMemcpyDeviceToHost, stream));
+            HIPCHK(hipStreamSynchronize(stream));
+            const int64_t executed = st.clock - clock;
+            clock = st.clock;
+            for (int64_t q = 0; q < executed; q++) t += cfg.dt;
+            nstep += executed;
+            done += executed;
+            since_rebuild += executed;
+            if (st.rebuild != 0 && done < n) {
+                if ((rc = update_neighbors(0))) return rc;
+                interval = (interval + (double)since_rebuild) / 2;
+                since_rebuild = 0;
+                gbuilds = builds;
+            } else if ((double)since_rebuild > interval) {
+                interval = std::min((double)since_rebuild, 4.0 * 256);
+            }
+        }
+        return FCC_OK;
+    }
+
+    int compute_gradients(int frame) override {
+        if (has_contact && !valid) return fail(FCC_ERR_STATE, "neighbour list not built (call fcc_update_neighbors)");
+        int rc = launch_forces(0);
+        if (rc) return rc;
+        if (frame) {
+            const unsigned g = blocks(N, 256);
+            switch (KM) {
+            case 5: hipLaunchKernelGGL((k_body_frame<T, 5>), g, 256, 0, stream, p, d); break;
+            case 7: hipLaunchKernelGGL((k_body_frame<T, 7>), g, 256, 0, stream, p, d); break;
+            default: hipLaunchKernelGGL((k_body_frame<T, 15>), g, 256, 0, stream, p, d);
+            }
+            HIPCHK(hipGetLastError());
+        }
+        HIPCHK(hipStreamSynchronize(stream));
+        return FCC_OK;
+    }
+
+    // ---------------------------------------------------------- transfers
+    template <typename U>
+    int upload(U *dst, const std::vector<U> &src) {
+        HIPCHK(hipMemcpyAsync(dst, src.data(), sizeof(U) * src.size(), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        return FCC_OK;
+    }
+    template <typename U>
+    int download(std::vector<U> &dst, const U *src, size_t n) {
+        dst.resize(n);
+        HIPCHK(hipMemcpyAsync(dst.data(), src, sizeof(U) * n, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        return FCC_OK;
+    }
+
+    int set_bodies(const fcc_bodies *b) override {
+        std::vector<double> x(b->x, b->x + N), y(b->y, b->y + N);
+        std::vector<int32_t> ix(N, 0), iy(N, 0), ty(N, 0), pa(N, 0);
+        std::vector<T> th(N), r0(N), co((size_t)N * ldc, T(0));
+        for (int64_t i = 0; i < N; i++) {
+            if (b->imgx) ix[i] = b->imgx[i];
+            if (b->imgy) iy[i] = b->imgy[i];
+            ty[i] = (b->type ? b->type[i] : 1) - 1;
+            pa[i] = (b->parent ? b->parent[i] : (int32_t)(i + 1)) - 1;
+            th[i] = (T)b->theta[i];
+            r0[i] = (T)b->R0[i];
+            for (int c = 0; c < nc; c++) co[(size_t)i * ldc + c] = (T)b->coef[(size_t)i * nc + c];
+        }
+        int rc;
+        if ((rc = upload(d.x, x)) || (rc = upload(d.y, y)) || (rc = upload(d.imgx, ix)) || (rc = upload(d.imgy, iy)) ||
+            (rc = upload(d.type, ty)) || (rc = upload(d.parent, pa)) || (rc = upload(d.th, th)) ||
+            (rc = upload(d.R0, r0)) || (rc = upload(d.coef, co)))
+            return rc;
+        if ((rc = launch_init())) return rc;
+        HIPCHK(hipStreamSynchronize(stream));
+        valid = false;
+        return FCC_OK;
+    }
+
+    int get_bodies(fcc_bodies *b) override {
+        std::vector<double> x, y;
+        std::vector<int32_t> ix, iy, ty, pa;
+        std::vector<T> th, r0, co;
+        int rc;
+        if ((rc = download(x, d.x, N)) || (rc = download(y, d.y, N)) || (rc = download(ix, d.imgx, N)) ||
+            (rc = download(iy, d.imgy, N)) || (rc = download(ty, d.type, N)) || (rc = download(pa, d.parent, N)) ||
+            (rc = download(th, d.th, N)) || (rc = download(r0, d.R0, N)) || (rc = download(co, d.coef, (size_t)N * ldc)))
+            return rc;
+        for (int64_t i = 0; i < N; i++) {
+            if (b->x) b->x[i] = x[i];
+            if (b->y) b->y[i] = y[i];
+            if (b->imgx) b->imgx[i] = ix[i];
+            if (b->imgy) b->imgy[i] = iy[i];
+            if (b->type) b->type[i] = ty[i] + 1;
+            if (b->parent) b->parent[i] = pa[i] + 1;
+            if (b->theta) b->theta[i] = (double)th[i];
+            if (b->R0) b->R0[i] = (double)r0[i];
+            if (b->coef)
+                for (int c = 0; c < nc; c++) b->coef[(size_t)i * nc + c] = (double)co[(size_t)i * ldc + c];
+        }
+        return FCC_OK;
+    }
+
+    int set_dt(double dt) override {
+        cfg.dt = dt;
+        p.dt = (T)dt;
+        gen++;  // dt is baked into the captured graph
+        return FCC_OK;
+    }
+
+    int rows_out(double *out, const T *src) {
+        if (!out) return FCC_OK;
+        std::vector<T> h;
+        int rc = download(h, src, (size_t)N * ldc);
+        if (rc) return rc;
+        for (int64_t i = 0; i < N; i++)
+            for (int c = 0; c < nc; c++) out[(size_t)i * nc + c] = (double)h[(size_t)i * ldc + c];
+        return FCC_OK;
+    }
+    int vec_out(double *out, const T *src, size_t n) {
+        if (!out) return FCC_OK;
+        std::vector<T> h;
+        int rc = download(h, src, n);
+        if (rc) return rc;
+        for (size_t i = 0; i < n; i++) out[i] = (double)h[i];
+        return FCC_OK;
+    }
+
+    int get_gradients(double *gx, double *gy, double *gth, double *glab, double *gcoef, double *E) override {
+        int rc;
+        if ((rc = vec_out(gx, d.gx, N)) || (rc = vec_out(gy, d.gy, N)) || (rc = vec_out(gth, d.gth, N)) ||
+            (rc = rows_out(glab, d.glab)) || (rc = rows_out(gcoef, d.gcoef)) || (rc = vec_out(E, d.Ebody, N)))
+            return rc;
+        return FCC_OK;
+    }
+
+    int get_contour_gradients(double *P, double *GR, double *GD, double *GDD) override {
+        if (!debug) return fail(FCC_ERR_STATE, "contour gradients need FCC_FLAG_DEBUG_BUFFERS");
+        const size_t m = (size_t)N * nth;
+        int rc;
+        if ((rc = vec_out(P, d.P, m)) || (rc = vec_out(GR, d.GR, m)) || (rc = vec_out(GD, d.GD, m)) ||
+            (rc = vec_out(GDD, d.GDD, m)))
+            return rc;
+        return FCC_OK;
+    }
+
+    int launch_geometry(T *R, T *dR, T *ddR, T *per, T *area) {
+        const unsigned g = blocks(N, 256);
+        switch (KM) {
+        case 5: hipLaunchKernelGGL((k_geometry<T, 5>), g, 256, 0, stream, p, d, R, dR, ddR, per, area); break;
+        case 7: hipLaunchKernelGGL((k_geometry<T, 7>), g, 256, 0, stream, p, d, R, dR, ddR, per, area); break;
+        default: hipLaunchKernelGGL((k_geometry<T, 15>), g, 256, 0, stream, p, d, R, dR, ddR, per, area);
+        }
+        HIPCHK(hipGetLastError());
+        return FCC_OK;
+    }
+
+    int get_geometry(double *lab, double *rmax, double *R, double *dR, double *ddR) override {
+        int rc;
+        if ((rc = rows_out(lab, d.lab)) || (rc = vec_out(rmax, d.rmax, N))) return rc;
+        if (R || dR || ddR) {
+            const size_t m = (size_t)N * nth;
+            T *tR, *tD, *tDD;
+            if ((rc = alloc(&tR, m)) || (rc = alloc(&tD, m)) || (rc = alloc(&tDD, m))) return rc;
+            if (!(rc = launch_geometry(tR, tD, tDD, nullptr, nullptr))) {
+                if (!(rc = vec_out(R, tR, m)) && !(rc = vec_out(dR, tD, m))) rc = vec_out(ddR, tDD, m);
+            }
+            release(tR);
+            release(tD);
+            release(tDD);
+            return rc;
+        }
+        return FCC_OK;
+    }
+
+    int shape_index(double *per, double *area, double *pidx) override {
+        T *tp, *ta;
+        int rc;
+        if ((rc = alloc(&tp, N)) || (rc = alloc(&ta, N))) return rc;
+        std::vector<T> hp, ha;
+        if (!(rc = launch_geometry(nullptr, nullptr, nullptr, tp, ta)) && !(rc = download(hp, tp, N)))
+            rc = download(ha, ta, N);
+        release(tp);
+        release(ta);
+        if (rc) return rc;
+        for (int64_t i = 0; i < N; i++) {
+            if (per) per[i] = (double)hp[i];
+            if (area) area[i] = (double)ha[i];
+            if (pidx) pidx[i] = (double)(hp[i] / std::sqrt(ha[i]));
+        }
+        return FCC_OK;
+    }
+
+    int get_celllist(int32_t *ox, int32_t *oy, int32_t *cid, int32_t *pm, int32_t *st, int32_t *sp) override {
+        if (ox) *ox = ncx;
+        if (oy) *oy = ncy;
+        std::vector<int32_t> a, b, s, c;
+        int rc;
+        if ((rc = download(a, cellid, N)) || (rc = download(b, perm, N)) || (rc = download(s, cstart, nce)) ||
+            (rc = download(c, ccnt, nce)))
+            return rc;
+        for (int64_t i = 0; i < N; i++) {
+            if (cid) cid[i] = a[i] + 1;
+            if (pm) pm[i] = b[i] + 1;
+        }
+        for (int64_t q = 0; q < nce; q++) {
+            const bool empty = c[q] == 0;
+            if (st) st[q] = empty ? 1 : s[q] + 1;
+            if (sp) sp[q] = empty ? 0 : s[q] + c[q];
+        }
+        return FCC_OK;
+    }
+
+    int get_neighbors(int32_t *cap, int32_t *count, int32_t *idx) override {
+        if (cap) *cap = d.cap;
+        std::vector<int32_t> cn, ni;
+        int rc;
+        if ((rc = download(cn, d.ncount, N))) return rc;
+        if (count) std::copy(cn.begin(), cn.end(), count);
+        if (idx) {
+            if ((rc = download(ni, d.nidx, (size_t)d.cap * N))) return rc;
+            for (int64_t i = 0; i < N; i++)
+                for (int32_t q = 0; q < d.cap; q++)
+                    idx[(size_t)i * d.cap + q] = q < cn[i] ? ni[(size_t)q * N + i] + 1 : 0;
+        }
+        return FCC_OK;
+    }
+
+    int set_neighbor_capacity(int32_t cap) override {
+        if (cap < 1) return fail(FCC_ERR_ARGUMENT, "capacity must be positive");
+        int32_t *ni = nullptr;
+        int rc = alloc(&ni, (size_t)cap * N);
+        if (rc) return rc;
+        HIPCHK(hipMemsetAsync(ni, 0, sizeof(int32_t) * cap * N, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        release(d.nidx);
+        d.nidx = ni;
+        d.cap = cap;
+        valid = false;
+        gen++;
+        return FCC_OK;
+    }
+
+    int time_step_kernels(int64_t steps, double *fms, double *ums) override {
+        hipEvent_t e0, e1, e2;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventCreate(&e2));
+        double sf = 0.0, su = 0.0;
+        int rc = FCC_OK;
+        for (int64_t q = 0; q < steps && !rc; q++) {
+            if (has_contact && (rc = update_neighbors(0))) break;
+            p.skin_check = (has_contact && valid && skin > 0.0) ? 1 : 0;
+            hipError_t e = hipEventRecord(e0, stream);
+            if (e == hipSuccess && !(rc = launch_forces(1))) {
+                e = hipEventRecord(e1, stream);
+                if (e == hipSuccess && !(rc = launch_update())) e = hipEventRecord(e2, stream);
+            }
+            if (!rc && e == hipSuccess) e = hipEventSynchronize(e2);
+            float a = 0.f, b = 0.f;
+            if (!rc && e == hipSuccess) e = hipEventElapsedTime(&a, e0, e1);
+            if (!rc && e == hipSuccess) e = hipEventElapsedTime(&b, e1, e2);
+            if (!rc && e != hipSuccess) rc = fail(FCC_ERR_DEVICE, "event timing: %s", hipGetErrorString(e));
+            sf += a;
+            su += b;
+            t += cfg.dt;
+            nstep++;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipEventDestroy(e2);
+        if (rc) return rc;
+        if (fms) *fms = steps > 0 ? sf / (double)steps : 0.0;
+        if (ums) *ums = steps > 0 ? su / (double)steps : 0.0;
+        return FCC_OK;
+    }
+
+    int get_counters(int64_t *st, double *tt, int64_t *clock, int64_t *b) override {
+        DevStatus s;
+        HIPCHK(hipMemcpyAsync(&s, d.st, sizeof s, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        if (st) *st = nstep;
+        if (tt) *tt = t;
+        if (clock) *clock = s.clock;
+        if (b) *b = builds;
+        return FCC_OK;
+    }
+};
+
+// ------------------------------------------------------------------ C ABI
+
+extern "C" {
+
+int fcc_abi_version(void) { return FCC_ABI_VERSION; }
+const char *fcc_last_error(void) { return g_err; }
+
+int fcc_create(const fcc_config *cfg, const fcc_bodies *bodies, fcc_sim **out) {
+    if (!out) return fail(FCC_ERR_ARGUMENT, "null output handle");
+    *out = nullptr;
+    int rc = validate(cfg, bodies);
+    if (rc) return rc;
+    fcc_sim *s = nullptr;
+    if (cfg->precision == FCC_FP64) {
+        auto *q = new Sim<double>();
+        rc = q->init(cfg, bodies);
+        s = q;
+    } else {
+        auto *q = new Sim<float>();
+        rc = q->init(cfg, bodies);
+        s = q;
+    }
+    if (rc) {
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return FCC_OK;
+}
+
+int fcc_destroy(fcc_sim *s) {
+    delete s;
+    return FCC_OK;
+}
+
+#define FCC_HANDLE(s) \
+    if (!(s)) return fail(FCC_ERR_ARGUMENT, "null simulation handle")
+
+int fcc_set_bodies(fcc_sim *s, const fcc_bodies *b) { FCC_HANDLE(s); return s->set_bodies(b); }
+int fcc_get_bodies(fcc_sim *s, fcc_bodies *b) { FCC_HANDLE(s); return s->get_bodies(b); }
+int fcc_set_dt(fcc_sim *s, double dt) { FCC_HANDLE(s); return s->set_dt(dt); }
+int fcc_step(fcc_sim *s, int64_t n) { FCC_HANDLE(s); return s->step(n); }
+int fcc_run(fcc_sim *s, int64_t n) { FCC_HANDLE(s); return s->run(n); }
+int fcc_update_neighbors(fcc_sim *s, int32_t force) { FCC_HANDLE(s); return s->update_neighbors(force); }
+int fcc_compute_gradients(fcc_sim *s, int32_t frame) { FCC_HANDLE(s); return s->compute_gradients(frame); }
+int fcc_get_gradients(fcc_sim *s, double *gx, double *gy, double *gth, double *glab, double *gcoef, double *E) {
+    FCC_HANDLE(s);
+    return s->get_gradients(gx, gy, gth, glab, gcoef, E);
+}
+int fcc_get_contour_gradients(fcc_sim *s, double *P, double *GR, double *GD, double *GDD) {
+    FCC_HANDLE(s);
+    return s->get_contour_gradients(P, GR, GD, GDD);
+}
+int fcc_get_geometry(fcc_sim *s, double *lab, double *rmax, double *R, double *dR, double *ddR) {
+    FCC_HANDLE(s);
+    return s->get_geometry(lab, rmax, R, dR, ddR);
+}
+int fcc_shape_index(fcc_sim *s, double *per, double *area, double *p) { FCC_HANDLE(s); return s->shape_index(per, area, p); }
+int fcc_get_celllist(fcc_sim *s, int32_t *ncx, int32_t *ncy, int32_t *cellid, int32_t *perm, int32_t *start,
+                     int32_t *stop) {
+    FCC_HANDLE(s);
+    return s->get_celllist(ncx, ncy, cellid, perm, start, stop);
+}
+int fcc_get_neighbors(fcc_sim *s, int32_t *cap, int32_t *count, int32_t *idx) {
+    FCC_HANDLE(s);
+    return s->get_neighbors(cap, count, idx);
+}
+int fcc_set_neighbor_capacity(fcc_sim *s, int32_t cap) { FCC_HANDLE(s); return s->set_neighbor_capacity(cap); }
+int fcc_get_counters(fcc_sim *s, int64_t *step, double *t, int64_t *clock, int64_t *builds) {
+    FCC_HANDLE(s);
+    return s->get_counters(step, t, clock, builds);
+}
+
+int fcc_time_step_kernels(fcc_sim *s, int64_t steps, double *fms, double *ums) {
+    FCC_HANDLE(s);
+    if (steps < 0) return fail(FCC_ERR_ARGUMENT, "steps must be non-negative");
+    return s->time_step_kernels(steps, fms, ums);
+}
+
+int fcc_philox_noise(uint64_t seed, uint64_t draw, int64_t n, int32_t precision, void *out) {
+    if (n < 0 || !out) return fail(FCC_ERR_ARGUMENT, "bad noise arguments");
+    if (precision != FCC_FP64 && precision != FCC_FP32) return fail(FCC_ERR_ARGUMENT, "precision must be FP64 or FP32");
+    if (n == 0) return FCC_OK;
+    const size_t bytes = (size_t)3 * n * (precision == FCC_FP64 ? 8 : 4);
+    void *dv = nullptr;
+    HIPCHK(hipMalloc(&dv, bytes));
+    const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    if (precision == FCC_FP64)
+        hipLaunchKernelGGL((k_noise<double>), blocks(n, 256), 256, 0, 0, n, k0, k1, draw, (double *)dv);
+    else
+        hipLaunchKernelGGL((k_noise<float>), blocks(n, 256), 256, 0, 0, n, k0, k1, draw, (float *)dv);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpy(out, dv, bytes, hipMemcpyDeviceToHost);
+    (void)hipFree(dv);
+    HIPCHK(e);
+    return FCC_OK;
+}
+
+int fcc_philox4x32(const uint32_t *ctr, const uint32_t *key, int64_t n, uint32_t *out) {
+    if (n < 0 || !ctr || !key || !out) return fail(FCC_ERR_ARGUMENT, "bad philox arguments");
+    if (n == 0) return FCC_OK;
+    uint32_t *dc = nullptr, *dk = nullptr, *dout = nullptr;
+    HIPCHK(hipMalloc(&dc, 16 * n));
+    HIPCHK(hipMalloc(&dk, 8 * n));
+    HIPCHK(hipMalloc(&dout, 16 * n));
+    hipError_t e = hipMemcpy(dc, ctr, 16 * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dk, key, 8 * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_philox, blocks(n, 256), 256, 0, 0, n, dc, dk, dout);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(out, dout, 16 * n, hipMemcpyDeviceToHost);
+    (void)hipFree(dc);
+    (void)hipFree(dk);
+    (void)hipFree(dout);
+    HIPCHK(e);
+    return FCC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,247 @@
+// fcc_device.hpp — device-side building blocks of the gfx950 time step:
+// kernel parameter blocks, Philox4x32-10, periodic-box helpers, the built-in
+// body terms and the off-grid contour radius.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/fcc.h"
+
+namespace fcc {
+
+constexpr int MAXB = 4;  // native body terms per simulation
+
+// Device run state (src/utils.jl:50-78): `rebuild` is STATUS_REBUILD, set by
+// the fused skin check; `update` is STATUS_UPDATE, the copy of `rebuild`
+// taken at the start of a step, which gates the update kernel; `overflow`
+// holds the largest neighbour count of the last build; `clock` counts the
+// executed steps and is the Philox draw index.
+struct DevStatus {
+    int32_t rebuild, update, overflow, pad;
+    int64_t clock;
+};
+
+// Scalar parameters of one simulation, passed by value to every kernel.
+template <typename T>
+struct Params {
+    int64_t N;
+    int K, nc, ldc, nth;
+    T dphi;
+    double Lx, Ly;
+    // body terms (Tension, PerimeterElasticity, Bending) with OnTypes masks
+    int nbody;
+    int bkind[MAXB];
+    uint32_t bmask[MAXB];
+    T bp0[MAXB], bp1[MAXB];
+    int need_dd;  // a term uses R'' (Bending)
+    // contact: Σ over Overlap terms of applies(ti)·J, by (related, ti, tj)
+    int ncontact;
+    T Jc[2][FCC_MAX_TYPES][FCC_MAX_TYPES];
+    T range;
+    // body types
+    T v0[FCC_MAX_TYPES], Dr[FCC_MAX_TYPES], Dt[FCC_MAX_TYPES], mur[FCC_MAX_TYPES],
+        muth[FCC_MAX_TYPES], muab[FCC_MAX_TYPES];
+    int pdrag[FCC_MAX_TYPES];
+    T dt;
+    uint32_t key0, key1;
+    T skin;
+    int skin_check;  // fused Verlet skin check in the update kernel
+};
+
+// Device buffers of one simulation. Per-body coefficient rows are stored
+// body-major with stride ldc (AoS) so a neighbour's row is one contiguous
+// read; the neighbour list is slot-major (nidx[q*N + i]).
+template <typename T>
+struct Ptrs {
+    double *x, *y;
+    int32_t *imgx, *imgy;
+    T *th, *coef, *R0;
+    int32_t *type, *parent;
+    T *lab, *rmax;
+    T *glab, *gx, *gy, *gth, *Ebody, *gcoef;
+    T *P, *GR, *GD, *GDD;  // debug (may be null)
+    int32_t *nidx, *ncount;
+    int32_t cap;
+    double *x0, *y0;
+    T *rmax0;
+    int32_t *cidx, *ccount;
+    int32_t ccap;
+    T *disp;
+    const T *S;   // [3][nth][ldc]: C, D, DD rows
+    const T *cs;  // [nth][2]: cos φk, sin φk
+    DevStatus *st;
+};
+
+#define FCC_CONST __attribute__((address_space(4)))
+
+__device__ __forceinline__ void fsincos(double x, double *s, double *c) { sincos(x, s, c); }
+__device__ __forceinline__ void fsincos(float x, float *s, float *c) { sincosf(x, s, c); }
+
+template <typename T>
+__device__ __forceinline__ const FCC_CONST T *as_const(const T *p) {
+    return (const FCC_CONST T *)p;
+}
+
+// ------------------------------------------------------------------ Philox
+
+// Philox4x32-10 (Salmon et al., SC'11), src/random.jl:10-36.
+__device__ __forceinline__ void philox4x32(uint32_t &c0, uint32_t &c1, uint32_t &c2, uint32_t &c3,
+                                           uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+        uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+        uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+// uniform_open + box_muller (src/random.jl:42-51) for T = float / double.
+__device__ __forceinline__ void box_muller(uint32_t a0, uint32_t a1, uint32_t a2, uint32_t a3,
+                                           double &z1, double &z2) {
+    double u1 = ((double)((((uint64_t)a0 << 32) | a1) >> 12) + 0.5) * 0x1p-52;
+    double u2 = ((double)((((uint64_t)a2 << 32) | a3) >> 12) + 0.5) * 0x1p-52;
+    double r = sqrt(-2.0 * log(u1));
+    double s, c;
+    sincos(2.0 * 3.141592653589793 * u2, &s, &c);
+    z1 = r * c;
+    z2 = r * s;
+}
+
+__device__ __forceinline__ void box_muller(uint32_t a0, uint32_t, uint32_t a2, uint32_t,
+                                           float &z1, float &z2) {
+    float u1 = ((float)(a0 >> 9) + 0.5f) * 0x1p-23f;
+    float u2 = ((float)(a2 >> 9) + 0.5f) * 0x1p-23f;
+    float r = sqrtf(-2.0f * logf(u1));
+    float s, c;
+    sincosf(2.0f * 3.14159274f * u2, &s, &c);
+    z1 = r * c;
+    z2 = r * s;
+}
+
+// philox_pair(T, key, i, n, block), src/random.jl:75-78 (i 1-based).
+template <typename T>
+__device__ __forceinline__ void philox_pair(uint32_t k0, uint32_t k1, uint32_t i1, uint64_t n,
+                                            uint32_t block, T &z1, T &z2) {
+    uint32_t c0 = i1, c1 = (uint32_t)n, c2 = (uint32_t)(n >> 32), c3 = block;
+    philox4x32(c0, c1, c2, c3, k0, k1);
+    box_muller(c0, c1, c2, c3, z1, z2);
+}
+
+// --------------------------------------------------------------------- box
+
+// minimum_image (src/box.jl:14), round ties-to-even.
+template <typename T>
+__device__ __forceinline__ T min_image(T d, T L) {
+    return d - L * rint(d / L);
+}
+
+// wrap (src/box.jl:21-30).
+__device__ __forceinline__ double wrap(double x, double L, int32_t &shift) {
+    double s = floor(x / L);
+    double xp = x - s * L;
+    if (xp >= L) {
+        xp = 0.0;
+        s += 1.0;
+    }
+    shift = (int32_t)s;
+    return xp;
+}
+
+// ------------------------------------------------------------- body terms
+
+// contour_integrands of the built-ins (src/energies/body_terms.jl:10,31;
+// Bending test/test_energies.jl:9).
+template <typename T>
+__device__ __forceinline__ T term_q(int kind, T R, T dR, T ddR) {
+    if (kind == FCC_TERM_BENDING) {
+        T s2 = R * R + dR * dR;
+        T num = R * R + T(2) * dR * dR - R * ddR;
+        T is2 = T(1) / s2;
+        return num * num * is2 * is2 / sqrt(s2);
+    }
+    return sqrt(R * R + dR * dR);
+}
+
+// contour_integrand_gradients (src/energies/body_terms.jl:12-15; closed forms
+// of the ForwardDiff derivatives of the other two).
+template <typename T>
+__device__ __forceinline__ void term_J(int kind, T R, T dR, T ddR, T &j0, T &j1, T &j2) {
+    if (kind == FCC_TERM_BENDING) {
+        T s2 = R * R + dR * dR;
+        T num = R * R + T(2) * dR * dR - R * ddR;
+        T is2 = T(1) / s2;
+        T s5 = is2 * is2 / sqrt(s2);
+        T s7 = s5 * is2;
+        j0 = T(2) * num * (T(2) * R - ddR) * s5 - T(5) * num * num * R * s7;
+        j1 = T(2) * num * (T(4) * dR) * s5 - T(5) * num * num * dR * s7;
+        j2 = T(-2) * num * R * s5;
+        return;
+    }
+    T s = sqrt(R * R + dR * dR);
+    j0 = R / s;
+    j1 = dR / s;
+    j2 = T(0);
+}
+
+// energy and energy_gradient (src/energies/body_terms.jl:11,16,32).
+template <typename T>
+__device__ __forceinline__ T term_E(int kind, T p0, T p1, T Q, T r0, T &dF) {
+    if (kind == FCC_TERM_PERIMETER_ELASTICITY) {
+        T x = Q - p1 * T(1.7724538509055159) * r0;
+        dF = p0 * (T(2) * x);
+        return p0 * (x * x);
+    }
+    dF = p0;
+    return p0 * Q;
+}
+
+// Register layout of a coefficient row with at most KM modes: l[0] = a0,
+// l[m] = cosine coefficient of mode m+1, l[KM+m] = sine coefficient (m =
+// 1..K; unused slots are zero). Indices stay compile-time for any runtime K.
+template <typename T, int KM>
+__device__ __forceinline__ void load_row(T (&l)[2 * KM + 1], const T *row, int K) {
+    l[0] = row[0];
+#pragma unroll
+    for (int m = 1; m <= KM; m++) {
+        l[m] = m <= K ? row[m] : T(0);
+        l[KM + m] = m <= K ? row[K + m] : T(0);
+    }
+}
+
+template <typename T, int KM>
+__device__ __forceinline__ void store_row(const T (&l)[2 * KM + 1], T *row, int K) {
+    row[0] = l[0];
+#pragma unroll
+    for (int m = 1; m <= KM; m++) {
+        if (m <= K) {
+            row[m] = l[m];
+            row[K + m] = l[KM + m];
+        }
+    }
+}
+
+// contour_radius_dir without the R0 factor (src/geometry.jl:282-291).
+template <typename T, int KM>
+__device__ __forceinline__ T contour_dir(const T (&l)[2 * KM + 1], int K, T c1, T s1) {
+    T c = c1, s = s1, r = l[0];
+#pragma unroll
+    for (int m = 1; m <= KM; m++) {
+        if (m <= K) {
+            T cn = c * c1 - s * s1;
+            T sn = s * c1 + c * s1;
+            c = cn;
+            s = sn;
+            r += l[m] * c + l[KM + m] * s;
+        }
+    }
+    return r;
+}
+
+}  // namespace fcc

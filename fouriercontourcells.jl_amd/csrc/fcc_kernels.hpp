@@ -1,0 +1,675 @@
+// fcc_kernels.hpp — gfx950 kernels of the time step (VALU path).
+//
+// Layout: one work item per body ("body-parallel"), 64-lane workgroups. The
+// contour R, R', R'' is never stored in HBM: it is synthesised from the
+// body's lab-frame coefficients in registers, node by node, with the basis
+// rows read through the scalar cache (uniform node index), and the pullback
+// to the coefficients is accumulated in registers in the same pass.
+#pragma once
+
+#include "fcc_device.hpp"
+
+namespace fcc {
+
+// Basis row stride on the device: [a0 | cos modes m = 1..15 | sin modes at 16+m].
+constexpr int SB = 32;
+
+template <typename T, int KM>
+__device__ __forceinline__ void synth(const FCC_CONST T *S, int nth, int k, const T (&l)[2 * KM + 1],
+                                      int K, bool need_dd, T r0, T &R, T &dR, T &ddR) {
+    const FCC_CONST T *C = S + (size_t)k * SB;
+    const FCC_CONST T *D = S + (size_t)(nth + k) * SB;
+    const FCC_CONST T *DD = S + (size_t)(2 * nth + k) * SB;
+    T r = l[0], dr = T(0), ddr = T(0);
+#pragma unroll
+    for (int m = 1; m <= KM; m++) {
+        if (m <= K) {
+            r += C[m] * l[m] + C[16 + m] * l[KM + m];
+            dr += D[m] * l[m] + D[16 + m] * l[KM + m];
+            if (need_dd) ddr += DD[m] * l[m] + DD[16 + m] * l[KM + m];
+        }
+    }
+    R = r0 * r;
+    dR = r0 * dr;
+    ddR = r0 * ddr;
+}
+
+// Forces: body terms (src/gradients.jl:70-128), Overlap contact
+// (src/energies/contact.jl:58-82,149-187) and the pullback to the lab frame
+// (src/gradients.jl:374-386), fused. Writes glab (∂E/∂(R0·lab)), gx, gy,
+// gθ and Ebody. With use_status it also performs the status mirror and the
+// clock tick of a step (src/utils.jl:50-78).
+template <typename T, int NTH_, int KM>
+__global__ __launch_bounds__(64) void k_forces(const Params<T> p, const Ptrs<T> d, int use_status,
+                                               int debug) {
+    constexpr int NTHM = NTH_ ? NTH_ : FCC_MAX_THETA;
+    constexpr int NCM = 2 * KM + 1;
+    __shared__ T sR[NTHM][64];
+    __shared__ T sP[NTHM][64];
+    __shared__ T scs[2][NTHM];
+    const int nth = NTH_ ? NTH_ : p.nth;
+    const int K = p.K;
+    const int lane = threadIdx.x;
+    for (int k = lane; k < nth; k += 64) {
+        scs[0][k] = d.cs[2 * k];
+        scs[1][k] = d.cs[2 * k + 1];
+    }
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * 64 + lane;
+    if (use_status) {
+        if (i == 0) {
+            int r = d.st->rebuild;
+            d.st->update = r;
+            if (r == 0) d.st->clock += 1;
+        }
+        if (d.st->rebuild != 0) return;
+    }
+    if (i >= p.N) return;
+    const FCC_CONST T *S = as_const(d.S);
+    T lab[NCM];
+    load_row<T, KM>(lab, d.lab + i * p.ldc, K);
+    const T r0 = d.R0[i];
+    const int ti = d.type[i];
+    const uint32_t tbit = 1u << ti;
+    const bool need_dd = p.need_dd != 0;
+    const bool contact = p.ncontact != 0;
+
+    // pass 1: contour integrals Q of every body term (sum over nodes k = 1..nθ)
+    T Q[MAXB];
+#pragma unroll
+    for (int b = 0; b < MAXB; b++) Q[b] = T(0);
+    for (int k = 0; k < nth; k++) {
+        T R, dR, ddR;
+        synth<T, KM>(S, nth, k, lab, K, need_dd, r0, R, dR, ddR);
+        if (contact) sR[k][lane] = R;
+#pragma unroll
+        for (int b = 0; b < MAXB; b++)
+            if (b < p.nbody) Q[b] += term_q<T>(p.bkind[b], R, dR, ddR);
+    }
+    T w[MAXB], Et[MAXB];
+#pragma unroll
+    for (int b = 0; b < MAXB; b++) {
+        w[b] = T(0);
+        Et[b] = T(0);
+        if (b < p.nbody && (p.bmask[b] & tbit)) {
+            T dF;
+            Et[b] = term_E<T>(p.bkind[b], p.bp0[b], p.bp1[b], Q[b] * p.dphi, r0, dF);
+            w[b] = dF * p.dphi;
+        }
+    }
+    T E = T(0);  // right fold e1 + (e2 + (... + 0)), src/gradients.jl:70-74
+#pragma unroll
+    for (int b = MAXB - 1; b >= 0; b--)
+        if (b < p.nbody) E = Et[b] + E;
+
+    // contact density P at every node, accumulated over the neighbour list in
+    // list order; only the nodes inside the angular window that can reach the
+    // neighbour's bounding disc are tested.
+    if (contact) {
+        for (int k = 0; k < nth; k++) sP[k][lane] = T(0);
+        const double xi = d.x[i], yi = d.y[i];
+        const T rmi = d.rmax[i];
+        const int pi = d.parent[i];
+        const T Lxf = (T)p.Lx, Lyf = (T)p.Ly;
+        int cnt = d.ncount[i];
+        if (cnt > d.cap) cnt = d.cap;
+        for (int q = 0; q < cnt; q++) {
+            const int j = d.nidx[(int64_t)q * p.N + i];
+            const T dxij = (T)min_image(xi - d.x[j], p.Lx);
+            const T dyij = (T)min_image(yi - d.y[j], p.Ly);
+            const T lim = d.rmax[j] + p.range;
+            const T lim2 = lim * lim;
+            const T D2 = dxij * dxij + dyij * dyij;
+            const T reach = (rmi + lim) * T(1.0001) + T(1e-6);
+            if (D2 > reach * reach) continue;
+            int k0 = 0, wlen = nth;
+            const T sd2 = lim2 * T(1.0002) / D2;
+            if (sd2 < T(1)) {
+                const T delta = asin(sqrt(sd2)) + T(2e-3);
+                const T psi = atan2(-dyij, -dxij);
+                const int klo = (int)floor((psi - delta) / p.dphi);
+                const int khi = (int)ceil((psi + delta) / p.dphi);
+                if (khi - klo + 1 < nth) {
+                    wlen = khi - klo + 1;
+                    k0 = klo % nth;
+                    if (k0 < 0) k0 += nth;
+                }
+            }
+            uint64_t mask = 0;
+            int k = k0;
+            for (int t = 0; t < wlen; t++) {
+                const T Rk = sR[k][lane];
+                const T dx = min_image(dxij + Rk * scs[0][k], Lxf);
+                const T dy = min_image(dyij + Rk * scs[1][k], Lyf);
+                if (dx * dx + dy * dy <= lim2) mask |= 1ull << t;
+                if (++k == nth) k = 0;
+            }
+            if (!mask) continue;
+            T lj[NCM];
+            load_row<T, KM>(lj, d.lab + (int64_t)j * p.ldc, K);
+            const T r0j = d.R0[j];
+            const T Jv = p.Jc[pi == d.parent[j] ? 1 : 0][ti][d.type[j]];
+            while (mask) {
+                const int t = __builtin_ctzll(mask);
+                mask &= mask - 1;
+                int kk = k0 + t;
+                if (kk >= nth) kk -= nth;
+                const T Rk = sR[kk][lane];
+                const T dx = min_image(dxij + Rk * scs[0][kk], Lxf);
+                const T dy = min_image(dyij + Rk * scs[1][kk], Lyf);
+                const T dist = sqrt(dx * dx + dy * dy);
+                const T cp = dist > T(0) ? dx / dist : T(1);
+                const T sp = dist > T(0) ? dy / dist : T(0);
+                const T dd = r0j * contour_dir<T, KM>(lj, K, cp, sp) - dist;
+                sP[kk][lane] += dd >= T(0) ? Jv : T(0);
+            }
+        }
+    }
+
+    // pass 2: contour gradients per node, contact force/torque, pullback
+    T gl[NCM];
+#pragma unroll
+    for (int c = 0; c < NCM; c++) gl[c] = T(0);
+    T fx = T(0), fy = T(0), tt = T(0);
+    for (int k = 0; k < nth; k++) {
+        T R, dR, ddR;
+        synth<T, KM>(S, nth, k, lab, K, need_dd, r0, R, dR, ddR);
+        if (contact) R = sR[k][lane];
+        T GR = T(0), GD = T(0), GDD = T(0);
+#pragma unroll
+        for (int b = 0; b < MAXB; b++) {
+            if (b < p.nbody && (p.bmask[b] & tbit)) {
+                T j0, j1, j2;
+                term_J<T>(p.bkind[b], R, dR, ddR, j0, j1, j2);
+                GR += j0 * w[b];
+                GD += j1 * w[b];
+                GDD += j2 * w[b];
+            }
+        }
+        if (contact) {
+            const T g = sP[k][lane];
+            const T pp = g * p.dphi;
+            const T c = scs[0][k], s = scs[1][k];
+            GR += pp * R;
+            fx += pp * (R * c + dR * s);
+            fy += pp * (R * s - dR * c);
+            tt += (-pp) * R * dR;
+            if (debug) d.P[i * nth + k] = g;
+        }
+        const FCC_CONST T *C = S + (size_t)k * SB;
+        const FCC_CONST T *D = S + (size_t)(nth + k) * SB;
+        const FCC_CONST T *DD = S + (size_t)(2 * nth + k) * SB;
+        gl[0] += C[0] * GR;
+#pragma unroll
+        for (int m = 1; m <= KM; m++) {
+            if (m <= K) {
+                T a = C[m] * GR + D[m] * GD;
+                T b = C[16 + m] * GR + D[16 + m] * GD;
+                if (need_dd) {
+                    a += DD[m] * GDD;
+                    b += DD[16 + m] * GDD;
+                }
+                gl[m] += a;
+                gl[KM + m] += b;
+            }
+        }
+        if (debug) {
+            d.GR[i * nth + k] = GR;
+            d.GD[i * nth + k] = GD;
+            d.GDD[i * nth + k] = GDD;
+        }
+    }
+    store_row<T, KM>(gl, d.glab + i * p.ldc, K);
+    d.gx[i] = fx;
+    d.gy[i] = fy;
+    d.gth[i] = tt;
+    d.Ebody[i] = E;
+}
+
+// ----------------------------------------------------------- per-body math
+
+// (cos nθ, sin nθ) for n = 2..K+1: direct in FP64, by rotation from (cos θ,
+// sin θ) in FP32 (geometry.jl:271-273 recurrence).
+template <typename T, int KM>
+__device__ __forceinline__ void harmonics(T th, int K, T (&cn)[KM + 1], T (&sn)[KM + 1]) {
+    if constexpr (sizeof(T) == 8) {
+#pragma unroll
+        for (int m = 1; m <= KM; m++)
+            if (m <= K) fsincos((T)(m + 1) * th, &sn[m], &cn[m]);
+    } else {
+        T c1, s1;
+        sincosf(th, &s1, &c1);
+        T c = c1, s = s1;
+#pragma unroll
+        for (int m = 1; m <= KM; m++) {
+            T c2 = c * c1 - s * s1;
+            T s2 = s * c1 + c * s1;
+            c = c2;
+            s = s2;
+            cn[m] = c;
+            sn[m] = s;
+        }
+    }
+}
+
+// enforce_shape! (src/dynamics.jl:174-194) on a register row, in U precision.
+template <typename U, typename T, int KM>
+__device__ __forceinline__ void enforce_shape(T (&c)[2 * KM + 1], int K) {
+    U sm = U(0);
+#pragma unroll
+    for (int m = 1; m <= KM; m++)
+        if (m <= K) sm += sqrt((U)c[m] * (U)c[m] + (U)c[KM + m] * (U)c[KM + m]);
+    const U bound = U(0.816496580927726);
+    U f = U(1);
+    const bool scale = sm > bound;
+    if (scale) f = bound / sm;
+    U q = U(0);
+#pragma unroll
+    for (int m = 1; m <= KM; m++) {
+        if (m <= K) {
+            U a = (U)c[m], b = (U)c[KM + m];
+            if (scale) {
+                a *= f;
+                b *= f;
+                c[m] = (T)a;
+                c[KM + m] = (T)b;
+            }
+            q += a * a + b * b;
+        }
+    }
+    c[0] = (T)sqrt(U(1) - q / U(2));
+}
+
+// lab_frame! (src/geometry.jl:208-227) on registers.
+template <typename T, int KM>
+__device__ __forceinline__ T lab_frame(const T (&c)[2 * KM + 1], T th, T r0, int K, T (&l)[2 * KM + 1]) {
+    T cn[KM + 1], sn[KM + 1];
+    harmonics<T, KM>(th, K, cn, sn);
+    l[0] = c[0];
+    T bound = c[0];
+#pragma unroll
+    for (int m = 1; m <= KM; m++) {
+        l[m] = T(0);
+        l[KM + m] = T(0);
+        if (m <= K) {
+            T a = c[m], b = c[KM + m];
+            l[m] = a * cn[m] - b * sn[m];
+            l[KM + m] = a * sn[m] + b * cn[m];
+            bound += sqrt(a * a + b * b);
+        }
+    }
+    return r0 * bound;
+}
+
+// Construction / refresh: enforce_shape! then lab_frame! (src/simulation.jl:102-103).
+template <typename T, int KM, bool F64S>
+__global__ __launch_bounds__(256) void k_init(const Params<T> p, const Ptrs<T> d) {
+    constexpr int NCM = 2 * KM + 1;
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= p.N) return;
+    T c[NCM], l[NCM];
+    load_row<T, KM>(c, d.coef + i * p.ldc, p.K);
+    if constexpr (F64S)
+        enforce_shape<double, T, KM>(c, p.K);
+    else
+        enforce_shape<T, T, KM>(c, p.K);
+    store_row<T, KM>(c, d.coef + i * p.ldc, p.K);
+    d.rmax[i] = lab_frame<T, KM>(c, d.th[i], d.R0[i], p.K, l);
+    store_row<T, KM>(l, d.lab + i * p.ldc, p.K);
+}
+
+// body_frame! (src/gradients.jl:347-362): gcoef from glab at the current state.
+template <typename T, int KM>
+__device__ __forceinline__ void body_frame(const T (&gl)[2 * KM + 1], const T (&c)[2 * KM + 1], T th, T r0,
+                                           int K, T (&gc)[2 * KM + 1]) {
+    T cn[KM + 1], sn[KM + 1];
+    harmonics<T, KM>(th, K, cn, sn);
+    const T ga0 = r0 * gl[0];
+    const T a0 = c[0];
+    gc[0] = ga0;
+#pragma unroll
+    for (int m = 1; m <= KM; m++) {
+        gc[m] = T(0);
+        gc[KM + m] = T(0);
+        if (m <= K) {
+            T ga = r0 * gl[m], gb = r0 * gl[KM + m];
+            T a = c[m], b = c[KM + m];
+            gc[m] = ga * cn[m] + gb * sn[m] - ga0 * a / (T(2) * a0);
+            gc[KM + m] = (-ga) * sn[m] + gb * cn[m] - ga0 * b / (T(2) * a0);
+        }
+    }
+}
+
+template <typename T, int KM>
+__global__ __launch_bounds__(256) void k_body_frame(const Params<T> p, const Ptrs<T> d) {
+    constexpr int NCM = 2 * KM + 1;
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= p.N) return;
+    T gl[NCM], c[NCM], gc[NCM];
+    load_row<T, KM>(gl, d.glab + i * p.ldc, p.K);
+    load_row<T, KM>(c, d.coef + i * p.ldc, p.K);
+    body_frame<T, KM>(gl, c, d.th[i], d.R0[i], p.K, gc);
+    store_row<T, KM>(gc, d.gcoef + i * p.ldc, p.K);
+}
+
+// substrate_displacement (src/dynamics.jl:84-99) of body c.
+template <typename T>
+__device__ __forceinline__ void substrate_disp(const Params<T> &p, const Ptrs<T> &d, int64_t c,
+                                               uint64_t draw, T &ux, T &uy) {
+    const int tc = d.type[c];
+    T fx = d.gx[c], fy = d.gy[c];
+    const int nch = d.ccount[c];
+    for (int q = 0; q < nch; q++) {
+        const int k = d.cidx[(int64_t)q * p.N + c];
+        if (p.pdrag[d.type[k]]) {
+            fx += d.gx[k];
+            fy += d.gy[k];
+        }
+    }
+    T s, co;
+    fsincos(d.th[c], &s, &co);
+    const T st = sqrt(T(2) * p.Dt[tc] * p.dt);
+    T x1 = T(0), x2 = T(0);
+    if (p.Dt[tc] != T(0)) philox_pair<T>(p.key0, p.key1, (uint32_t)(c + 1), draw, 0, x1, x2);
+    ux = p.dt * (p.v0[tc] * co - p.mur[tc] * fx) + st * x1;
+    uy = p.dt * (p.v0[tc] * s - p.mur[tc] * fy) + st * x2;
+}
+
+// displacement_kernel! (src/dynamics.jl:258-277), used when some body type
+// has parent drag: a child needs its parent's θ before the update.
+template <typename T>
+__global__ __launch_bounds__(256) void k_displacement(const Params<T> p, const Ptrs<T> d) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= p.N || d.st->update != 0) return;
+    const uint64_t draw = (uint64_t)(d.st->clock - 1);
+    const int ti = d.type[i];
+    T d0, d1;
+    if (p.pdrag[ti]) {
+        T ux, uy;
+        substrate_disp<T>(p, d, d.parent[i], draw, ux, uy);
+        T s, co;
+        fsincos(d.th[i], &s, &co);
+        const T st = sqrt(T(2) * p.Dt[ti] * p.dt);
+        T x1 = T(0), x2 = T(0);
+        if (p.Dt[ti] != T(0)) philox_pair<T>(p.key0, p.key1, (uint32_t)(i + 1), draw, 0, x1, x2);
+        d0 = ux + p.dt * (p.v0[ti] * co - p.mur[ti] * d.gx[i]) + st * x1;
+        d1 = uy + p.dt * (p.v0[ti] * s - p.mur[ti] * d.gy[i]) + st * x2;
+    } else {
+        substrate_disp<T>(p, d, i, draw, d0, d1);
+    }
+    T x3 = T(0), x4;
+    if (p.Dr[ti] != T(0)) philox_pair<T>(p.key0, p.key1, (uint32_t)(i + 1), draw, 1, x3, x4);
+    d.disp[3 * i] = d0;
+    d.disp[3 * i + 1] = d1;
+    d.disp[3 * i + 2] = sqrt(T(2) * p.Dr[ti] * p.dt) * x3;
+}
+
+// Fused Euler–Maruyama update (src/dynamics.jl:310-333) with the
+// displacement and Philox noise of src/dynamics.jl:84-99,258-277 (bodies
+// without parent drag), the body-frame gradient, the shape constraint (FP64
+// when F64S), position wrapping in FP64, the lab frame of the new state and
+// the Verlet skin check (src/neighbors.jl:409-416).
+template <typename T, int KM, bool F64S>
+__global__ __launch_bounds__(256) void k_update(const Params<T> p, const Ptrs<T> d, int use_disp, int debug) {
+    constexpr int NCM = 2 * KM + 1;
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= p.N || d.st->update != 0) return;
+    const uint64_t draw = (uint64_t)(d.st->clock - 1);
+    const int K = p.K;
+    const int ti = d.type[i];
+    const T th = d.th[i], r0 = d.R0[i];
+    T d0, d1, d2;
+    if (use_disp) {
+        d0 = d.disp[3 * i];
+        d1 = d.disp[3 * i + 1];
+        d2 = d.disp[3 * i + 2];
+    } else {
+        T s, co;
+        fsincos(th, &s, &co);
+        T x1 = T(0), x2 = T(0), x3 = T(0), x4;
+        if (p.Dt[ti] != T(0)) philox_pair<T>(p.key0, p.key1, (uint32_t)(i + 1), draw, 0, x1, x2);
+        if (p.Dr[ti] != T(0)) philox_pair<T>(p.key0, p.key1, (uint32_t)(i + 1), draw, 1, x3, x4);
+        const T st = sqrt(T(2) * p.Dt[ti] * p.dt);
+        d0 = p.dt * (p.v0[ti] * co - p.mur[ti] * d.gx[i]) + st * x1;
+        d1 = p.dt * (p.v0[ti] * s - p.mur[ti] * d.gy[i]) + st * x2;
+        d2 = sqrt(T(2) * p.Dr[ti] * p.dt) * x3;
+    }
+    T gl[NCM], c[NCM], gc[NCM], l[NCM];
+    load_row<T, KM>(gl, d.glab + i * p.ldc, K);
+    load_row<T, KM>(c, d.coef + i * p.ldc, K);
+    body_frame<T, KM>(gl, c, th, r0, K, gc);
+    const T thn = th - p.dt * p.muth[ti] * d.gth[i] + d2;
+    const T h = p.dt * p.muab[ti];
+#pragma unroll
+    for (int m = 1; m <= KM; m++) {
+        c[m] -= h * gc[m];
+        c[KM + m] -= h * gc[KM + m];
+    }
+    if constexpr (F64S)
+        enforce_shape<double, T, KM>(c, K);
+    else
+        enforce_shape<T, T, KM>(c, K);
+    int32_t sx, sy;
+    const double xw = wrap(d.x[i] + (double)d0, p.Lx, sx);
+    const double yw = wrap(d.y[i] + (double)d1, p.Ly, sy);
+    const T rm = lab_frame<T, KM>(c, thn, r0, K, l);
+    d.th[i] = thn;
+    store_row<T, KM>(c, d.coef + i * p.ldc, K);
+    d.x[i] = xw;
+    d.y[i] = yw;
+    d.imgx[i] += sx;
+    d.imgy[i] += sy;
+    store_row<T, KM>(l, d.lab + i * p.ldc, K);
+    d.rmax[i] = rm;
+    if (debug) store_row<T, KM>(gc, d.gcoef + i * p.ldc, K);
+    if (p.skin_check) {
+        const double dx = min_image(xw - d.x0[i], p.Lx);
+        const double dy = min_image(yw - d.y0[i], p.Ly);
+        const double dr = (double)rm - (double)d.rmax0[i];
+        const double delta = sqrt(dx * dx + dy * dy) + (dr > 0.0 ? dr : 0.0);
+        if (2.0 * delta > (double)p.skin)
+            __hip_atomic_store(&d.st->rebuild, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// ------------------------------------------------------- cell / neighbours
+// No FMA contraction in this section: the decisions (cell index, distance
+// test) must equal the reference's IEEE evaluation bit for bit.
+
+// cell_index_kernel! (src/neighbors.jl:41-55) plus the per-cell histogram of
+// the counting sort.
+__global__ __launch_bounds__(256) void k_cell_index(int64_t N, const double *x, const double *y,
+                                                    int32_t ncx, int32_t ncy, double wx, double wy,
+                                                    int32_t *cellid, int32_t *count) {
+#pragma clang fp contract(off)
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= N) return;
+    int32_t cx = (int32_t)(x[i] / wx), cy = (int32_t)(y[i] / wy);
+    cx = cx < ncx - 1 ? cx : ncx - 1;
+    cy = cy < ncy - 1 ? cy : ncy - 1;
+    cx = cx > 0 ? cx : 0;  // bodies outside [0, L): clamp (no-op inside the box)
+    cy = cy > 0 ? cy : 0;
+    const int32_t c = cx + ncx * cy;
+    cellid[i] = c;
+    atomicAdd(&count[c], 1);
+}
+
+// Exclusive scan of the cell counts (one workgroup of 1024): start[c].
+__global__ __launch_bounds__(1024) void k_scan(int64_t n, const int32_t *count, int32_t *start,
+                                               int32_t *fill) {
+    __shared__ int32_t part[1024];
+    const int t = threadIdx.x;
+    const int64_t per = (n + 1023) / 1024;
+    const int64_t a = t * per, b = a + per < n ? a + per : n;
+    int32_t s = 0;
+    for (int64_t c = a; c < b; c++) s += count[c];
+    part[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        int32_t v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int32_t acc = part[t] - s;
+    for (int64_t c = a; c < b; c++) {
+        start[c] = acc;
+        fill[c] = acc;
+        acc += count[c];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_scatter(int64_t N, const int32_t *cellid, int32_t *fill,
+                                                 int32_t *perm) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= N) return;
+    perm[atomicAdd(&fill[cellid[i]], 1)] = (int32_t)i;
+}
+
+// Order each cell's bodies by index: the counting sort becomes the stable
+// sortperm! of src/neighbors.jl:86.
+__global__ __launch_bounds__(256) void k_cell_sort(int64_t nce, const int32_t *start, const int32_t *count,
+                                                   int32_t *perm) {
+    const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (c >= nce) return;
+    int32_t *seg = perm + start[c];
+    const int n = count[c];
+    for (int a = 1; a < n; a++) {
+        int32_t v = seg[a];
+        int b = a - 1;
+        while (b >= 0 && seg[b] > v) {
+            seg[b + 1] = seg[b];
+            b--;
+        }
+        seg[b + 1] = v;
+    }
+}
+
+// neighbor_kernel! (src/neighbors.jl:148-174): 3×3 cells, oy outer, ox
+// inner, cell members in sorted order; entries beyond `cap` are counted only.
+// The largest count goes to st->overflow.
+template <typename T>
+__global__ __launch_bounds__(256) void k_neighbors(int64_t N, const double *x, const double *y, const T *rmax,
+                                                   const int32_t *perm, const int32_t *start,
+                                                   const int32_t *count, double Lx, double Ly, int32_t ncx,
+                                                   int32_t ncy, double wx, double wy, double range,
+                                                   int32_t cap, int32_t *nidx, int32_t *ncount,
+                                                   DevStatus *st) {
+#pragma clang fp contract(off)
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= N) return;
+    const double xi = x[i], yi = y[i], ri = (double)rmax[i] + range;
+    int32_t cx = (int32_t)(xi / wx), cy = (int32_t)(yi / wy);
+    cx = cx < ncx - 1 ? cx : ncx - 1;
+    cy = cy < ncy - 1 ? cy : ncy - 1;
+    cx = cx > 0 ? cx : 0;  // bodies outside [0, L): clamp (no-op inside the box)
+    cy = cy > 0 ? cy : 0;
+    const int32_t rx = ncx >= 3 ? 1 : 0, ry = ncy >= 3 ? 1 : 0;
+    int32_t n = 0;
+    for (int32_t oy = -ry; oy <= ry; oy++) {
+        int32_t yy = (cy + oy) % ncy;
+        yy = yy < 0 ? yy + ncy : yy;
+        for (int32_t ox = -rx; ox <= rx; ox++) {
+            int32_t xx = (cx + ox) % ncx;
+            xx = xx < 0 ? xx + ncx : xx;
+            const int32_t cell = xx + ncx * yy;
+            const int32_t s0 = start[cell], s1 = s0 + count[cell];
+            for (int32_t q = s0; q < s1; q++) {
+                const int32_t j = perm[q];
+                if (j == i) continue;
+                double dx = xi - x[j];
+                dx = dx - Lx * rint(dx / Lx);
+                double dy = yi - y[j];
+                dy = dy - Ly * rint(dy / Ly);
+                const double lim = ri + (double)rmax[j];
+                if (dx * dx + dy * dy <= lim * lim) {
+                    n++;
+                    if (n <= cap) nidx[(int64_t)(n - 1) * N + i] = j;
+                }
+            }
+        }
+    }
+    ncount[i] = n;
+    atomicMax(&st->overflow, n);
+}
+
+__global__ __launch_bounds__(256) void k_fill_i32(int64_t n, int32_t *a, int32_t v) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+__global__ void k_clear_status(DevStatus *st) {
+    st->rebuild = 0;
+    st->update = 0;
+    st->overflow = 0;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_copy_skin_ref(int64_t N, const double *x, const double *y,
+                                                       const T *rmax, double *x0, double *y0, T *rmax0) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= N) return;
+    x0[i] = x[i];
+    y0[i] = y[i];
+    rmax0[i] = rmax[i];
+}
+
+// ------------------------------------------------------------ noise / misc
+
+// noise_kernel! (src/random.jl:113-121): column i = the first three normals
+// of philox_normals(T, key, i, draw).
+template <typename T>
+__global__ __launch_bounds__(256) void k_noise(int64_t n, uint32_t k0, uint32_t k1, uint64_t draw, T *out) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    T z1, z2, z3, z4;
+    philox_pair<T>(k0, k1, (uint32_t)(i + 1), draw, 0, z1, z2);
+    philox_pair<T>(k0, k1, (uint32_t)(i + 1), draw, 1, z3, z4);
+    out[3 * i] = z1;
+    out[3 * i + 1] = z2;
+    out[3 * i + 2] = z3;
+}
+
+__global__ __launch_bounds__(256) void k_philox(int64_t n, const uint32_t *ctr, const uint32_t *key,
+                                                uint32_t *out) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    uint32_t c0 = ctr[4 * i], c1 = ctr[4 * i + 1], c2 = ctr[4 * i + 2], c3 = ctr[4 * i + 3];
+    philox4x32(c0, c1, c2, c3, key[2 * i], key[2 * i + 1]);
+    out[4 * i] = c0;
+    out[4 * i + 1] = c1;
+    out[4 * i + 2] = c2;
+    out[4 * i + 3] = c3;
+}
+
+// Contours R, R', R'' at the nodes (nθ x N, column-major) and the perimeter
+// and area sums of src/observables.jl:6-14.
+template <typename T, int KM>
+__global__ __launch_bounds__(256) void k_geometry(const Params<T> p, const Ptrs<T> d, T *R, T *dR, T *ddR,
+                                                  T *per, T *area) {
+    constexpr int NCM = 2 * KM + 1;
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= p.N) return;
+    const FCC_CONST T *S = as_const(d.S);
+    T l[NCM];
+    load_row<T, KM>(l, d.lab + i * p.ldc, p.K);
+    const T r0 = d.R0[i];
+    T ps = T(0), as = T(0);
+    for (int k = 0; k < p.nth; k++) {
+        T r, dr, ddr;
+        synth<T, KM>(S, p.nth, k, l, p.K, true, r0, r, dr, ddr);
+        if (R) {
+            R[i * p.nth + k] = r;
+            dR[i * p.nth + k] = dr;
+            ddR[i * p.nth + k] = ddr;
+        }
+        ps += sqrt(r * r + dr * dr);
+        as += r * r;
+    }
+    if (per) {
+        per[i] = ps * p.dphi;
+        area[i] = as * (p.dphi / T(2));
+    }
+}
+
+}  // namespace fcc

@@ -133,6 +133,7 @@ _DEVICE_ONLY = {
     "philox4x32": [ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32), _i64,
                    ctypes.POINTER(ctypes.c_uint32)],
     "abi_version": [],
+    "time_step_kernels": [_vp, _i64, _dp, _dp],
 }
 
 # Every symbol include/fcc.h declares.
@@ -183,5 +184,13 @@ def hip_engine():
             raise RuntimeError(
                 f"libfcc_hip.so not found at {path}: build it first "
                 "(python -c 'import __graft_entry__ as g; g.build()')")
+        # One HIP runtime per process: PyTorch ships its own libamdhip64.so.7
+        # (same soname as /opt/rocm's), and its stack only initialises on that
+        # copy. Loading torch first makes libfcc_hip.so bind to it as well, so
+        # torch and this library can share the process in any import order.
+        try:
+            import torch  # noqa: F401
+        except Exception:
+            pass
         _ENGINE = Engine(ctypes.CDLL(path), "fcc_", "hip")
     return _ENGINE

@@ -1,0 +1,28 @@
+/*
+ * fcc_bench.h — measurement entry point of libfcc_hip.so.
+ *
+ * No counterpart in the reference (whose timings are wall-clock around run!,
+ * examples/benchmark.jl:18-24, plus CUDA.@profile): it times the kernels of
+ * eager steps with HIP events recorded on the simulation's own stream.
+ */
+#ifndef FCC_BENCH_H
+#define FCC_BENCH_H
+
+#include "fcc.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Run `steps` eager steps (advancing the simulation like fcc_step) and
+ * return the average device time of the forces kernel and of the update
+ * kernel(s) of a step, in milliseconds, from hipEventElapsedTime. Steps that
+ * rebuild the neighbour list are included; the rebuild itself is not timed. */
+int fcc_time_step_kernels(fcc_sim *sim, int64_t steps, double *forces_ms,
+                          double *update_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FCC_BENCH_H */

@@ -567,6 +567,8 @@ static void celllist_build(int64_t N, const double *x, const double *y, int32_t 
         int32_t cx = (int32_t)(x[i] / wx), cy = (int32_t)(y[i] / wy);
         if (cx > ncx - 1) cx = ncx - 1;
         if (cy > ncy - 1) cy = ncy - 1;
+        if (cx < 0) cx = 0;
+        if (cy < 0) cy = 0;
         cellid[i] = cx + ncx * cy;
     }
     for (int64_t c = 0; c < nce; c++) scratch[c] = 0;
@@ -601,6 +603,8 @@ static int32_t neighbor_build(int64_t N, const double *x, const double *y, const
         int32_t cx = (int32_t)(xi / wx), cy = (int32_t)(yi / wy);
         if (cx > ncx - 1) cx = ncx - 1;
         if (cy > ncy - 1) cy = ncy - 1;
+        if (cx < 0) cx = 0;
+        if (cy < 0) cy = 0;
         int32_t rx = ncx >= 3 ? 1 : 0, ry = ncy >= 3 ? 1 : 0;
         int32_t n = 0;
         for (int32_t oy = -ry; oy <= ry; oy++)

@@ -1,0 +1,78 @@
+"""The C-ABI library loads and exports every symbol include/fcc.h declares
+(no device calls), and the host mirror validates like the reference."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import fcc_hip as F
+from fcc_hip import _abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    text = ""
+    for h in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if h.endswith(".h"):
+            text += open(os.path.join(ROOT, "include", h)).read()
+    return sorted(set(re.findall(r"\b(fcc_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_symbols_match_bindings():
+    assert header_symbols() == _abi.HEADER_SYMBOLS
+
+
+def test_library_exports_header_symbols():
+    path = _abi.library_path()
+    assert os.path.exists(path), "libfcc_hip.so not built"
+    lib = ctypes.CDLL(path)
+    for sym in header_symbols():
+        assert hasattr(lib, sym), sym
+    lib.fcc_abi_version.restype = ctypes.c_int
+    assert lib.fcc_abi_version() == 1
+
+
+def test_struct_layout():
+    # sizes of the C structs as compiled by the C compiler (oracle build)
+    assert ctypes.sizeof(_abi.FccBodyType) == 56
+    assert ctypes.sizeof(_abi.FccTerm) == 8 + 16 + 16 + 2 * 8 * 16
+    assert ctypes.sizeof(_abi.FccBodies) == 9 * 8
+
+
+def test_host_validation_errors():
+    with pytest.raises(F.ArgumentError):
+        F.BodyType(drag="cytoplasm")
+    with pytest.raises(F.ArgumentError):
+        F.hexagonal_monolayer(4, 3)
+    with pytest.raises(F.DimensionMismatch):
+        F.Bodies([0.0, 1.0], [0.0], [0.0, 0.0], np.zeros((11, 2)), [1.0, 1.0])
+    with pytest.raises(F.DimensionMismatch):
+        F.Overlap([[1.0, 2.0]]).tables()
+
+
+def test_hexagonal_monolayer_geometry():
+    # src/initial.jl:16-24: box 2r nx × √3 r ny with N π R0² = packing Lx Ly
+    b, box = F.hexagonal_monolayer(32, 32, packing=0.95)
+    assert len(b) == 1024
+    assert abs(1024 * np.pi / (box.Lx * box.Ly) - 0.95) < 1e-12
+    assert abs(box.Lx / box.Ly - 2 / np.sqrt(3)) < 1e-12
+    assert np.all((b.x >= 0) & (b.x < box.Lx) & (b.y >= 0) & (b.y < box.Ly))
+    assert np.all((b.theta > 0) & (b.theta < 2 * np.pi))
+
+
+def test_with_nuclei_and_vcat():
+    # test/test_parents.jl:252-271
+    coef = np.zeros((11, 2))
+    coef[0] = 1
+    a = F.Bodies([1.0, 2.0], [1.0, 2.0], np.zeros(2), coef, np.ones(2), parent=[1, 1])
+    b = F.Bodies([3.0, 4.0], [3.0, 4.0], np.zeros(2), coef, np.ones(2), type=2, parent=[2, 2])
+    c = F.vcat(a, b)
+    assert list(c.parent) == [1, 1, 4, 4] and list(c.type) == [1, 1, 2, 2]
+    cells, box = F.hexagonal_monolayer(4, 4)
+    nb = F.with_nuclei(cells, ratio=0.4)
+    assert len(nb) == 32
+    assert list(nb.parent) == list(range(1, 17)) * 2
+    np.testing.assert_allclose(nb.R0[16:], 0.4 * cells.R0)

@@ -1,0 +1,213 @@
+"""Device path (libfcc_hip.so through the C ABI) vs the FP64 oracle:
+bit-exact Philox words and cell / neighbour lists, FP64 gradients and
+trajectories to the tolerances of test/test_cuda.jl, the N = 64 fixture of the
+original code, and graph mode == eager bitwise."""
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import fcc_hip as F
+import oracle
+from helpers import read_reference
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "reference_N64.txt")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def test_philox_words_bit_exact(hip):
+    say("philox words")
+    import ctypes
+    rng = np.random.default_rng(0)
+    n = 4096
+    ctr = rng.integers(0, 2 ** 32, size=(n, 4), dtype=np.uint64).astype(np.uint32)
+    key = rng.integers(0, 2 ** 32, size=(n, 2), dtype=np.uint64).astype(np.uint32)
+    ctr[:3] = [[0, 0, 0, 0], [0xFFFFFFFF] * 4, [0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344]]
+    key[:3] = [[0, 0], [0xFFFFFFFF] * 2, [0xA4093822, 0x299F31D0]]
+    out = np.zeros((n, 4), dtype=np.uint32)
+    u32 = lambda a: np.ascontiguousarray(a).ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+    assert hip.philox4x32(u32(ctr), u32(key), n, u32(out)) == 0
+    assert tuple(out[0]) == (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)
+    assert tuple(out[1]) == (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)
+    assert tuple(out[2]) == (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)
+    for q in range(3, 200):
+        assert tuple(out[q]) == oracle.philox4x32(tuple(ctr[q]), tuple(key[q]))
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_noise_matches_oracle(hip, oracle_engine, prec):
+    # test/test_cuda.jl:22-29: rtol 4 eps(T)
+    say("noise", prec)
+    g = F.philox_noise(100_000, 9, 12, prec, engine=hip)
+    c = F.philox_noise(100_000, 9, 12, prec, engine=oracle_engine)
+    eps = np.finfo(np.float64 if prec == "fp64" else np.float32).eps
+    np.testing.assert_allclose(g, c, rtol=4 * eps, atol=4 * eps)
+    big = F.philox_noise(3, 9, 12 + (1 << 32), prec, engine=hip)
+    assert not np.array_equal(big, g[:, :3])
+
+
+def jittered(nx, ny, packing=0.98, seed=3, amp=0.4, modes=6):
+    b, box = F.hexagonal_monolayer(nx, ny, packing=packing, seed=2, maxmode=modes)
+    rng = np.random.default_rng(seed)
+    N = len(b)
+    b.x = np.mod(b.x + amp * rng.standard_normal(N), box.Lx)
+    b.y = np.mod(b.y + amp * rng.standard_normal(N), box.Ly)
+    b.coef[1:, :] = 0.05 * rng.standard_normal((b.coef.shape[0] - 1, N))
+    return b, box
+
+
+@pytest.mark.parametrize("size,prec", [((8, 8), "fp64"), ((32, 32), "fp64"), ((32, 32), "fp32"),
+                                       ((512, 512), "fp32")])
+def test_lists_bit_exact(size, prec):
+    say("lists", size, prec)
+    modes = 8 if size[0] >= 512 else 6
+    b, box = jittered(*size, modes=modes)
+    sim = F.Simulation(b, box, types=(F.BodyType(),), energies=(F.Overlap(1.0),), precision=prec)
+    F.update_neighbors(sim, force=True)
+    bod, geo = sim.bodies, sim.geom
+    cutoff = 2 * 1.0 * (1 + math.sqrt(2 / 3)) + 0.3
+    ref = oracle.build_lists(bod.x, bod.y, geo.rmax, box.Lx, box.Ly, cutoff, 0.3,
+                             capacity=sim.neighbors.capacity)
+    cl, nl = sim.cells, sim.neighbors
+    assert (cl.ncx, cl.ncy) == (ref["ncx"], ref["ncy"])
+    for k in ("cellid", "perm", "start", "stop"):
+        assert np.array_equal(getattr(cl, k), ref[k]), k
+    assert np.array_equal(nl.count, ref["count"])
+    assert np.array_equal(nl.idx, ref["idx"])
+
+
+def layout_bodies(nx=5, ny=4):
+    # test/test_layout.jl:21-29 (nucleated, deformed), numpy generators
+    cells, box = F.hexagonal_monolayer(nx, ny, packing=0.97, seed=2)
+    b = F.with_nuclei(cells)
+    rng = np.random.default_rng(5)
+    b.coef[1:, :] = 0.04 * rng.standard_normal((b.coef.shape[0] - 1, len(b)))
+    return b, box
+
+
+LAYOUT_TYPES = (F.BodyType(v0=0.2, Dr=0.1), F.BodyType(Dr=0.1, drag="parent"))
+LAYOUT_ENERGIES = (F.Tension(0.1), F.OnTypes(F.PerimeterElasticity(0.2, 3.7), 2),
+                   F.OnTypes(F.Bending(0.01), 2),
+                   F.Overlap([[1.0, 1.0], [1.0, 1.0]], related=[[1.0, -2.0], [-2.0, 1.0]]))
+
+
+def layout_sim(engine, ntheta=40, precision="fp64", v0=0.2, skin=0.3, seed=7):
+    b, box = layout_bodies()
+    types = (F.BodyType(v0=v0, Dr=0.1), F.BodyType(Dr=0.1, drag="parent"))
+    return F.Simulation(b, box, types=types, energies=LAYOUT_ENERGIES, ntheta=ntheta, precision=precision,
+                        seed=seed, skin=skin, debug=True, engine=engine)
+
+
+@pytest.mark.parametrize("ntheta", [40, 37])
+def test_gradients_match_oracle_fp64(hip, oracle_engine, ntheta):
+    # test/test_cuda.jl:10-20: rtol 1e-10, atol 1e-10
+    say("gradients fp64 ntheta", ntheta)
+    a = F.refresh(layout_sim(oracle_engine, ntheta))
+    b = F.refresh(layout_sim(hip, ntheta))
+    ga, gb = a.grad, b.grad
+    assert np.count_nonzero(ga.P > 0) > 100
+    np.testing.assert_allclose(b.geom.R, a.geom.R, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(b.geom.ddR, a.geom.ddR, rtol=1e-12, atol=1e-12)
+    for f in ("P", "GR", "GD", "GDD", "gx", "gy", "gth", "gcoef", "Ebody"):
+        np.testing.assert_allclose(getattr(gb, f), getattr(ga, f), rtol=1e-10, atol=1e-10, err_msg=f)
+
+
+def test_short_trajectory_matches_oracle_fp64(hip, oracle_engine):
+    # test/test_cuda.jl:31-39: 30 steps, rtol 1e-8, same rebuild count
+    say("trajectory fp64")
+    a, b = layout_sim(oracle_engine), layout_sim(hip)
+    F.step(a, 30)
+    F.step(b, 30)
+    ba, bb = a.bodies, b.bodies
+    np.testing.assert_allclose(bb.x, ba.x, rtol=1e-8)
+    np.testing.assert_allclose(bb.theta, ba.theta, rtol=1e-8)
+    np.testing.assert_allclose(bb.coef, ba.coef, rtol=1e-8, atol=1e-12)
+    assert b.counters().builds == a.counters().builds
+    assert b.counters().clock == a.counters().clock == 30
+
+
+def test_regression_fixture_on_device_fp64(hip):
+    # test/test_dynamics.jl:53-71 on the device, atol 1e-10
+    say("fixture fp64")
+    init, snaps = read_reference(GOLDEN)
+    N = init.shape[1]
+    Ls = math.sqrt(N * math.pi / (0.95 * 2 * math.sqrt(3)))
+    coef = np.zeros((11, N))
+    coef[0] = 1.0
+    b = F.Bodies(init[0], init[1], init[2], coef, np.ones(N))
+    sim = F.Simulation(b, F.PeriodicBox(2 * Ls, math.sqrt(3) * Ls),
+                       types=(F.BodyType(v0=0.06, Dr=0.0, muab=0.1),),
+                       energies=(F.Tension(0.01), F.Overlap(1.0)), dt=0.005)
+    for n in sorted(snaps):
+        F.step(sim, n - sim.step_count)
+        ref = snaps[n]
+        pos = F.unwrapped_positions(sim)
+        bod = sim.bodies
+        np.testing.assert_allclose(pos[0], ref[3], atol=1e-10, rtol=0)
+        np.testing.assert_allclose(pos[1], ref[4], atol=1e-10, rtol=0)
+        np.testing.assert_allclose(bod.theta, ref[2], atol=1e-10, rtol=0)
+        np.testing.assert_allclose(bod.coef[1:], ref[5:15], atol=1e-10, rtol=0)
+
+
+def soft_fluid(precision="fp64", v0=2.0, skin=0.05, seed=1, n=8):
+    b, box = F.hexagonal_monolayer(n, n, packing=0.95)
+    return F.Simulation(b, box, types=(F.BodyType(v0=v0, Dr=0.5, Dt=0.01),),
+                        energies=(F.Tension(0.01), F.Overlap(1.0)), dt=0.005, skin=skin, seed=seed,
+                        precision=precision)
+
+
+def state(s):
+    b = s.bodies
+    g = s.grad
+    c = s.counters()
+    return (b.x, b.y, b.theta, b.coef, b.imgx, g.gx, g.glab), (c.t, c.step, c.builds, c.clock)
+
+
+def same(a, b):
+    sa, ca = state(a)
+    sb, cb = state(b)
+    return all(np.array_equal(u, v) for u, v in zip(sa, sb)) and ca == cb
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_graph_mode_equals_eager(prec):
+    # test/test_cuda.jl:41-93: many rebuilds, frozen chunks, callbacks, dt change
+    say("graph vs eager", prec)
+    a, b = soft_fluid(prec), soft_fluid(prec)
+    F.step(a, 200)
+    F.run(b, 200)
+    assert a.counters().builds > 10
+    assert same(a, b)
+    a, b = soft_fluid(prec), soft_fluid(prec)
+    seen = []
+    F.step(a, 100)
+    F.run(b, 100, every=7, callback=lambda s: seen.append(s.step_count))
+    assert seen == list(range(7, 99, 7)) + [100]
+    assert same(a, b)
+    a.dt = b.dt = 0.004
+    F.step(a, 30)
+    F.run(b, 30)
+    assert same(a, b)
+    for s in (a, b):
+        s.set_neighbor_capacity(1)
+    F.step(a, 30)
+    F.run(b, 30)
+    assert b.neighbors.capacity > 1
+    assert same(a, b)
+
+
+def test_body_terms_only_graph(hip):
+    say("body terms only graph")
+    def mk():
+        b, box = F.hexagonal_monolayer(4, 4)
+        return F.Simulation(b, box, types=(F.BodyType(v0=0.3, Dr=0.5),),
+                            energies=(F.PerimeterElasticity(0.2, 3.7),))
+    assert same(F.step(mk(), 50), F.run(mk(), 50))
