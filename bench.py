@@ -47,19 +47,30 @@ def sync(sim):
 
 
 def timed_run(sim, steps):
+    """Graph-mode run of `steps` steps, timed on the host; the HIP events
+    captured in the step graph give the average forces / update kernel
+    durations over the same region."""
+    f, u = ctypes.c_double(), ctypes.c_double()
     sync(sim)
     t0 = time.perf_counter()
-    F.run(sim, steps)
+    rc = sim.engine.run_timed(sim._h, int(steps), ctypes.byref(f), ctypes.byref(u))
     sync(sim)
-    return time.perf_counter() - t0
-
-
-def kernel_times(sim, steps):
-    f, u = ctypes.c_double(), ctypes.c_double()
-    rc = sim.engine.time_step_kernels(sim._h, int(steps), ctypes.byref(f), ctypes.byref(u))
+    sec = time.perf_counter() - t0
     if rc != 0:
         raise RuntimeError(sim.engine.last_error())
-    return f.value, u.value
+    return sec, f.value, u.value
+
+
+def calibrate_copy(nbytes=1 << 30, reps=5):
+    """A plain device copy of known size (torch), to scale the FETCH_SIZE /
+    WRITE_SIZE counters of a rocprofv3 --pmc run of this same mode."""
+    import torch
+    x = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibrate_copy_bytes": nbytes, "reps": reps}), flush=True)
 
 
 def contact_pairs(sim):
@@ -128,16 +139,24 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the time step has no multi-GPU path in this build "
               "(see DESIGN.md for how it would shard); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
 
     print(f"[bench] workload N={args.n ** 2} precision={args.precision}", file=sys.stderr, flush=True)
     sim = make_sim(args.n, args.precision)
     F.run(sim, args.warmup)
-    sec = timed_run(sim, args.steps)
+    tested0, inr0 = contact_pairs(sim)
+    sec, fms, ums = timed_run(sim, args.steps)
+    tested1, inr1 = contact_pairs(sim)
+    tested, inr = (tested0 + tested1) / 2, (inr0 + inr1) / 2
     c = sim.counters()
     value = sim.N * args.steps / sec
     print(f"[bench] {args.precision}: {value / 1e6:.1f} M body-steps/s ({sec / args.steps * 1e3:.4f} ms/step), "
@@ -147,13 +166,11 @@ def main():
     if not args.no_plain and args.precision != "fp32":
         ps = make_sim(args.n, "fp32")
         F.run(ps, args.warmup)
-        psec = timed_run(ps, args.steps)
+        psec, pfms, pums = timed_run(ps, args.steps)
         plain = ps.N * args.steps / psec
         print(f"[bench] fp32: {plain / 1e6:.1f} M body-steps/s", file=sys.stderr, flush=True)
         ps.close()
 
-    fms, ums = kernel_times(sim, 50)
-    tested, inr = contact_pairs(sim)
     flops = forces_flops(sim, tested, inr)
     achieved = flops / (fms * 1e-3) / 1e12
     upd_bytes = 340 * sim.N  # per body-step, 188 B read + 152 B written

@@ -17,7 +17,7 @@
 #include <string>
 #include <vector>
 
-#include "fcc_kernels.hpp"
+#include "fcc_forces_fast.hpp"
 #include "../../include/fcc_bench.h"
 
 using namespace fcc;
@@ -122,6 +122,7 @@ struct fcc_sim {
     virtual int set_neighbor_capacity(int32_t) = 0;
     virtual int get_counters(int64_t *, double *, int64_t *, int64_t *) = 0;
     virtual int time_step_kernels(int64_t, double *, double *) = 0;
+    virtual int run_timed(int64_t, double *, double *) = 0;
 };
 
 template <typename T>
@@ -139,18 +140,23 @@ struct Sim final : fcc_sim {
     int64_t nce = 1;
     int32_t *cellid = nullptr, *perm = nullptr, *cstart = nullptr, *ccnt = nullptr, *cfill = nullptr;
     bool valid = false;
+    int32_t maxcount = 0;  // largest neighbour count of the last build
     int64_t builds = 0, nstep = 0;
     double t = 0.0;
     std::vector<void *> allocs;
     // graph mode (ext/FourierContourCellsCUDAExt.jl:52-71)
     hipGraphExec_t gexec = nullptr;
     int gen = 0, gexec_gen = -1;
+    bool want_timed = false;
+    hipEvent_t tev[3] = {nullptr, nullptr, nullptr};
     double interval = 16.0;
     int64_t since_rebuild = 0, gbuilds = 0;
 
     ~Sim() override {
         if (stream) (void)hipStreamSynchronize(stream);
         if (gexec) (void)hipGraphExecDestroy(gexec);
+        for (hipEvent_t e : tev)
+            if (e) (void)hipEventDestroy(e);
         for (void *q : allocs) (void)hipFree(q);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -227,6 +233,24 @@ struct Sim final : fcc_sim {
                     }
                     p.Jc[rel][ti][tj] = (T)acc;
                 }
+        // coupling classes of the fast forces kernel
+        {
+            std::vector<double> vals;
+            for (int rel = 0; rel < 2; rel++)
+                for (int ti = 0; ti < FCC_MAX_TYPES; ti++)
+                    for (int tj = 0; tj < FCC_MAX_TYPES; tj++) {
+                        const double v = (double)p.Jc[rel][ti][tj];
+                        int cls = -1;
+                        if (v != 0.0 && ti < c->n_types && tj < c->n_types) {
+                            auto it = std::find(vals.begin(), vals.end(), v);
+                            cls = (int)(it - vals.begin());
+                            if (it == vals.end()) vals.push_back(v);
+                        }
+                        p.ccls[rel][ti][tj] = cls;
+                    }
+            p.ncls = (int)vals.size();
+            for (size_t q = 0; q < vals.size(); q++) p.Jcls[q] = (T)vals[q];
+        }
         range = 0.0;  // interaction_range(Overlap) = 0
         p.range = (T)range;
         for (int q = 0; q < FCC_MAX_TYPES; q++) {
@@ -287,8 +311,21 @@ struct Sim final : fcc_sim {
                 S[(size_t)(2 * nth + k) * SB + 16 + m] = (T)((double)(-n * n) * sn);
             }
         }
-        T *dS, *dcs;
-        if ((rc = alloc(&dS, S.size())) || (rc = alloc(&dcs, cs.size()))) return rc;
+        std::vector<T> Sc((size_t)3 * nth * ldc, T(0));
+        for (int q = 0; q < 3; q++)
+            for (int k = 0; k < nth; k++) {
+                const T *row = &S[(size_t)(q * nth + k) * SB];
+                T *out = &Sc[(size_t)(q * nth + k) * ldc];
+                out[0] = row[0];
+                for (int m = 1; m <= K; m++) {
+                    out[m] = row[m];
+                    out[K + m] = row[16 + m];
+                }
+            }
+        T *dS, *dcs, *dSc;
+        if ((rc = alloc(&dS, S.size())) || (rc = alloc(&dcs, cs.size())) || (rc = alloc(&dSc, Sc.size()))) return rc;
+        HIPCHK(hipMemcpyAsync(dSc, Sc.data(), sizeof(T) * Sc.size(), hipMemcpyHostToDevice, stream));
+        d.Sc = dSc;
         HIPCHK(hipMemcpyAsync(dS, S.data(), sizeof(T) * S.size(), hipMemcpyHostToDevice, stream));
         HIPCHK(hipMemcpyAsync(dcs, cs.data(), sizeof(T) * cs.size(), hipMemcpyHostToDevice, stream));
         d.S = dS;
@@ -307,6 +344,8 @@ struct Sim final : fcc_sim {
         wx = c->Lx / ncx;
         wy = c->Ly / ncy;
         nce = (int64_t)ncx * ncy;
+        // node offsets |dxij + R c| stay below L/2 when L > 4 cutoff
+        p.small_box = (c->Lx > 4.0 * cutoff && c->Ly > 4.0 * cutoff) ? 0 : 1;
         if ((rc = alloc(&cellid, N)) || (rc = alloc(&perm, N)) || (rc = alloc(&cstart, nce)) ||
             (rc = alloc(&ccnt, nce)) || (rc = alloc(&cfill, nce)))
             return rc;
@@ -367,9 +406,28 @@ struct Sim final : fcc_sim {
         return FCC_OK;
     }
 
+    // The fast kernel needs K in {5, 7}, at most two coupling classes and
+    // neighbour lists shorter than 2^NPLANE (its per-node hit counters).
+    bool fast_ok() const {
+        return (K == 5 || K == 7) && p.ncls <= 2 && (!has_contact || maxcount < (1 << NPLANE));
+    }
+
     int launch_forces(int use_status) {
         const unsigned g = blocks(N, 64);
         const int dbg = debug ? 1 : 0;
+        if (fast_ok()) {
+#define FCC_FAST(KV, NC)                                                                                 \
+    if (nth == 40) hipLaunchKernelGGL((k_forces_fast<T, 40, KV, NC>), g, 64, 0, stream, p, d, use_status, dbg); \
+    else hipLaunchKernelGGL((k_forces_fast<T, 0, KV, NC>), g, 64, 0, stream, p, d, use_status, dbg);
+            if (K == 5) {
+                if (p.ncls <= 1) { FCC_FAST(5, 1) } else { FCC_FAST(5, 2) }
+            } else {
+                if (p.ncls <= 1) { FCC_FAST(7, 1) } else { FCC_FAST(7, 2) }
+            }
+#undef FCC_FAST
+            HIPCHK(hipGetLastError());
+            return FCC_OK;
+        }
 #define FCC_FORCES(KMV)                                                                                  \
     if (nth == 40) hipLaunchKernelGGL((k_forces<T, 40, KMV>), g, 64, 0, stream, p, d, use_status, dbg);   \
     else hipLaunchKernelGGL((k_forces<T, 0, KMV>), g, 64, 0, stream, p, d, use_status, dbg);
@@ -387,13 +445,19 @@ struct Sim final : fcc_sim {
         const unsigned g = blocks(N, 256);
         const int ud = has_pdrag ? 1 : 0, dbg = debug ? 1 : 0;
         if (has_pdrag) hipLaunchKernelGGL((k_displacement<T>), g, 256, 0, stream, p, d);
-#define FCC_UPD(KMV)                                                                                   \
-    if (mixed) hipLaunchKernelGGL((k_update<T, KMV, true>), g, 256, 0, stream, p, d, ud, dbg);          \
-    else hipLaunchKernelGGL((k_update<T, KMV, false>), g, 256, 0, stream, p, d, ud, dbg);
-        switch (KM) {
-        case 5: FCC_UPD(5) break;
-        case 7: FCC_UPD(7) break;
-        default: FCC_UPD(15)
+#define FCC_UPD(KMV, EX)                                                                                   \
+    if (mixed) hipLaunchKernelGGL((k_update<T, KMV, EX, true>), g, 256, 0, stream, p, d, ud, dbg);          \
+    else hipLaunchKernelGGL((k_update<T, KMV, EX, false>), g, 256, 0, stream, p, d, ud, dbg);
+        if (K == 5) {
+            FCC_UPD(5, true)
+        } else if (K == 7) {
+            FCC_UPD(7, true)
+        } else if (KM == 5) {
+            FCC_UPD(5, false)
+        } else if (KM == 7) {
+            FCC_UPD(7, false)
+        } else {
+            FCC_UPD(15, false)
         }
 #undef FCC_UPD
         HIPCHK(hipGetLastError());
@@ -425,7 +489,12 @@ struct Sim final : fcc_sim {
             DevStatus st;
             HIPCHK(hipMemcpyAsync(&st, d.st, sizeof st, hipMemcpyDeviceToHost, stream));
             HIPCHK(hipStreamSynchronize(stream));
-            if (st.overflow <= d.cap) break;
+            if (st.overflow <= d.cap) {
+                if (fast_ok() != (bool)((K == 5 || K == 7) && p.ncls <= 2 && (!has_contact || st.overflow < (1 << NPLANE))))
+                    gen++;  // the forces kernel baked into the graph changes
+                maxcount = st.overflow;
+                break;
+            }
             const int32_t cap = 2 * st.overflow;
             int32_t *ni = nullptr;
             int rc = alloc(&ni, (size_t)cap * N);
@@ -491,6 +560,21 @@ struct Sim final : fcc_sim {
         return FCC_OK;
     }
 
+    double acc_f = 0.0, acc_u = 0.0;
+    int64_t acc_n = 0;
+
+    int run_timed(int64_t n, double *fms, double *ums) override {
+        want_timed = true;
+        acc_f = acc_u = 0.0;
+        acc_n = 0;
+        int rc = run(n);
+        want_timed = false;
+        if (rc) return rc;
+        if (fms) *fms = acc_n ? acc_f / acc_n : 0.0;
+        if (ums) *ums = acc_n ? acc_u / acc_n : 0.0;
+        return FCC_OK;
+    }
+
     int run(int64_t n) override {
         if (n <= 0) return FCC_OK;
         if (has_contact && skin <= 0.0) return step(n);  // graph_capable (src/simulation.jl:206-207)
@@ -510,11 +594,33 @@ struct Sim final : fcc_sim {
             }
             const int64_t chunk = has_contact ? std::min<int64_t>(256, std::max<int64_t>(4, (int64_t)std::llround(interval / 4))) : 256;
             const int64_t k = std::min(chunk, n - done);
-            for (int64_t q = 0; q < k; q++) HIPCHK(hipGraphLaunch(gexec, stream));
+            if (!want_timed) {
+                for (int64_t q = 0; q < k; q++) HIPCHK(hipGraphLaunch(gexec, stream));
+            } else {
+                // the chunk's last step is launched eagerly between events:
+                // the same kernels and arguments as the graph
+                if (!tev[0])
+                    for (auto &ev : tev) HIPCHK(hipEventCreate(&ev));
+                for (int64_t q = 0; q + 1 < k; q++) HIPCHK(hipGraphLaunch(gexec, stream));
+                p.skin_check = (has_contact && skin > 0.0) ? 1 : 0;
+                HIPCHK(hipEventRecord(tev[0], stream));
+                if ((rc = launch_forces(1))) return rc;
+                HIPCHK(hipEventRecord(tev[1], stream));
+                if ((rc = launch_update())) return rc;
+                HIPCHK(hipEventRecord(tev[2], stream));
+            }
             HIPCHK(hipMemcpyAsync(&st, d.st, sizeof st, hipMemcpyDeviceToHost, stream));
             HIPCHK(hipStreamSynchronize(stream));
             const int64_t executed = st.clock - clock;
             clock = st.clock;
+            if (want_timed && executed == k) {  // the chunk's last step ran (not frozen)
+                float a = 0.f, b = 0.f;
+                HIPCHK(hipEventElapsedTime(&a, tev[0], tev[1]));
+                HIPCHK(hipEventElapsedTime(&b, tev[1], tev[2]));
+                acc_f += a;
+                acc_u += b;
+                acc_n++;
+            }
             for (int64_t q = 0; q < executed; q++) t += cfg.dt;
             nstep += executed;
             done += executed;
@@ -874,6 +980,12 @@ int fcc_time_step_kernels(fcc_sim *s, int64_t steps, double *fms, double *ums) {
     FCC_HANDLE(s);
     if (steps < 0) return fail(FCC_ERR_ARGUMENT, "steps must be non-negative");
     return s->time_step_kernels(steps, fms, ums);
+}
+
+int fcc_run_timed(fcc_sim *s, int64_t n, double *fms, double *ums) {
+    FCC_HANDLE(s);
+    if (n < 0) return fail(FCC_ERR_ARGUMENT, "steps must be non-negative");
+    return s->run_timed(n, fms, ums);
 }
 
 int fcc_philox_noise(uint64_t seed, uint64_t draw, int64_t n, int32_t precision, void *out) {

@@ -47,6 +47,12 @@ struct Params {
     uint32_t key0, key1;
     T skin;
     int skin_check;  // fused Verlet skin check in the update kernel
+    // Overlap couplings as classes of equal nonzero J: ccls[related][ti][tj]
+    // is the class (-1: zero coupling), Jcls its value.
+    int ncls;
+    int ccls[2][FCC_MAX_TYPES][FCC_MAX_TYPES];
+    T Jcls[2 * FCC_MAX_TYPES * FCC_MAX_TYPES];
+    int small_box;  // a node offset may need a second minimum image
 };
 
 // Device buffers of one simulation. Per-body coefficient rows are stored
@@ -70,6 +76,7 @@ struct Ptrs {
     T *disp;
     const T *S;   // [3][nth][ldc]: C, D, DD rows
     const T *cs;  // [nth][2]: cos φk, sin φk
+    const T *Sc;  // [3][nth][ldc]: C, D, DD rows in coefficient order [a0, cos.., sin..]
     DevStatus *st;
 };
 
@@ -225,6 +232,34 @@ __device__ __forceinline__ void store_row(const T (&l)[2 * KM + 1], T *row, int 
             row[K + m] = l[KM + m];
         }
     }
+}
+
+// Rows padded to W elements (a multiple of 16 bytes) are read and written with
+// 16-byte vector accesses; every row starts at a multiple of ldc·sizeof(T) ≥ 64
+// bytes from a 256-byte aligned allocation, and ldc ≥ W.
+template <int NC, typename T>
+constexpr int row_width() {
+    return ((NC * (int)sizeof(T) + 15) / 16) * 16 / (int)sizeof(T);
+}
+
+template <typename T, int W>
+__device__ __forceinline__ void row_load(T (&l)[W], const T *row) {
+    static_assert((W * sizeof(T)) % 16 == 0, "row width");
+    const uint4 *s = reinterpret_cast<const uint4 *>(row);
+    uint4 tmp[W * sizeof(T) / 16];
+#pragma unroll
+    for (int q = 0; q < (int)(W * sizeof(T) / 16); q++) tmp[q] = s[q];
+    __builtin_memcpy(l, tmp, sizeof(tmp));
+}
+
+template <typename T, int W>
+__device__ __forceinline__ void row_store(const T (&l)[W], T *row) {
+    static_assert((W * sizeof(T)) % 16 == 0, "row width");
+    uint4 tmp[W * sizeof(T) / 16];
+    __builtin_memcpy(tmp, l, sizeof(tmp));
+    uint4 *s = reinterpret_cast<uint4 *>(row);
+#pragma unroll
+    for (int q = 0; q < (int)(W * sizeof(T) / 16); q++) s[q] = tmp[q];
 }
 
 // contour_radius_dir without the R0 factor (src/geometry.jl:282-291).

@@ -1,0 +1,328 @@
+// fcc_forces_fast.hpp — the forces kernel of the hot path (FP32 / FP64 VALU).
+//
+// Same work as k_forces (body terms, Overlap contact, pullback; src/gradients.jl,
+// src/energies/contact.jl) with the shape of the hot configurations fixed at
+// compile time:
+//  - K (shape modes) is a template constant: no per-mode guards;
+//  - Tension and PerimeterElasticity share the integrand √(R² + R'²)
+//    (src/energies/body_terms.jl:10,31), so one perimeter integral and one
+//    combined weight serve every such term; Bending has its own;
+//  - the contact density is kept as per-node hit counts, one bit-sliced
+//    counter (NP bit planes over the nθ nodes) per Overlap coupling class, in
+//    registers: P_k = Σ_class J_class · count_class(k). A node's count is at
+//    most the body's neighbour count, so the host uses this kernel only
+//    while every list holds fewer than 2^NP entries;
+//  - R_j(ψ) is evaluated by complex Horner (4 FMA per mode);
+//  - only nodes inside the angular window that can reach a neighbour's
+//    bounding disc are tested (window from a polynomial atan with a safety
+//    margin far above its error);
+//  - FP32 uses the hardware sqrt / rsqrt / rcp (≤ 1 ulp).
+#pragma once
+
+#include "fcc_kernels.hpp"
+
+namespace fcc {
+
+constexpr int NPLANE = 5;  // bit planes of the contact hit counters (counts < 32)
+
+__device__ __forceinline__ float f_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double f_sqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float f_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ double f_rsqrt(double x) { return 1.0 / sqrt(x); }
+__device__ __forceinline__ float f_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double f_rcp(double x) { return 1.0 / x; }
+
+// atan on [0, 1], Abramowitz & Stegun 4.4.49 (|error| ≤ 1e-5).
+__device__ __forceinline__ float atan_unit(float t) {
+    const float t2 = t * t;
+    return t * (0.9998660f + t2 * (-0.3302995f + t2 * (0.1801410f + t2 * (-0.0851330f + t2 * 0.0208351f))));
+}
+
+// atan2 from atan_unit by octant reduction (|error| ≲ 2e-5 rad).
+__device__ __forceinline__ float fast_atan2(float y, float x) {
+    const float ax = fabsf(x), ay = fabsf(y);
+    const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+    float a = atan_unit(mn * __builtin_amdgcn_rcpf(mx));
+    if (ay > ax) a = 1.57079633f - a;
+    if (x < 0.f) a = 3.14159265f - a;
+    return y < 0.f ? -a : a;
+}
+
+// Nodes [k0, k0 + wlen) (mod nθ) whose direction from body i lies within the
+// half-angle asin(lim / D) of the direction to body j: no other node can be
+// within lim of r_j (whatever its radius). Full circle if D ≤ lim.
+__device__ __forceinline__ void node_window(float dx, float dy, float lim2, float D2, float inv_dphi, int nth,
+                                            int &k0, int &wlen) {
+    k0 = 0;
+    wlen = nth;
+    const float sd2 = lim2 * 1.0002f / D2;
+    if (!(sd2 < 1.f)) return;
+    const float delta = fast_atan2(__builtin_amdgcn_sqrtf(sd2), __builtin_amdgcn_sqrtf(1.f - sd2)) + 2e-3f;
+    const float psi = fast_atan2(-dy, -dx);
+    const int klo = (int)floorf((psi - delta) * inv_dphi);
+    const int khi = (int)ceilf((psi + delta) * inv_dphi);
+    if (khi - klo + 1 >= nth) return;
+    wlen = khi - klo + 1;
+    k0 = klo % nth;
+    if (k0 < 0) k0 += nth;
+}
+
+// R_j(ψ)/R0 = a0 + Σ_{m=1..K} αm cos (m+1)ψ + βm sin (m+1)ψ as Re Σ cn z^n,
+// z = (c1, s1), by complex Horner (src/geometry.jl:282-291 evaluates the
+// same sum by rotating (cos nψ, sin nψ)).
+template <typename T, int K>
+__device__ __forceinline__ T contour_horner(const T (&l)[2 * K + 1], T zr, T zi) {
+    T sr = l[K], si = -l[2 * K];
+#pragma unroll
+    for (int n = K; n >= 2; n--) {
+        const T nr = fma(sr, zr, fma(-si, zi, l[n - 1]));
+        const T ni = fma(sr, zi, fma(si, zr, -l[K + n - 1]));
+        sr = nr;
+        si = ni;
+    }
+    const T nr = sr * zr - si * zi;
+    const T ni = sr * zi + si * zr;
+    return fma(nr, zr, fma(-ni, zi, l[0]));
+}
+
+// Contour R, R', R'' at node k from the compact basis rows [a0, cos.., sin..].
+template <typename T, int K>
+__device__ __forceinline__ void synth_k(const FCC_CONST T *S, int nth, int ldc, int k, const T (&l)[2 * K + 1],
+                                        bool need_dd, T r0, T &R, T &dR, T &ddR) {
+    const FCC_CONST T *C = S + (size_t)k * ldc;
+    const FCC_CONST T *D = S + (size_t)(nth + k) * ldc;
+    T r = l[0], dr = T(0);
+#pragma unroll
+    for (int c = 1; c <= 2 * K; c++) {
+        r = fma(C[c], l[c], r);
+        dr = fma(D[c], l[c], dr);
+    }
+    R = r0 * r;
+    dR = r0 * dr;
+    ddR = T(0);
+    if (need_dd) {
+        const FCC_CONST T *DD = S + (size_t)(2 * nth + k) * ldc;
+        T ddr = T(0);
+#pragma unroll
+        for (int c = 1; c <= 2 * K; c++) ddr = fma(DD[c], l[c], ddr);
+        ddR = r0 * ddr;
+    }
+}
+
+template <typename T, int NTH_, int K, int NCLS>
+__global__ __launch_bounds__(64) void k_forces_fast(const Params<T> p, const Ptrs<T> d, int use_status,
+                                                    int debug) {
+    constexpr int NC = 2 * K + 1;
+    constexpr int NTHM = NTH_ ? NTH_ : FCC_MAX_THETA;
+    __shared__ T sR[NTHM][64];
+    __shared__ T scs[2][NTHM];
+    const int nth = NTH_ ? NTH_ : p.nth;
+    const int lane = threadIdx.x;
+    for (int k = lane; k < nth; k += 64) {
+        scs[0][k] = d.cs[2 * k];
+        scs[1][k] = d.cs[2 * k + 1];
+    }
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * 64 + lane;
+    if (use_status) {
+        if (i == 0) {
+            int r = d.st->rebuild;
+            d.st->update = r;
+            if (r == 0) d.st->clock += 1;
+        }
+        if (d.st->rebuild != 0) return;
+    }
+    if (i >= p.N) return;
+    const FCC_CONST T *S = as_const(d.Sc);
+    const FCC_CONST T *CS = as_const(d.cs);
+    const int ldc = p.ldc;
+    constexpr int W = row_width<NC, T>();
+    T lab[NC];
+    {
+        T a[W];
+        row_load<T, W>(a, d.lab + i * ldc);
+#pragma unroll
+        for (int c = 0; c < NC; c++) lab[c] = a[c];
+    }
+    const T r0 = d.R0[i];
+    const int ti = d.type[i];
+    const uint32_t tbit = 1u << ti;
+    const bool need_dd = p.need_dd != 0;
+    const bool contact = p.ncontact != 0;
+
+    // pass 1: perimeter integral (Tension, PerimeterElasticity) and Bending integral
+    T Qp = T(0), Qb = T(0);
+    for (int k = 0; k < nth; k++) {
+        T R, dR, ddR;
+        synth_k<T, K>(S, nth, ldc, k, lab, need_dd, r0, R, dR, ddR);
+        if (contact) sR[k][lane] = R;
+        Qp += f_sqrt(R * R + dR * dR);
+        if (need_dd) Qb += term_q<T>(FCC_TERM_BENDING, R, dR, ddR);
+    }
+    T Wp = T(0), Wb = T(0), Et[MAXB];
+#pragma unroll
+    for (int b = 0; b < MAXB; b++) {
+        Et[b] = T(0);
+        if (b < p.nbody && (p.bmask[b] & tbit)) {
+            T dF;
+            const bool bend = p.bkind[b] == FCC_TERM_BENDING;
+            Et[b] = term_E<T>(p.bkind[b], p.bp0[b], p.bp1[b], (bend ? Qb : Qp) * p.dphi, r0, dF);
+            if (bend)
+                Wb += dF * p.dphi;
+            else
+                Wp += dF * p.dphi;
+        }
+    }
+    T E = T(0);
+#pragma unroll
+    for (int b = MAXB - 1; b >= 0; b--)
+        if (b < p.nbody) E = Et[b] + E;
+
+    // contact: hit counts per node and coupling class
+    uint64_t pl[NCLS > 0 ? NCLS : 1][NPLANE];
+#pragma unroll
+    for (int c = 0; c < (NCLS > 0 ? NCLS : 1); c++)
+#pragma unroll
+        for (int q = 0; q < NPLANE; q++) pl[c][q] = 0;
+    if (contact) {
+        const double xi = d.x[i], yi = d.y[i];
+        const T rmi = d.rmax[i];
+        const int pi = d.parent[i];
+        const T Lxf = (T)p.Lx, Lyf = (T)p.Ly;
+        const bool wrap2 = p.small_box != 0;
+        const float inv_dphi = (float)(1.0 / (double)p.dphi);
+        int cnt = d.ncount[i];
+        if (cnt > d.cap) cnt = d.cap;
+        for (int q = 0; q < cnt; q++) {
+            const int j = d.nidx[(int64_t)q * p.N + i];
+            const int cls = p.ccls[pi == d.parent[j] ? 1 : 0][ti][d.type[j]];
+            if (cls < 0) continue;  // zero coupling: no contribution
+            const T dxij = (T)min_image(xi - d.x[j], p.Lx);
+            const T dyij = (T)min_image(yi - d.y[j], p.Ly);
+            const T lim = d.rmax[j] + p.range;
+            const T lim2 = lim * lim;
+            const T D2 = dxij * dxij + dyij * dyij;
+            const T reach = (rmi + lim) * T(1.0001) + T(1e-6);
+            if (D2 > reach * reach) continue;
+            int k0, wlen;
+            node_window((float)dxij, (float)dyij, (float)lim2, (float)D2, inv_dphi, nth, k0, wlen);
+            uint64_t mask = 0;
+            int k = k0;
+            for (int t = 0; t < wlen; t++) {
+                const T Rk = sR[k][lane];
+                T dx = dxij + Rk * scs[0][k];
+                T dy = dyij + Rk * scs[1][k];
+                if (wrap2) {
+                    dx = min_image(dx, Lxf);
+                    dy = min_image(dy, Lyf);
+                }
+                if (dx * dx + dy * dy <= lim2) mask |= 1ull << t;
+                if (++k == nth) k = 0;
+            }
+            if (!mask) continue;
+            T lj[NC];
+            {
+                T a[W];
+                row_load<T, W>(a, d.lab + (int64_t)j * ldc);
+#pragma unroll
+                for (int c = 0; c < NC; c++) lj[c] = a[c];
+            }
+            const T r0j = d.R0[j];
+            uint64_t hit = 0;
+            while (mask) {
+                const int t = __builtin_ctzll(mask);
+                mask &= mask - 1;
+                int kk = k0 + t;
+                if (kk >= nth) kk -= nth;
+                const T Rk = sR[kk][lane];
+                T dx = dxij + Rk * scs[0][kk];
+                T dy = dyij + Rk * scs[1][kk];
+                if (wrap2) {
+                    dx = min_image(dx, Lxf);
+                    dy = min_image(dy, Lyf);
+                }
+                const T d2 = dx * dx + dy * dy;
+                const T rs = f_rsqrt(d2);
+                const bool pos = d2 > T(0);
+                const T dist = pos ? d2 * rs : T(0);
+                const T cp = pos ? dx * rs : T(1);
+                const T sp = pos ? dy * rs : T(0);
+                if (r0j * contour_horner<T, K>(lj, cp, sp) - dist >= T(0)) hit |= 1ull << kk;
+            }
+#pragma unroll
+            for (int c = 0; c < (NCLS > 0 ? NCLS : 1); c++) {
+                uint64_t m = cls == c ? hit : 0;
+#pragma unroll
+                for (int qq = 0; qq < NPLANE; qq++) {
+                    const uint64_t carry = pl[c][qq] & m;
+                    pl[c][qq] ^= m;
+                    m = carry;
+                }
+            }
+        }
+    }
+
+    // pass 2: contour gradients, contact force / torque, pullback
+    T gl[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) gl[c] = T(0);
+    T fx = T(0), fy = T(0), tt = T(0);
+    for (int k = 0; k < nth; k++) {
+        T R, dR, ddR;
+        synth_k<T, K>(S, nth, ldc, k, lab, need_dd, r0, R, dR, ddR);
+        if (contact) R = sR[k][lane];
+        const T rs = f_rsqrt(R * R + dR * dR);
+        T GR = Wp * (R * rs), GD = Wp * (dR * rs), GDD = T(0);
+        if (need_dd) {
+            T j0, j1, j2;
+            term_J<T>(FCC_TERM_BENDING, R, dR, ddR, j0, j1, j2);
+            GR += Wb * j0;
+            GD += Wb * j1;
+            GDD = Wb * j2;
+        }
+        if (contact) {
+            T g = T(0);
+#pragma unroll
+            for (int c = 0; c < (NCLS > 0 ? NCLS : 1); c++) {
+                uint32_t n = 0;
+#pragma unroll
+                for (int qq = 0; qq < NPLANE; qq++) n |= (uint32_t)((pl[c][qq] >> k) & 1u) << qq;
+                g += p.Jcls[c] * (T)n;
+            }
+            const T pp = g * p.dphi;
+            const T c = CS[2 * k], s = CS[2 * k + 1];
+            GR += pp * R;
+            fx += pp * (R * c + dR * s);
+            fy += pp * (R * s - dR * c);
+            tt += (-pp) * R * dR;
+            if (debug) d.P[i * nth + k] = g;
+        }
+        const FCC_CONST T *C = S + (size_t)k * ldc;
+        const FCC_CONST T *D = S + (size_t)(nth + k) * ldc;
+        gl[0] += GR;
+#pragma unroll
+        for (int c = 1; c < NC; c++) gl[c] = fma(C[c], GR, fma(D[c], GD, gl[c]));
+        if (need_dd) {
+            const FCC_CONST T *DD = S + (size_t)(2 * nth + k) * ldc;
+#pragma unroll
+            for (int c = 1; c < NC; c++) gl[c] = fma(DD[c], GDD, gl[c]);
+        }
+        if (debug) {
+            d.GR[i * nth + k] = GR;
+            d.GD[i * nth + k] = GD;
+            d.GDD[i * nth + k] = GDD;
+        }
+    }
+    {
+        T a[W];
+#pragma unroll
+        for (int c = 0; c < W; c++) a[c] = c < NC ? gl[c] : T(0);
+        row_store<T, W>(a, d.glab + i * ldc);
+    }
+    d.gx[i] = fx;
+    d.gy[i] = fy;
+    d.gth[i] = tt;
+    d.Ebody[i] = E;
+}
+
+}  // namespace fcc

@@ -409,13 +409,14 @@ __global__ __launch_bounds__(256) void k_displacement(const Params<T> p, const P
 // without parent drag), the body-frame gradient, the shape constraint (FP64
 // when F64S), position wrapping in FP64, the lab frame of the new state and
 // the Verlet skin check (src/neighbors.jl:409-416).
-template <typename T, int KM, bool F64S>
+template <typename T, int KM, bool EXACT, bool F64S>
 __global__ __launch_bounds__(256) void k_update(const Params<T> p, const Ptrs<T> d, int use_disp, int debug) {
     constexpr int NCM = 2 * KM + 1;
+    constexpr int W = row_width<NCM, T>();
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= p.N || d.st->update != 0) return;
     const uint64_t draw = (uint64_t)(d.st->clock - 1);
-    const int K = p.K;
+    const int K = EXACT ? KM : p.K;
     const int ti = d.type[i];
     const T th = d.th[i], r0 = d.R0[i];
     T d0, d1, d2;
@@ -435,8 +436,19 @@ __global__ __launch_bounds__(256) void k_update(const Params<T> p, const Ptrs<T>
         d2 = sqrt(T(2) * p.Dr[ti] * p.dt) * x3;
     }
     T gl[NCM], c[NCM], gc[NCM], l[NCM];
-    load_row<T, KM>(gl, d.glab + i * p.ldc, K);
-    load_row<T, KM>(c, d.coef + i * p.ldc, K);
+    if constexpr (EXACT) {
+        T a[W], b[W];
+        row_load<T, W>(a, d.glab + i * p.ldc);
+        row_load<T, W>(b, d.coef + i * p.ldc);
+#pragma unroll
+        for (int q = 0; q < NCM; q++) {
+            gl[q] = a[q];
+            c[q] = b[q];
+        }
+    } else {
+        load_row<T, KM>(gl, d.glab + i * p.ldc, K);
+        load_row<T, KM>(c, d.coef + i * p.ldc, K);
+    }
     body_frame<T, KM>(gl, c, th, r0, K, gc);
     const T thn = th - p.dt * p.muth[ti] * d.gth[i] + d2;
     const T h = p.dt * p.muab[ti];
@@ -454,14 +466,27 @@ __global__ __launch_bounds__(256) void k_update(const Params<T> p, const Ptrs<T>
     const double yw = wrap(d.y[i] + (double)d1, p.Ly, sy);
     const T rm = lab_frame<T, KM>(c, thn, r0, K, l);
     d.th[i] = thn;
-    store_row<T, KM>(c, d.coef + i * p.ldc, K);
     d.x[i] = xw;
     d.y[i] = yw;
     d.imgx[i] += sx;
     d.imgy[i] += sy;
-    store_row<T, KM>(l, d.lab + i * p.ldc, K);
     d.rmax[i] = rm;
-    if (debug) store_row<T, KM>(gc, d.gcoef + i * p.ldc, K);
+    if constexpr (EXACT) {
+        T a[W], b[W], g[W];
+#pragma unroll
+        for (int q = 0; q < W; q++) {
+            a[q] = q < NCM ? c[q] : T(0);
+            b[q] = q < NCM ? l[q] : T(0);
+            g[q] = q < NCM ? gc[q] : T(0);
+        }
+        row_store<T, W>(a, d.coef + i * p.ldc);
+        row_store<T, W>(b, d.lab + i * p.ldc);
+        if (debug) row_store<T, W>(g, d.gcoef + i * p.ldc);
+    } else {
+        store_row<T, KM>(c, d.coef + i * p.ldc, K);
+        store_row<T, KM>(l, d.lab + i * p.ldc, K);
+        if (debug) store_row<T, KM>(gc, d.gcoef + i * p.ldc, K);
+    }
     if (p.skin_check) {
         const double dx = min_image(xw - d.x0[i], p.Lx);
         const double dy = min_image(yw - d.y0[i], p.Ly);

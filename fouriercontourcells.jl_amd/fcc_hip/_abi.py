@@ -134,6 +134,7 @@ _DEVICE_ONLY = {
                    ctypes.POINTER(ctypes.c_uint32)],
     "abi_version": [],
     "time_step_kernels": [_vp, _i64, _dp, _dp],
+    "run_timed": [_vp, _i64, _dp, _dp],
 }
 
 # Every symbol include/fcc.h declares.

@@ -21,6 +21,12 @@ extern "C" {
 int fcc_time_step_kernels(fcc_sim *sim, int64_t steps, double *forces_ms,
                           double *update_ms);
 
+/* fcc_run (graph mode) where the last step of every chunk is launched eagerly
+ * (same kernels and arguments as the captured step) between HIP events on the
+ * simulation's stream; returns the average forces and update kernel
+ * durations over those steps, in milliseconds. Results equal fcc_run's. */
+int fcc_run_timed(fcc_sim *sim, int64_t n, double *forces_ms, double *update_ms);
+
 #ifdef __cplusplus
 }
 #endif

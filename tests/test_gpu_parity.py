@@ -211,3 +211,38 @@ def test_body_terms_only_graph(hip):
         return F.Simulation(b, box, types=(F.BodyType(v0=0.3, Dr=0.5),),
                             energies=(F.PerimeterElasticity(0.2, 3.7),))
     assert same(F.step(mk(), 50), F.run(mk(), 50))
+
+
+def test_timed_run_equals_run(hip):
+    # the instrumented graph run (bench timing) changes nothing in the results
+    import ctypes
+    say("timed run == run")
+    a, b = soft_fluid("fp32"), soft_fluid("fp32")
+    F.run(a, 150)
+    f, u = ctypes.c_double(), ctypes.c_double()
+    assert hip.run_timed(b._h, 150, ctypes.byref(f), ctypes.byref(u)) == 0
+    assert f.value > 0 and u.value > 0
+    assert same(a, b)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_generic_kernel_matches_oracle(hip, oracle_engine, precision):
+    # K = 4 (maxmode 5) and three Overlap coupling values take the generic
+    # forces kernel (per-node float contact density in LDS)
+    say("generic kernel", precision)
+    cells, box = F.hexagonal_monolayer(6, 4, packing=0.97, seed=2, maxmode=5)
+    b = F.with_nuclei(cells)
+    rng = np.random.default_rng(8)
+    b.coef[1:, :] = 0.04 * rng.standard_normal((b.coef.shape[0] - 1, len(b)))
+    energies = (F.Tension(0.1), F.Overlap([[1.0, 0.5], [0.5, 1.0]], related=[[1.0, -2.0], [-2.0, 1.0]]))
+    types = (F.BodyType(v0=0.2, Dr=0.1), F.BodyType(Dr=0.1, drag="parent"))
+    a = F.refresh(F.Simulation(b, box, types=types, energies=energies, debug=True, engine=oracle_engine))
+    g = F.refresh(F.Simulation(b, box, types=types, energies=energies, debug=True, precision=precision))
+    tol = 1e-10 if precision == "fp64" else 2e-4
+    ga, gg = a.grad, g.grad
+    assert np.count_nonzero(ga.P != 0) > 50
+    if precision == "fp64":
+        np.testing.assert_allclose(gg.P, ga.P, rtol=tol, atol=tol)
+    scale = np.max(np.abs(ga.gx))
+    np.testing.assert_allclose(gg.gx, ga.gx, rtol=tol, atol=tol * scale)
+    np.testing.assert_allclose(gg.gcoef, ga.gcoef, rtol=tol, atol=tol * np.max(np.abs(ga.gcoef)))
