@@ -17,7 +17,7 @@
 #include <string>
 #include <vector>
 
-#include "fcc_forces_fast.hpp"
+#include "fcc_forces_mfma.hpp"
 #include "../../include/fcc_bench.h"
 
 using namespace fcc;
@@ -102,6 +102,59 @@ static uint32_t type_mask(const fcc_term &t) {
     return m;
 }
 
+// ------------------------------------------------------- MFMA operand tables
+
+static uint16_t to_f16_bits(double x) {
+    const _Float16 h = (_Float16)(float)x;
+    uint16_t u;
+    std::memcpy(&u, &h, 2);
+    return u;
+}
+static double f16_value(uint16_t u) {
+    _Float16 h;
+    std::memcpy(&h, &u, 2);
+    return (double)(float)h;
+}
+static uint16_t to_bf16_bits(double x) {  // round to nearest even
+    const float f = (float)x;
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    const uint32_t r = u + 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(r >> 16);
+}
+static double bf16_value(uint16_t u) {
+    const uint32_t v = (uint32_t)u << 16;
+    float f;
+    std::memcpy(&f, &v, 4);
+    return (double)f;
+}
+
+// Basis value of quantity q (0: C, 1: D, 2: DD) at node k for coefficient c of
+// the reference order [a0, cos modes, sin modes] (src/basis.jl:46-58).
+static double basis_value(int q, int k, int c, int K, int nth) {
+    const double phi = (double)k * (2.0 * 3.141592653589793 / (double)nth);
+    if (c == 0) return q == 0 ? 1.0 : 0.0;
+    const int m = c <= K ? c : c - K;
+    const int n = m + 1;
+    const bool cosine = c <= K;
+    const double cn = std::cos((double)n * phi), sn = std::sin((double)n * phi);
+    if (q == 0) return cosine ? cn : sn;
+    if (q == 1) return cosine ? (double)(-n) * sn : (double)n * cn;
+    return cosine ? (double)(-n * n) * cn : (double)(-n * n) * sn;
+}
+
+// Physical meaning of row w (0..31) of output tile t of the MFMA layout:
+// slot σ = 16t + r of lane half hh holds quantity σ / 20 at node 20·hh + σ % 20.
+static bool row_phys(int t, int w, int T, int &q, int &k) {
+    const int r = 4 * (w / 8) + (w % 4), hh = (w / 4) % 2;
+    const int sig = 16 * t + r;
+    const int nq = T == 4 ? 3 : 2;
+    if (sig >= nq * NH) return false;
+    q = sig / NH;
+    k = NH * hh + sig % NH;
+    return true;
+}
+
 // ------------------------------------------------------------- simulation
 
 struct fcc_sim {
@@ -139,6 +192,8 @@ struct Sim final : fcc_sim {
     double wx = 0, wy = 0;
     int64_t nce = 1;
     int32_t *cellid = nullptr, *perm = nullptr, *cstart = nullptr, *ccnt = nullptr, *cfill = nullptr;
+    uint16_t *asyn = nullptr, *bpull = nullptr;  // MFMA operand tables (mixed modes)
+    int mfma_T = 0;
     bool valid = false;
     int32_t maxcount = 0;  // largest neighbour count of the last build
     int64_t builds = 0, nstep = 0;
@@ -274,7 +329,7 @@ struct Sim final : fcc_sim {
         if ((rc = alloc(&d.x, N)) || (rc = alloc(&d.y, N)) || (rc = alloc(&d.imgx, N)) || (rc = alloc(&d.imgy, N)) ||
             (rc = alloc(&d.th, N)) || (rc = alloc(&d.coef, N * ldc)) || (rc = alloc(&d.R0, N)) ||
             (rc = alloc(&d.type, N)) || (rc = alloc(&d.parent, N)) || (rc = alloc(&d.lab, N * ldc)) ||
-            (rc = alloc(&d.rmax, N)) || (rc = alloc(&d.glab, N * ldc)) || (rc = alloc(&d.gx, N)) ||
+            (rc = alloc(&d.rmax, N)) || (rc = alloc(&d.rmin, N)) || (rc = alloc(&d.glab, N * ldc)) || (rc = alloc(&d.gx, N)) ||
             (rc = alloc(&d.gy, N)) || (rc = alloc(&d.gth, N)) || (rc = alloc(&d.Ebody, N)) ||
             (rc = alloc(&d.gcoef, N * ldc)) || (rc = alloc(&d.st, 1)))
             return rc;
@@ -330,6 +385,46 @@ struct Sim final : fcc_sim {
         HIPCHK(hipMemcpyAsync(dcs, cs.data(), sizeof(T) * cs.size(), hipMemcpyHostToDevice, stream));
         d.S = dS;
         d.cs = dcs;
+        if (mixed && nth == NTH_M && (K == 5 || K == 7)) {
+            const bool bf = c->precision == FCC_MIXED_BF16;
+            const int TT = p.need_dd ? 4 : 3;
+            auto enc = [&](double x, uint16_t &hi, uint16_t &lo) {
+                hi = bf ? to_bf16_bits(x) : to_f16_bits(x);
+                const double rem = x - (bf ? bf16_value(hi) : f16_value(hi));
+                lo = bf ? to_bf16_bits(rem) : to_f16_bits(rem);
+            };
+            std::vector<uint16_t> A((size_t)TT * 2 * 64 * 8, 0), B((size_t)2 * TT * 2 * 64 * 8, 0);
+            for (int t = 0; t < TT; t++)
+                for (int l = 0; l < 64; l++)
+                    for (int sl = 0; sl < 8; sl++) {
+                        int q, k;
+                        const int cc = 8 * (l / 32) + sl;
+                        double v = 0.0;
+                        if (cc < nc && row_phys(t, l % 32, TT, q, k)) v = basis_value(q, k, cc, K, nth);
+                        uint16_t hi, lo;
+                        enc(v, hi, lo);
+                        A[((size_t)(t * 2 + 0) * 64 + l) * 8 + sl] = hi;
+                        A[((size_t)(t * 2 + 1) * 64 + l) * 8 + sl] = lo;
+                    }
+            for (int ch = 0; ch < 2 * TT; ch++)
+                for (int l = 0; l < 64; l++)
+                    for (int sl = 0; sl < 8; sl++) {
+                        const int n = l % 32, hh = l / 32, t = ch / 2;
+                        const int w = 16 * (ch % 2) + 8 * (sl / 4) + 4 * hh + (sl % 4);
+                        int q, k;
+                        const int coefi = n < 16 ? n : n - 16;
+                        double v = 0.0;
+                        if (coefi < nc && row_phys(t, w, TT, q, k)) v = basis_value(q, k, coefi, K, nth);
+                        uint16_t hi, lo;
+                        enc(v, hi, lo);
+                        B[((size_t)(ch * 2 + 0) * 64 + l) * 8 + sl] = n < 16 ? hi : lo;
+                        B[((size_t)(ch * 2 + 1) * 64 + l) * 8 + sl] = n < 16 ? hi : 0;
+                    }
+            if ((rc = alloc(&asyn, A.size())) || (rc = alloc(&bpull, B.size()))) return rc;
+            HIPCHK(hipMemcpy(asyn, A.data(), A.size() * 2, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(bpull, B.data(), B.size() * 2, hipMemcpyHostToDevice));
+            mfma_T = TT;
+        }
 
         // cell grid (src/simulation.jl:90, src/neighbors.jl:238-244)
         double maxR0 = 0.0;
@@ -415,6 +510,25 @@ struct Sim final : fcc_sim {
     int launch_forces(int use_status) {
         const unsigned g = blocks(N, 64);
         const int dbg = debug ? 1 : 0;
+        if constexpr (sizeof(T) == 4) {
+            if (mfma_T && fast_ok()) {
+                const unsigned gm = blocks(N, 32);
+                const bool bf = cfg.precision == FCC_MIXED_BF16;
+#define FCC_MF(KV, TV, NCV)                                                                             \
+    if (bf) hipLaunchKernelGGL((k_forces_mfma<KV, TV, NCV, true>), gm, 64, 0, stream, p, d, asyn, bpull, use_status, dbg); \
+    else hipLaunchKernelGGL((k_forces_mfma<KV, TV, NCV, false>), gm, 64, 0, stream, p, d, asyn, bpull, use_status, dbg);
+#define FCC_MF_T(KV, NCV) if (mfma_T == 3) { FCC_MF(KV, 3, NCV) } else { FCC_MF(KV, 4, NCV) }
+                if (K == 5) {
+                    if (p.ncls <= 1) { FCC_MF_T(5, 1) } else { FCC_MF_T(5, 2) }
+                } else {
+                    if (p.ncls <= 1) { FCC_MF_T(7, 1) } else { FCC_MF_T(7, 2) }
+                }
+#undef FCC_MF_T
+#undef FCC_MF
+                HIPCHK(hipGetLastError());
+                return FCC_OK;
+            }
+        }
         if (fast_ok()) {
 #define FCC_FAST(KV, NC)                                                                                 \
     if (nth == 40) hipLaunchKernelGGL((k_forces_fast<T, 40, KV, NC>), g, 64, 0, stream, p, d, use_status, dbg); \

@@ -65,6 +65,7 @@ struct Ptrs {
     T *th, *coef, *R0;
     int32_t *type, *parent;
     T *lab, *rmax;
+    T *rmin;  // lower bound R0 (a0 - Σ √(an² + bn²)) of the contour radius
     T *glab, *gx, *gy, *gth, *Ebody, *gcoef;
     T *P, *GR, *GD, *GDD;  // debug (may be null)
     int32_t *nidx, *ncount;

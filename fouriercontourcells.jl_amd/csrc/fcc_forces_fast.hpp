@@ -12,7 +12,12 @@
 //    registers: P_k = Σ_class J_class · count_class(k). A node's count is at
 //    most the body's neighbour count, so the host uses this kernel only
 //    while every list holds fewer than 2^NP entries;
-//  - R_j(ψ) is evaluated by complex Horner (4 FMA per mode);
+//  - Overlap needs only the sign of d = R_j(ψ) - |u - r_j|: a node closer to
+//    r_j than the lower bound rmin_j of R_j (minus a margin far above the
+//    rounding of the evaluation) is inside without evaluating R_j(ψ); R_j(ψ)
+//    is evaluated, by complex Horner (4 FMA per mode), only for nodes in the
+//    shell rmin_j ≤ |u - r_j| ≤ rmax_j, so every decision equals the
+//    reference's;
 //  - only nodes inside the angular window that can reach a neighbour's
 //    bounding disc are tested (window from a polynomial atan with a safety
 //    margin far above its error);
@@ -65,6 +70,14 @@ __device__ __forceinline__ void node_window(float dx, float dy, float lim2, floa
     wlen = khi - klo + 1;
     k0 = klo % nth;
     if (k0 < 0) k0 += nth;
+}
+
+// Node-indexed bits (bit k) of a window starting at node k0 as window
+// positions (bit t = k - k0 mod nθ).
+__device__ __forceinline__ uint64_t sure_shift(uint64_t bits, int k0, int nth) {
+    if (k0 == 0) return bits;
+    const uint64_t full = nth == 64 ? ~0ull : ((1ull << nth) - 1);
+    return ((bits >> k0) | (bits << (nth - k0))) & full;
 }
 
 // R_j(ψ)/R0 = a0 + Σ_{m=1..K} αm cos (m+1)ψ + βm sin (m+1)ψ as Re Σ cn z^n,
@@ -206,7 +219,11 @@ __global__ __launch_bounds__(64) void k_forces_fast(const Params<T> p, const Ptr
             if (D2 > reach * reach) continue;
             int k0, wlen;
             node_window((float)dxij, (float)dyij, (float)lim2, (float)D2, inv_dphi, nth, k0, wlen);
-            uint64_t mask = 0;
+            // sure-inside radius: R_j(ψ) ≥ rmin_j exactly, and the evaluated
+            // R_j differs from the exact one by far less than the margin
+            T rin = d.rmin[j] * (T(1) - (sizeof(T) == 8 ? T(1e-12) : T(2e-5))) - (sizeof(T) == 8 ? T(1e-13) : T(1e-6));
+            const T rin2 = rin > T(0) ? rin * rin : T(-1);
+            uint64_t mask = 0, sure = 0;
             int k = k0;
             for (int t = 0; t < wlen; t++) {
                 const T Rk = sR[k][lane];
@@ -216,10 +233,15 @@ __global__ __launch_bounds__(64) void k_forces_fast(const Params<T> p, const Ptr
                     dx = min_image(dx, Lxf);
                     dy = min_image(dy, Lyf);
                 }
-                if (dx * dx + dy * dy <= lim2) mask |= 1ull << t;
+                const T d2 = dx * dx + dy * dy;
+                if (d2 <= lim2) mask |= 1ull << t;
+                if (d2 < rin2) sure |= 1ull << k;
                 if (++k == nth) k = 0;
             }
-            if (!mask) continue;
+            uint64_t hit = sure;
+            mask &= ~(sure_shift(sure, k0, nth));
+            if (!mask) goto count_hits;
+            {
             T lj[NC];
             {
                 T a[W];
@@ -228,7 +250,6 @@ __global__ __launch_bounds__(64) void k_forces_fast(const Params<T> p, const Ptr
                 for (int c = 0; c < NC; c++) lj[c] = a[c];
             }
             const T r0j = d.R0[j];
-            uint64_t hit = 0;
             while (mask) {
                 const int t = __builtin_ctzll(mask);
                 mask &= mask - 1;
@@ -249,6 +270,8 @@ __global__ __launch_bounds__(64) void k_forces_fast(const Params<T> p, const Ptr
                 const T sp = pos ? dy * rs : T(0);
                 if (r0j * contour_horner<T, K>(lj, cp, sp) - dist >= T(0)) hit |= 1ull << kk;
             }
+            }
+        count_hits:
 #pragma unroll
             for (int c = 0; c < (NCLS > 0 ? NCLS : 1); c++) {
                 uint64_t m = cls == c ? hit : 0;

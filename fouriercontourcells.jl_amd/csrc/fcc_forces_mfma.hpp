@@ -1,0 +1,368 @@
+// fcc_forces_mfma.hpp — forces kernel of the mixed-precision path: the contour
+// synthesis [R; R'; R''] = S·lab and the pullback glab = Sᵀ·G run on the
+// gfx950 matrix cores (v_mfma_f32_32x32x16_{f16,bf16}) with a hi+lo split of
+// both operands and FP32 accumulation; body terms and contact run in FP32.
+//
+// Layout: 32 bodies per wave, two lanes per body. Lane l works on body
+// b = l % 32 of the wave and half h = l / 32 of its contour (nodes 20h..20h+19
+// for nθ = 40). The 32x32 MFMA output holds, in lane l, rows
+// 8(r/4) + 4h + r%4 (r = 0..15) of column l % 32 of each 32-row tile; the basis
+// rows are permuted on the host so that the slots σ = 16t + r of lane half h
+// are R (σ < 20), R' (σ < 40) and R'' (σ < 60, with Bending) at its nodes.
+// These registers feed the body terms, the contact pass (through a 5 KB LDS
+// copy of R) and, overwritten by the contour gradients, the pullback as the A
+// operand (row slots of a K-chunk are exactly the lane's 8 rows of that chunk).
+//
+// hi+lo: x = hi + lo with hi = rn(x), lo = rn(x - hi) in the 16-bit type;
+// A·B ≈ Ah·Bh + Ah·Bl + Al·Bh. lab is pre-scaled by 2^10 and G by 2^8 (powers
+// of two, exact) so FP16 parts stay in the normal range.
+#pragma once
+
+#include "fcc_forces_fast.hpp"
+
+namespace fcc {
+
+typedef _Float16 h8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 b8_t __attribute__((ext_vector_type(8)));
+typedef float f16v_t __attribute__((ext_vector_type(16)));
+
+constexpr float SYN_SCALE = 1024.f;  // lab pre-scale
+constexpr float PULL_SCALE = 256.f;  // contour-gradient pre-scale
+constexpr int NTH_M = 40;            // nodes handled by the MFMA kernel
+constexpr int NH = NTH_M / 2;        // nodes per lane half
+
+template <bool BF>
+struct MfmaType;
+template <>
+struct MfmaType<false> {
+    typedef _Float16 s_t;
+    typedef h8_t v_t;
+    static __device__ __forceinline__ f16v_t mfma(v_t a, v_t b, f16v_t c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+    }
+};
+template <>
+struct MfmaType<true> {
+    typedef __bf16 s_t;
+    typedef b8_t v_t;
+    static __device__ __forceinline__ f16v_t mfma(v_t a, v_t b, f16v_t c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+    }
+};
+
+template <bool BF>
+__device__ __forceinline__ void split8(const float (&x)[8], typename MfmaType<BF>::v_t &hi,
+                                       typename MfmaType<BF>::v_t &lo) {
+    typedef typename MfmaType<BF>::s_t s_t;
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        const s_t h = (s_t)x[s];
+        hi[s] = h;
+        lo[s] = (s_t)(x[s] - (float)h);
+    }
+}
+
+template <bool BF>
+__device__ __forceinline__ typename MfmaType<BF>::v_t load_op(const uint16_t *p) {
+    typename MfmaType<BF>::v_t v;
+    const uint4 q = *reinterpret_cast<const uint4 *>(p);
+    __builtin_memcpy(&v, &q, 16);
+    return v;
+}
+
+// Bit-sliced sum of two per-node counters (NPLANE planes each).
+__device__ __forceinline__ void planes_add(uint64_t (&a)[NPLANE], const uint64_t (&b)[NPLANE]) {
+    uint64_t c = 0;
+#pragma unroll
+    for (int q = 0; q < NPLANE; q++) {
+        const uint64_t s = a[q] ^ b[q] ^ c;
+        c = (a[q] & b[q]) | (c & (a[q] ^ b[q]));
+        a[q] = s;
+    }
+}
+
+__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
+    const uint32_t lo = __shfl_xor((uint32_t)v, m), hi = __shfl_xor((uint32_t)(v >> 32), m);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Operand tables (built on the host, 16-bit values):
+//  asyn[(t * 2 + part) * 64 + lane][8]   synthesis A (S rows), part 0 hi / 1 lo
+//  bpull[(c * 2 + v) * 64 + lane][8]     pullback B, v 0 = [Sh | Sl], 1 = [Sh | 0]
+template <int K, int T, int NCLS, bool BF>
+__global__ __launch_bounds__(64) void k_forces_mfma(const Params<float> p, const Ptrs<float> d,
+                                                    const uint16_t *asyn, const uint16_t *bpull, int use_status,
+                                                    int debug) {
+    typedef MfmaType<BF> MT;
+    typedef typename MT::v_t v_t;
+    constexpr int NC = 2 * K + 1;
+    constexpr int W = row_width<NC, float>();
+    __shared__ float sR[NTH_M][32];
+    __shared__ float scs[2][NTH_M];
+    const int lane = threadIdx.x;
+    const int b = lane & 31, h = lane >> 5;
+    for (int k = lane; k < NTH_M; k += 64) {
+        scs[0][k] = d.cs[2 * k];
+        scs[1][k] = d.cs[2 * k + 1];
+    }
+    const int64_t i = (int64_t)blockIdx.x * 32 + b;
+    if (use_status) {
+        if (blockIdx.x == 0 && lane == 0) {
+            int r = d.st->rebuild;
+            d.st->update = r;
+            if (r == 0) d.st->clock += 1;
+        }
+        if (d.st->rebuild != 0) return;  // uniform over the grid
+    }
+    const bool live = i < p.N;
+    const int64_t ii = live ? i : 0;
+
+    // ---- synthesis on MFMA: lane (b, h) supplies lab coefficients 8h..8h+7 of body b
+    f16v_t acc[T];
+    {
+        float x[8];
+        const float4 *row = reinterpret_cast<const float4 *>(d.lab + ii * p.ldc + 8 * h);
+        const float4 u0 = row[0], u1 = row[1];
+        x[0] = u0.x; x[1] = u0.y; x[2] = u0.z; x[3] = u0.w;
+        x[4] = u1.x; x[5] = u1.y; x[6] = u1.z; x[7] = u1.w;
+#pragma unroll
+        for (int s = 0; s < 8; s++) x[s] = (live && 8 * h + s < NC) ? x[s] * SYN_SCALE : 0.f;
+        v_t bh, bl;
+        split8<BF>(x, bh, bl);
+#pragma unroll
+        for (int t = 0; t < T; t++) {
+            const v_t ah = load_op<BF>(asyn + ((t * 2 + 0) * 64 + lane) * 8);
+            const v_t al = load_op<BF>(asyn + ((t * 2 + 1) * 64 + lane) * 8);
+            f16v_t c = {};
+            c = MT::mfma(ah, bh, c);
+            c = MT::mfma(ah, bl, c);
+            c = MT::mfma(al, bh, c);
+            acc[t] = c;
+        }
+    }
+    const float r0 = live ? d.R0[ii] : 1.f;
+    const float unsc = r0 * (1.f / SYN_SCALE);
+    // slot σ = 16t + r → acc[t][r]; R at σ = j, R' at 20 + j, R'' at 40 + j
+#define SLOT(sig) acc[(sig) >> 4][(sig)&15]
+#pragma unroll
+    for (int t = 0; t < T; t++)
+#pragma unroll
+        for (int r = 0; r < 16; r++) acc[t][r] *= unsc;
+
+    const int ti = live ? d.type[ii] : 0;
+    const uint32_t tbit = 1u << ti;
+    const bool need_dd = T == 4;
+    const bool contact = p.ncontact != 0;
+
+    // ---- pass 1: perimeter and bending integrals over both halves
+    float Qp = 0.f, Qb = 0.f;
+#pragma unroll
+    for (int j = 0; j < NH; j++) {
+        const float R = SLOT(j), dR = SLOT(NH + j);
+        Qp += f_sqrt(R * R + dR * dR);
+        if constexpr (T == 4) Qb += term_q<float>(FCC_TERM_BENDING, R, dR, SLOT(2 * NH + j));
+        sR[h * NH + j][b] = R;
+    }
+    Qp += __shfl_xor(Qp, 32);
+    if constexpr (T == 4) Qb += __shfl_xor(Qb, 32);
+    float Wp = 0.f, Wb = 0.f, Et[MAXB];
+#pragma unroll
+    for (int q = 0; q < MAXB; q++) {
+        Et[q] = 0.f;
+        if (q < p.nbody && (p.bmask[q] & tbit)) {
+            float dF;
+            const bool bend = p.bkind[q] == FCC_TERM_BENDING;
+            Et[q] = term_E<float>(p.bkind[q], p.bp0[q], p.bp1[q], (bend ? Qb : Qp) * p.dphi, r0, dF);
+            if (bend)
+                Wb += dF * p.dphi;
+            else
+                Wp += dF * p.dphi;
+        }
+    }
+    float E = 0.f;
+#pragma unroll
+    for (int q = MAXB - 1; q >= 0; q--)
+        if (q < p.nbody) E = Et[q] + E;
+    __syncthreads();  // sR, scs complete
+
+    // ---- contact: lane half h takes neighbour slots q ≡ h (mod 2), full circle
+    uint64_t pl[NCLS][NPLANE];
+#pragma unroll
+    for (int c = 0; c < NCLS; c++)
+#pragma unroll
+        for (int q = 0; q < NPLANE; q++) pl[c][q] = 0;
+    if (contact && live) {
+        const double xi = d.x[ii], yi = d.y[ii];
+        const float rmi = d.rmax[ii];
+        const int pi = d.parent[ii];
+        const float Lxf = (float)p.Lx, Lyf = (float)p.Ly;
+        const bool wrap2 = p.small_box != 0;
+        const float inv_dphi = (float)(1.0 / (double)p.dphi);
+        int cnt = d.ncount[ii];
+        if (cnt > d.cap) cnt = d.cap;
+        for (int q = h; q < cnt; q += 2) {
+            const int j = d.nidx[(int64_t)q * p.N + ii];
+            const int cls = p.ccls[pi == d.parent[j] ? 1 : 0][ti][d.type[j]];
+            if (cls < 0) continue;
+            const float dxij = (float)min_image(xi - d.x[j], p.Lx);
+            const float dyij = (float)min_image(yi - d.y[j], p.Ly);
+            const float lim = d.rmax[j] + p.range;
+            const float lim2 = lim * lim;
+            const float D2 = dxij * dxij + dyij * dyij;
+            const float reach = (rmi + lim) * 1.0001f + 1e-6f;
+            if (D2 > reach * reach) continue;
+            int k0, wlen;
+            node_window(dxij, dyij, lim2, D2, inv_dphi, NTH_M, k0, wlen);
+            float rin = d.rmin[j] * (1.f - 2e-5f) - 1e-6f;
+            const float rin2 = rin > 0.f ? rin * rin : -1.f;
+            uint64_t mask = 0, sure = 0;
+            int k = k0;
+            for (int t = 0; t < wlen; t++) {
+                const float Rk = sR[k][b];
+                float dx = dxij + Rk * scs[0][k];
+                float dy = dyij + Rk * scs[1][k];
+                if (wrap2) {
+                    dx = min_image(dx, Lxf);
+                    dy = min_image(dy, Lyf);
+                }
+                const float d2 = dx * dx + dy * dy;
+                if (d2 <= lim2) mask |= 1ull << t;
+                if (d2 < rin2) sure |= 1ull << k;
+                if (++k == NTH_M) k = 0;
+            }
+            uint64_t hit = sure;
+            mask &= ~(sure_shift(sure, k0, NTH_M));
+            if (mask) {
+                float lj[NC];
+                {
+                    float a[W];
+                    row_load<float, W>(a, d.lab + (int64_t)j * p.ldc);
+#pragma unroll
+                    for (int c = 0; c < NC; c++) lj[c] = a[c];
+                }
+                const float r0j = d.R0[j];
+                while (mask) {
+                    const int t = __builtin_ctzll(mask);
+                    mask &= mask - 1;
+                    int kk = k0 + t;
+                    if (kk >= NTH_M) kk -= NTH_M;
+                    const float Rk = sR[kk][b];
+                    float dx = dxij + Rk * scs[0][kk];
+                    float dy = dyij + Rk * scs[1][kk];
+                    if (wrap2) {
+                        dx = min_image(dx, Lxf);
+                        dy = min_image(dy, Lyf);
+                    }
+                    const float d2 = dx * dx + dy * dy;
+                    const float rs = f_rsqrt(d2);
+                    const bool pos = d2 > 0.f;
+                    const float dist = pos ? d2 * rs : 0.f;
+                    const float cp = pos ? dx * rs : 1.f;
+                    const float sp = pos ? dy * rs : 0.f;
+                    if (r0j * contour_horner<float, K>(lj, cp, sp) - dist >= 0.f) hit |= 1ull << kk;
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < NCLS; c++) {
+                uint64_t m = cls == c ? hit : 0;
+#pragma unroll
+                for (int qq = 0; qq < NPLANE; qq++) {
+                    const uint64_t carry = pl[c][qq] & m;
+                    pl[c][qq] ^= m;
+                    m = carry;
+                }
+            }
+        }
+    }
+    // both lanes of a body: add the partner's counters
+#pragma unroll
+    for (int c = 0; c < NCLS; c++) {
+        uint64_t other[NPLANE];
+#pragma unroll
+        for (int q = 0; q < NPLANE; q++) other[q] = shfl_xor64(pl[c][q], 32);
+        planes_add(pl[c], other);
+    }
+
+    // ---- pass 2: contour gradients of the lane's nodes (in place of R, R', R'')
+    float fx = 0.f, fy = 0.f, tt = 0.f;
+#pragma unroll
+    for (int j = 0; j < NH; j++) {
+        const int k = h * NH + j;
+        const float R = SLOT(j), dR = SLOT(NH + j);
+        const float rs = f_rsqrt(R * R + dR * dR);
+        float GR = Wp * (R * rs), GD = Wp * (dR * rs), GDD = 0.f;
+        if constexpr (T == 4) {
+            float j0, j1, j2;
+            term_J<float>(FCC_TERM_BENDING, R, dR, SLOT(2 * NH + j), j0, j1, j2);
+            GR += Wb * j0;
+            GD += Wb * j1;
+            GDD = Wb * j2;
+        }
+        if (contact) {
+            float g = 0.f;
+#pragma unroll
+            for (int c = 0; c < NCLS; c++) {
+                uint32_t n = 0;
+#pragma unroll
+                for (int qq = 0; qq < NPLANE; qq++) n |= (uint32_t)((pl[c][qq] >> k) & 1u) << qq;
+                g += p.Jcls[c] * (float)n;
+            }
+            const float pp = g * p.dphi;
+            const float c = scs[0][k], s = scs[1][k];
+            GR += pp * R;
+            fx += pp * (R * c + dR * s);
+            fy += pp * (R * s - dR * c);
+            tt += (-pp) * R * dR;
+            if (debug && live) d.P[ii * NTH_M + k] = g;
+        }
+        if (debug && live) {
+            d.GR[ii * NTH_M + k] = GR;
+            d.GD[ii * NTH_M + k] = GD;
+            d.GDD[ii * NTH_M + k] = GDD;
+        }
+        const float sc = live ? PULL_SCALE : 0.f;
+        SLOT(j) = GR * sc;
+        SLOT(NH + j) = GD * sc;
+        if constexpr (T == 4) SLOT(2 * NH + j) = GDD * sc;
+    }
+#pragma unroll
+    for (int sig = NH * (T == 4 ? 3 : 2); sig < 16 * T; sig++) SLOT(sig) = 0.f;
+#undef SLOT
+    fx += __shfl_xor(fx, 32);
+    fy += __shfl_xor(fy, 32);
+    tt += __shfl_xor(tt, 32);
+
+    // ---- pullback on MFMA: glabᵀ (bodies x coefs) = Gᵀ·S over 2T K-chunks of 16 rows
+    f16v_t gacc = {};
+#pragma unroll
+    for (int c = 0; c < 2 * T; c++) {
+        float x[8];
+#pragma unroll
+        for (int s = 0; s < 8; s++) x[s] = acc[c >> 1][8 * (c & 1) + s];
+        v_t ah, al;
+        split8<BF>(x, ah, al);
+        const v_t b1 = load_op<BF>(bpull + ((c * 2 + 0) * 64 + lane) * 8);
+        const v_t b2 = load_op<BF>(bpull + ((c * 2 + 1) * 64 + lane) * 8);
+        gacc = MT::mfma(ah, b1, gacc);
+        gacc = MT::mfma(al, b2, gacc);
+    }
+    // column n of lane l: n < 16 hi·hi + lo·hi of coefficient n, n ≥ 16 hi·lo of n - 16
+#pragma unroll
+    for (int r = 0; r < 16; r++) gacc[r] += __shfl_down(gacc[r], 16);
+    const int n = lane & 31;
+    if (n < NC) {
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int64_t body = (int64_t)blockIdx.x * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
+            if (body < p.N) d.glab[body * p.ldc + n] = gacc[r] * (1.f / PULL_SCALE);
+        }
+    }
+    if (live && h == 0) {
+        d.gx[ii] = fx;
+        d.gy[ii] = fy;
+        d.gth[ii] = tt;
+        d.Ebody[ii] = E;
+    }
+}
+
+}  // namespace fcc

@@ -282,7 +282,8 @@ __device__ __forceinline__ void enforce_shape(T (&c)[2 * KM + 1], int K) {
 
 // lab_frame! (src/geometry.jl:208-227) on registers.
 template <typename T, int KM>
-__device__ __forceinline__ T lab_frame(const T (&c)[2 * KM + 1], T th, T r0, int K, T (&l)[2 * KM + 1]) {
+__device__ __forceinline__ T lab_frame(const T (&c)[2 * KM + 1], T th, T r0, int K, T (&l)[2 * KM + 1],
+                                       T &rmin) {
     T cn[KM + 1], sn[KM + 1];
     harmonics<T, KM>(th, K, cn, sn);
     l[0] = c[0];
@@ -298,6 +299,7 @@ __device__ __forceinline__ T lab_frame(const T (&c)[2 * KM + 1], T th, T r0, int
             bound += sqrt(a * a + b * b);
         }
     }
+    rmin = r0 * (c[0] - (bound - c[0]));
     return r0 * bound;
 }
 
@@ -314,7 +316,9 @@ __global__ __launch_bounds__(256) void k_init(const Params<T> p, const Ptrs<T> d
     else
         enforce_shape<T, T, KM>(c, p.K);
     store_row<T, KM>(c, d.coef + i * p.ldc, p.K);
-    d.rmax[i] = lab_frame<T, KM>(c, d.th[i], d.R0[i], p.K, l);
+    T rmn;
+    d.rmax[i] = lab_frame<T, KM>(c, d.th[i], d.R0[i], p.K, l, rmn);
+    d.rmin[i] = rmn;
     store_row<T, KM>(l, d.lab + i * p.ldc, p.K);
 }
 
@@ -464,13 +468,15 @@ __global__ __launch_bounds__(256) void k_update(const Params<T> p, const Ptrs<T>
     int32_t sx, sy;
     const double xw = wrap(d.x[i] + (double)d0, p.Lx, sx);
     const double yw = wrap(d.y[i] + (double)d1, p.Ly, sy);
-    const T rm = lab_frame<T, KM>(c, thn, r0, K, l);
+    T rmn;
+    const T rm = lab_frame<T, KM>(c, thn, r0, K, l, rmn);
     d.th[i] = thn;
     d.x[i] = xw;
     d.y[i] = yw;
     d.imgx[i] += sx;
     d.imgy[i] += sy;
     d.rmax[i] = rm;
+    d.rmin[i] = rmn;
     if constexpr (EXACT) {
         T a[W], b[W], g[W];
 #pragma unroll

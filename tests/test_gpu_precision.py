@@ -1,0 +1,133 @@
+"""Error of the FP32 and mixed-precision (MFMA hi+lo) paths against the FP64
+oracle on identical input states.
+
+Metric (the contact density is a Heaviside of the penetration depth, so a node
+whose depth is within rounding of zero may flip): nodes whose contact density
+differs from the oracle's are counted as flips and reported separately; on the
+bodies without a flipped node, the per-body relative error of the force
+(gx, gy) and torque is |Δ| / max(|ref|, scale), where scale is the summed
+magnitude of the body's node contributions Σk |Pk| Δφ |nk| (nk the outward
+normal element (R cos + R' sin, R sin - R' cos), resp. |R R'| for the torque):
+the natural error scale of a sum whose terms cancel (a nucleus inside its cell
+feels -2 at every node and a net force ≈ 0). It must stay below 1e-4 (the
+north-star bound). Shape index and shape-mode gradients are checked too.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import fcc_hip as F
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def deformed_state(n=32, maxmode=8, packing=1.0, seed=11, amp=0.03):
+    b, box = F.hexagonal_monolayer(n, n, packing=packing, maxmode=maxmode, seed=seed)
+    rng = np.random.default_rng(seed)
+    N = len(b)
+    b.x = np.mod(b.x + 0.05 * rng.standard_normal(N), box.Lx)
+    b.y = np.mod(b.y + 0.05 * rng.standard_normal(N), box.Ly)
+    b.coef[1:, :] = amp * rng.standard_normal((b.coef.shape[0] - 1, N))
+    b.R0 = np.exp(0.05 * rng.standard_normal(N))
+    return b, box
+
+
+def gradients(b, box, precision, engine=None, energies=None):
+    energies = energies or (F.Tension(0.01), F.Overlap(1.0))
+    sim = F.Simulation(b, box, types=(F.BodyType(v0=0.1, Dr=0.01),), energies=energies,
+                       precision=precision, debug=True, engine=engine)
+    F.refresh(sim)
+    return sim, sim.grad
+
+
+def rel_err(a, ref, scale):
+    return np.abs(a - ref) / np.maximum(np.maximum(np.abs(ref), scale), 1e-300)
+
+
+def force_scales(sim, g):
+    """Σk |Pk| Δφ |nk| and Σk |Pk| Δφ |Rk R'k| per body from the oracle."""
+    geo = sim.geom
+    dphi = 2 * math.pi / sim.ntheta
+    phi = dphi * np.arange(sim.ntheta)[:, None]
+    R, dR = geo.R, geo.dR
+    nx = R * np.cos(phi) + dR * np.sin(phi)
+    ny = R * np.sin(phi) - dR * np.cos(phi)
+    w = np.abs(g.P) * dphi
+    return np.sum(w * np.hypot(nx, ny), axis=0), np.sum(w * np.abs(R * dR), axis=0)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "mixed_fp16", "mixed_bf16"])
+@pytest.mark.parametrize("maxmode", [6, 8])
+def test_force_error_vs_fp64(oracle_engine, precision, maxmode):
+    say("precision", precision, "maxmode", maxmode)
+    b, box = deformed_state(maxmode=maxmode)
+    ref_sim, ref = gradients(b, box, "fp64", engine=oracle_engine)
+    dev_sim, dev = gradients(b, box, precision)
+    flips = dev.P != ref.P
+    nflip = int(np.count_nonzero(flips))
+    nodes_in_contact = int(np.count_nonzero(ref.P))
+    ok = ~np.any(flips, axis=0)
+    fs, ts = force_scales(ref_sim, ref)
+    ef = np.maximum(rel_err(dev.gx, ref.gx, fs), rel_err(dev.gy, ref.gy, fs))[ok]
+    et = rel_err(dev.gth, ref.gth, ts)[ok]
+    ec = np.max(np.abs(dev.gcoef[1:] - ref.gcoef[1:])) / np.max(np.abs(ref.gcoef[1:]))
+    p_dev, p_ref = F.shape_index(dev_sim), F.shape_index(ref_sim)
+    ep = np.max(np.abs(p_dev - p_ref) / p_ref)
+    say(f"  flips {nflip} of {nodes_in_contact} contact nodes; force {ef.max():.2e}, torque {et.max():.2e}, "
+        f"gcoef {ec:.2e}, shape index {ep:.2e}; bodies compared {ok.sum()} of {ok.size}")
+    assert nflip <= max(2, nodes_in_contact // 1000)
+    assert ok.sum() >= 0.99 * ok.size
+    assert ef.max() <= 1e-4
+    assert et.max() <= 1e-4
+    assert ec <= 1e-4
+    assert ep <= 1e-5
+
+
+@pytest.mark.parametrize("precision", ["mixed_fp16", "mixed_bf16"])
+def test_mixed_with_bending_and_types(oracle_engine, precision):
+    # R'' on the matrix cores (Bending) and two coupling classes
+    say("mixed bending", precision)
+    cells, box = F.hexagonal_monolayer(12, 12, packing=0.95, maxmode=8, seed=3)
+    b = F.with_nuclei(cells, ratio=0.4)
+    rng = np.random.default_rng(4)
+    b.coef[1:, :] = 0.03 * rng.standard_normal((b.coef.shape[0] - 1, len(b)))
+    energies = (F.OnTypes(F.Tension(0.01), 1), F.OnTypes(F.PerimeterElasticity(0.1, 3.7), 1),
+                F.OnTypes(F.Bending(0.01), 2),
+                F.Overlap([[1.0, 1.0], [1.0, 1.0]], related=[[0.0, -2.0], [-2.0, 0.0]]))
+    types = (F.BodyType(v0=0.06, Dr=0.01), F.BodyType(Dr=0.01, drag="parent"))
+    ref = F.refresh(F.Simulation(b, box, types=types, energies=energies, debug=True, engine=oracle_engine))
+    dev = F.refresh(F.Simulation(b, box, types=types, energies=energies, debug=True, precision=precision))
+    gr, gd = ref.grad, dev.grad
+    ok = ~np.any(gd.P != gr.P, axis=0)
+    assert ok.sum() >= 0.98 * ok.size
+    fs, ts = force_scales(ref, gr)
+    ef = np.maximum(rel_err(gd.gx, gr.gx, fs), rel_err(gd.gy, gr.gy, fs))[ok]
+    ec = np.max(np.abs(gd.gcoef[1:] - gr.gcoef[1:])) / np.max(np.abs(gr.gcoef[1:]))
+    eE = np.max(np.abs(gd.Ebody - gr.Ebody) / np.abs(gr.Ebody))
+    say(f"  force {ef.max():.2e}, gcoef {ec:.2e}, energy {eE:.2e}")
+    assert ef.max() <= 1e-4
+    assert ec <= 1e-4
+    assert eE <= 1e-5
+
+
+@pytest.mark.parametrize("precision", ["mixed_fp16", "fp32"])
+def test_mixed_graph_equals_eager(precision):
+    say("mixed graph vs eager", precision)
+    def mk():
+        b, box = F.hexagonal_monolayer(16, 16, packing=1.0, maxmode=8, seed=5)
+        return F.Simulation(b, box, types=(F.BodyType(v0=2.0, Dr=0.5),),
+                            energies=(F.Tension(0.01), F.Overlap(1.0)), skin=0.05, precision=precision)
+    a, c = mk(), mk()
+    F.step(a, 120)
+    F.run(c, 120)
+    ba, bc = a.bodies, c.bodies
+    assert a.counters().builds > 5
+    assert np.array_equal(ba.x, bc.x) and np.array_equal(ba.coef, bc.coef)
+    assert a.counters().clock == c.counters().clock == 120
