@@ -73,6 +73,19 @@ def calibrate_copy(nbytes=1 << 30, reps=5):
     print(json.dumps({"calibrate_copy_bytes": nbytes, "reps": reps}), flush=True)
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (profiles/timestep_N262144_hbm_traffic.txt), or None."""
+    path = os.path.join(ROOT, "profiles", "timestep_N262144_hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 3 and f[0] == kernel:
+            return int(f[1]) + int(f[2])
+    return None
+
+
 def contact_pairs(sim):
     """(tested, in-range) (node, neighbour) pairs of the current state: the
     reference tests every listed neighbour at every node and evaluates R_j(ψ)
@@ -172,6 +185,7 @@ def main():
         ps.close()
 
     flops = forces_flops(sim, tested, inr)
+    fkernel = "k_forces_mfma" if args.precision.startswith("mixed") else "k_forces_fast"
     achieved = flops / (fms * 1e-3) / 1e12
     upd_bytes = 340 * sim.N  # per body-step, 188 B read + 152 B written
     print(f"[bench] forces {fms * 1e3:.1f} us, update {ums * 1e3:.1f} us, pairs tested {tested}, "
@@ -198,10 +212,12 @@ def main():
                    "precision": args.precision, "N": sim.N},
         "plain_fp32_value": plain,
         "mixed_over_plain": (value / plain) if plain else None,
-        "roofline": {"kernel": "k_forces", "bound": "mfma", "achieved": achieved,
+        "roofline": {"kernel": fkernel, "bound": "mfma", "achieved": achieved,
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
-                     "traffic": None, "launch_ms": fms, "flops_per_launch": flops},
+                     "traffic": measured_traffic(fkernel) if sim.N == 262144 else None,
+                     "launch_ms": fms, "flops_per_launch": flops},
         "roofline_update": {"kernel": "k_update", "bound": "hbm",
+                            "traffic": measured_traffic("k_update") if sim.N == 262144 else None,
                             "achieved": upd_bytes / (ums * 1e-3) / 1e9, "peak": PEAK_HBM_GBS,
                             "unit": "GB/s", "frac": upd_bytes / (ums * 1e-3) / 1e9 / PEAK_HBM_GBS,
                             "launch_ms": ums},

@@ -1,0 +1,217 @@
+# FourierContourCellsHIPExt — package extension of FourierContourCells.jl that
+# runs the time step on an AMD MI355X through libfcc_hip.so (C ABI of
+# include/fcc.h). Loaded through the reference's weak-dependency mechanism
+# (Project.toml [weakdeps]/[extensions], like FourierContourCellsCUDAExt): the
+# weak dependency `FCCHIP` is a tiny package that provides `HIPBackend` and the
+# path of the built library.
+#
+#   using FourierContourCells, FCCHIP
+#   sim = Simulation(bodies, box, HIPBackend(); types, energies, dt, T = Float32)
+#   run!(sim, 10_000)
+#   p = shape_index(sim)
+#
+# Terms or sizes without a native kernel (user BodyTerms, LinearRepulsion, bond
+# terms, nθ > 64, more than 4 body types) make `fcc_create` return
+# FCC_ERR_UNSUPPORTED; the extension then builds the reference Simulation on
+# the CPU backend instead (the north star's fallback to the reference path).
+module FourierContourCellsHIPExt
+
+using FourierContourCells
+using FourierContourCells: Bodies, PeriodicBox, BodyType, EnergyTerm, OnTypes, Tension,
+    PerimeterElasticity, Overlap, is_body_term, is_contact_term, is_bond_term, check_parents,
+    nbodies, to_host
+import FourierContourCells: step!, run!, compute_gradients!, update_neighbors!, shape_index,
+    perimeters, areas, positions, unwrapped_positions, body_energy
+using FCCHIP: HIPBackend, libfcc_hip, Bending
+
+const FCC_OK = Cint(0)
+const FCC_ERR_UNSUPPORTED = Cint(-3)
+const PRECISION = Dict(Float64 => Int32(0), Float32 => Int32(1), :mixed => Int32(2), :mixed_bf16 => Int32(3))
+
+# ---- C structs (field for field as in include/fcc.h)
+
+struct FccBodyType
+    v0::Cdouble; Dr::Cdouble; Dt::Cdouble; mur::Cdouble; mutheta::Cdouble; muab::Cdouble
+    parent_drag::Int32; reserved::Int32
+end
+
+struct FccTerm
+    kind::Int32; n_on::Int32; on_types::NTuple{4, Int32}; p::NTuple{2, Cdouble}
+    J::NTuple{16, Cdouble}; Jrel::NTuple{16, Cdouble}
+end
+
+struct FccConfig
+    n_bodies::Int64; n_modes::Int32; n_theta::Int32; n_types::Int32; n_terms::Int32
+    types::NTuple{4, FccBodyType}; terms::NTuple{8, FccTerm}
+    dt::Cdouble; Lx::Cdouble; Ly::Cdouble; skin::Cdouble; seed::UInt64
+    precision::Int32; flags::Int32
+end
+
+struct FccBodies
+    x::Ptr{Cdouble}; y::Ptr{Cdouble}; theta::Ptr{Cdouble}; imgx::Ptr{Int32}; imgy::Ptr{Int32}
+    coef::Ptr{Cdouble}; R0::Ptr{Cdouble}; type::Ptr{Int32}; parent::Ptr{Int32}
+end
+
+lasterror() = unsafe_string(ccall((:fcc_last_error, libfcc_hip), Cstring, ()))
+check(rc) = rc == FCC_OK || error("libfcc_hip: ", lasterror())
+
+# ---- term and type conversion
+
+const EMPTY_TERM = FccTerm(0, 0, (0, 0, 0, 0), (0.0, 0.0), ntuple(_ -> 0.0, 16), ntuple(_ -> 0.0, 16))
+
+unwrap(t::OnTypes) = (t.term, Tuple(Int32.(t.types)))
+unwrap(t) = (t, ())
+
+table(J, nt) = ntuple(16) do q
+    a, b = divrem(q - 1, 4)
+    size(J, 1) == 1 ? J[1, 1] : (a < nt && b < nt ? J[a + 1, b + 1] : 0.0)
+end
+
+function fcc_term(t)
+    base, on = unwrap(t)
+    ont = ntuple(q -> q <= length(on) ? on[q] : Int32(0), 4)
+    if base isa Tension
+        return FccTerm(1, length(on), ont, (base.η, 0.0), ntuple(_ -> 0.0, 16), ntuple(_ -> 0.0, 16))
+    elseif base isa PerimeterElasticity
+        return FccTerm(2, length(on), ont, (base.λ, base.p0), ntuple(_ -> 0.0, 16), ntuple(_ -> 0.0, 16))
+    elseif base isa Bending
+        return FccTerm(3, length(on), ont, (base.κb, 0.0), ntuple(_ -> 0.0, 16), ntuple(_ -> 0.0, 16))
+    elseif base isa Overlap
+        nt = size(base.J, 1)
+        return FccTerm(16, length(on), ont, (0.0, 0.0), table(Float64.(base.J), nt), table(Float64.(base.Jrelated), nt))
+    end
+    return nothing  # no native kernel
+end
+
+fcc_type(b::BodyType) = FccBodyType(b.v0, b.Dr, b.Dt, b.μr, b.μθ, b.μab, b.parent_drag ? 1 : 0, 0)
+
+# ---- simulation handle
+
+mutable struct HIPSimulation{T}
+    handle::Ptr{Cvoid}
+    host::Bodies               # host mirror of the body state (copied in/out)
+    box::PeriodicBox{Float64}
+    nθ::Int
+    dt::Float64
+end
+
+"""
+    Simulation(bodies, box, ::HIPBackend; types, energies, dt = 0.005, nθ = 40,
+               T = Float64, seed = 1, skin = 0.3)
+
+`FourierContourCells.Simulation` on the MI355X. `T` is Float64, Float32,
+`:mixed` (FP16 hi+lo matrix-core contractions) or `:mixed_bf16`.
+"""
+function FourierContourCells.Simulation(bodies::Bodies, box::PeriodicBox, ::HIPBackend; types,
+        energies = (), bonds = (), dt = 0.005, nθ::Integer = 40, T = Float64, seed::Integer = 1,
+        skin::Real = 0.3, kwargs...)
+    types = map(BodyType{Float64}, Tuple(types))
+    energies = Tuple(energies)
+    all(t -> is_body_term(t) || is_contact_term(t) || is_bond_term(t), energies) ||
+        throw(ArgumentError("energy terms must be BodyTerms, ContactTerms or BondTerms"))
+    host = to_host(bodies)
+    maximum(host.type) <= length(types) || throw(ArgumentError("body type without BodyType entry"))
+    host = Bodies(map(f -> f isa AbstractArray{<:AbstractFloat} ? Float64.(f) : f,
+        (getfield(host, f) for f in fieldnames(Bodies)))...)
+    check_parents(host, types)
+    terms = map(fcc_term, energies)
+    native = isempty(bonds) && all(!isnothing, terms) && length(types) <= 4 && length(terms) <= 8
+    if !native
+        @warn "no native kernel for this configuration: using the reference path" maxlog = 1
+        return FourierContourCells.Simulation(bodies, box; types, energies, bonds, dt, nθ,
+            T = T isa Type ? T : Float32, seed, skin, kwargs...)
+    end
+    K = size(host.coef, 1) ÷ 2
+    tt = ntuple(q -> q <= length(types) ? fcc_type(types[q]) : fcc_type(BodyType()), 4)
+    te = ntuple(q -> q <= length(terms) ? terms[q] : EMPTY_TERM, 8)
+    cfg = Ref(FccConfig(nbodies(host), K, nθ, length(types), length(terms), tt, te, dt, box.Lx, box.Ly,
+        skin, UInt64(seed), PRECISION[T], 0))
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    GC.@preserve host begin
+        b = Ref(bodies_struct(host))
+        rc = ccall((:fcc_create, libfcc_hip), Cint, (Ref{FccConfig}, Ref{FccBodies}, Ref{Ptr{Cvoid}}),
+            cfg, b, h)
+    end
+    if rc == FCC_ERR_UNSUPPORTED
+        @warn "libfcc_hip: $(lasterror()); using the reference path" maxlog = 1
+        return FourierContourCells.Simulation(bodies, box; types, energies, dt, nθ, seed, skin)
+    end
+    check(rc)
+    sim = HIPSimulation{T}(h[], host, PeriodicBox{Float64}(box.Lx, box.Ly), nθ, dt)
+    finalizer(s -> ccall((:fcc_destroy, libfcc_hip), Cint, (Ptr{Cvoid},), s.handle), sim)
+    return sim
+end
+
+bodies_struct(b::Bodies) = FccBodies(pointer(b.x), pointer(b.y), pointer(b.θ), pointer(b.imgx),
+    pointer(b.imgy), pointer(b.coef), pointer(b.R0), pointer(b.type), pointer(b.parent))
+
+function sync_host!(sim::HIPSimulation)
+    GC.@preserve sim begin
+        b = Ref(bodies_struct(sim.host))
+        check(ccall((:fcc_get_bodies, libfcc_hip), Cint, (Ptr{Cvoid}, Ref{FccBodies}), sim.handle, b))
+    end
+    return sim.host
+end
+
+FourierContourCells.nbodies(sim::HIPSimulation) = nbodies(sim.host)
+
+# step!(sim, n) — src/simulation.jl:165-173
+step!(sim::HIPSimulation, n::Integer = 1) =
+    (check(ccall((:fcc_step, libfcc_hip), Cint, (Ptr{Cvoid}, Int64), sim.handle, n)); sim)
+
+# run!(sim, n; every, callback, graph) — src/simulation.jl:220-235 with the
+# graph replay of ext/FourierContourCellsCUDAExt.jl:144-180
+function run!(sim::HIPSimulation, nsteps::Integer; every::Integer = 0, callback = nothing, graph::Bool = true)
+    adv(n) = graph ? check(ccall((:fcc_run, libfcc_hip), Cint, (Ptr{Cvoid}, Int64), sim.handle, n)) : step!(sim, n)
+    if callback === nothing || every <= 0
+        adv(nsteps)
+    else
+        done = 0
+        while done < nsteps
+            n = min(every, nsteps - done)
+            adv(n)
+            done += n
+            callback(sim)
+        end
+    end
+    return sim
+end
+
+update_neighbors!(sim::HIPSimulation; force::Bool = false) =
+    (check(ccall((:fcc_update_neighbors, libfcc_hip), Cint, (Ptr{Cvoid}, Int32), sim.handle, force)); sim)
+
+function compute_gradients!(sim::HIPSimulation; frame::Bool = true)
+    check(ccall((:fcc_compute_gradients, libfcc_hip), Cint, (Ptr{Cvoid}, Int32), sim.handle, frame))
+    N, nc = nbodies(sim), size(sim.host.coef, 1)
+    gx, gy, gθ, E = zeros(N), zeros(N), zeros(N), zeros(N)
+    glab, gcoef = zeros(nc, N), zeros(nc, N)
+    check(ccall((:fcc_get_gradients, libfcc_hip), Cint,
+        (Ptr{Cvoid}, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}),
+        sim.handle, gx, gy, gθ, glab, gcoef, E))
+    return (; gx, gy, gθ, glab, gcoef, Ebody = E)
+end
+
+function observables(sim::HIPSimulation)
+    N = nbodies(sim)
+    l, a, p = zeros(N), zeros(N), zeros(N)
+    check(ccall((:fcc_shape_index, libfcc_hip), Cint, (Ptr{Cvoid}, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}),
+        sim.handle, l, a, p))
+    return l, a, p
+end
+
+select(v, sim, ::Nothing) = v
+select(v, sim, type::Integer) = v[sim.host.type .== type]
+
+# src/observables.jl:6-18
+perimeters(sim::HIPSimulation; type = nothing) = select(observables(sim)[1], sim, type)
+areas(sim::HIPSimulation; type = nothing) = select(observables(sim)[2], sim, type)
+shape_index(sim::HIPSimulation; type = nothing) = select(observables(sim)[3], sim, type)
+
+positions(sim::HIPSimulation) = (b = sync_host!(sim); permutedims(hcat(b.x, b.y)))
+function unwrapped_positions(sim::HIPSimulation)
+    b = sync_host!(sim)
+    return permutedims(hcat(b.x .+ b.imgx .* sim.box.Lx, b.y .+ b.imgy .* sim.box.Ly))
+end
+body_energy(sim::HIPSimulation) = sum(compute_gradients!(sim).Ebody)
+
+end

