@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Body-steps/s of the FourierContourCells time step on one MI355X.
 
-Workload (the north-star configuration): hexagonal_monolayer(512, 512;
+Workload (mixed precision at N = 262144): hexagonal_monolayer(512, 512;
 packing = 1.0), N = 262144 circles, shape modes n = 2..8 (ncoef 15), nθ = 40,
 BodyType(v0 = 0.1, Dr = 0.01, μab = 0.1), Tension(0.01) + Overlap(1.0),
 dt = 0.005, Philox seed 1234, Verlet skin 0.3. A step is one pass of the time
@@ -156,8 +156,8 @@ def main():
                     help="only run a 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: the time step has no multi-GPU path in this build "
-              "(see DESIGN.md for how it would shard); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the time step has no multi-GPU path "
+              "(one simulation runs on one device); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy()

@@ -342,6 +342,7 @@ struct Sim final : fcc_sim {
         HIPCHK(hipMemsetAsync(d.gth, 0, sizeof(T) * N, stream));
         HIPCHK(hipMemsetAsync(d.Ebody, 0, sizeof(T) * N, stream));
         HIPCHK(hipMemsetAsync(d.st, 0, sizeof(DevStatus), stream));
+        hipLaunchKernelGGL(k_set_target, 1, 1, 0, stream, d.st, INT64_MAX);
         if (debug) {
             if ((rc = alloc(&d.P, N * nth)) || (rc = alloc(&d.GR, N * nth)) || (rc = alloc(&d.GD, N * nth)) ||
                 (rc = alloc(&d.GDD, N * nth)))

@@ -13,14 +13,30 @@ namespace fcc {
 constexpr int MAXB = 4;  // native body terms per simulation
 
 // Device run state (src/utils.jl:50-78): `rebuild` is STATUS_REBUILD, set by
-// the fused skin check; `update` is STATUS_UPDATE, the copy of `rebuild`
-// taken at the start of a step, which gates the update kernel; `overflow`
-// holds the largest neighbour count of the last build; `clock` counts the
-// executed steps and is the Philox draw index.
+// the fused skin check; `update` is STATUS_UPDATE, the copy of the freeze
+// condition taken at the start of a step, which gates the update kernel;
+// `overflow` holds the largest neighbour count of the last build; `clock`
+// counts the executed steps and is the Philox draw index. `stop` is raised by
+// the update kernel once `clock` reaches `target` (ensemble runs give every
+// replica its own step budget); a raised `rebuild` or `stop` freezes the
+// following steps.
 struct DevStatus {
-    int32_t rebuild, update, overflow, pad;
+    int32_t rebuild, update, overflow, stop;
     int64_t clock;
+    int64_t target;
 };
+
+// Status prologue of the first kernel of a step: block 0 mirrors the freeze
+// condition into `update` and ticks the clock of an executed step. Returns
+// whether this step is frozen (uniform over the launch).
+__device__ __forceinline__ bool step_frozen(DevStatus *st, int64_t blk, int lane) {
+    const int32_t fr = st->rebuild | st->stop;
+    if (blk == 0 && lane == 0) {
+        st->update = fr;
+        if (fr == 0) st->clock += 1;
+    }
+    return fr != 0;
+}
 
 // Scalar parameters of one simulation, passed by value to every kernel.
 template <typename T>

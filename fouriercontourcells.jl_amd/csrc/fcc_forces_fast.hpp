@@ -123,8 +123,8 @@ __device__ __forceinline__ void synth_k(const FCC_CONST T *S, int nth, int ldc, 
 }
 
 template <typename T, int NTH_, int K, int NCLS>
-__global__ __launch_bounds__(64) void k_forces_fast(const Params<T> p, const Ptrs<T> d, int use_status,
-                                                    int debug) {
+__device__ __forceinline__ void k_forces_fast_impl(const Params<T> &p, const Ptrs<T> &d, int use_status,
+                                                    int debug, int64_t blk) {
     constexpr int NC = 2 * K + 1;
     constexpr int NTHM = NTH_ ? NTH_ : FCC_MAX_THETA;
     __shared__ T sR[NTHM][64];
@@ -136,15 +136,8 @@ __global__ __launch_bounds__(64) void k_forces_fast(const Params<T> p, const Ptr
         scs[1][k] = d.cs[2 * k + 1];
     }
     __syncthreads();
-    const int64_t i = (int64_t)blockIdx.x * 64 + lane;
-    if (use_status) {
-        if (i == 0) {
-            int r = d.st->rebuild;
-            d.st->update = r;
-            if (r == 0) d.st->clock += 1;
-        }
-        if (d.st->rebuild != 0) return;
-    }
+    const int64_t i = (int64_t)blk * 64 + lane;
+    if (use_status && step_frozen(d.st, blk, lane)) return;
     if (i >= p.N) return;
     const FCC_CONST T *S = as_const(d.Sc);
     const FCC_CONST T *CS = as_const(d.cs);
@@ -346,6 +339,19 @@ __global__ __launch_bounds__(64) void k_forces_fast(const Params<T> p, const Ptr
     d.gy[i] = fy;
     d.gth[i] = tt;
     d.Ebody[i] = E;
+}
+
+template <typename T, int NTH_, int K, int NCLS>
+__global__ __launch_bounds__(64) void k_forces_fast(const Params<T> p, const Ptrs<T> d, int use_status,
+                                                    int debug) {
+    k_forces_fast_impl<T, NTH_, K, NCLS>(p, d, use_status, debug, blockIdx.x);
+}
+
+template <typename T, int NTH_, int K, int NCLS>
+__global__ __launch_bounds__(64) void k_forces_fast_ens(const Params<T> *ps, const Ptrs<T> *ds, int use_status,
+                                                    int debug, const int32_t *brep, const int32_t *bfirst) {
+    const int r = brep[blockIdx.x];
+    k_forces_fast_impl<T, NTH_, K, NCLS>(ps[r], ds[r], use_status, debug, (int64_t)blockIdx.x - bfirst[r]);
 }
 
 }  // namespace fcc

@@ -9,9 +9,11 @@
 // 8(r/4) + 4h + r%4 (r = 0..15) of column l % 32 of each 32-row tile; the basis
 // rows are permuted on the host so that the slots σ = 16t + r of lane half h
 // are R (σ < 20), R' (σ < 40) and R'' (σ < 60, with Bending) at its nodes.
-// These registers feed the body terms, the contact pass (through a 5 KB LDS
-// copy of R) and, overwritten by the contour gradients, the pullback as the A
-// operand (row slots of a K-chunk are exactly the lane's 8 rows of that chunk).
+// After the synthesis the lane stores its R, R' (R'') in LDS ([node][body],
+// 5 KB each per wave), which frees the accumulators during the contact pass;
+// pass 2 overwrites them with the contour gradients, which the pullback reads
+// back as its A operand (the row slots of a K-chunk are exactly the lane's 8
+// rows of that chunk).
 //
 // hi+lo: x = hi + lo with hi = rn(x), lo = rn(x - hi) in the 16-bit type;
 // A·B ≈ Ah·Bh + Ah·Bl + Al·Bh. lab is pre-scaled by 2^10 and G by 2^8 (powers
@@ -90,30 +92,24 @@ __device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
 //  asyn[(t * 2 + part) * 64 + lane][8]   synthesis A (S rows), part 0 hi / 1 lo
 //  bpull[(c * 2 + v) * 64 + lane][8]     pullback B, v 0 = [Sh | Sl], 1 = [Sh | 0]
 template <int K, int T, int NCLS, bool BF>
-__global__ __launch_bounds__(64) void k_forces_mfma(const Params<float> p, const Ptrs<float> d,
+__device__ __forceinline__ void k_forces_mfma_impl(const Params<float> &p, const Ptrs<float> &d,
                                                     const uint16_t *asyn, const uint16_t *bpull, int use_status,
-                                                    int debug) {
+                                                    int debug, int64_t blk) {
     typedef MfmaType<BF> MT;
     typedef typename MT::v_t v_t;
     constexpr int NC = 2 * K + 1;
     constexpr int W = row_width<NC, float>();
-    __shared__ float sR[NTH_M][32];
+    __shared__ float sQ[T == 4 ? 3 : 2][NTH_M][32];  // R, R' (R''); then GR, GD (GDD)
     __shared__ float scs[2][NTH_M];
+    float(&sR)[NTH_M][32] = sQ[0];
     const int lane = threadIdx.x;
     const int b = lane & 31, h = lane >> 5;
     for (int k = lane; k < NTH_M; k += 64) {
         scs[0][k] = d.cs[2 * k];
         scs[1][k] = d.cs[2 * k + 1];
     }
-    const int64_t i = (int64_t)blockIdx.x * 32 + b;
-    if (use_status) {
-        if (blockIdx.x == 0 && lane == 0) {
-            int r = d.st->rebuild;
-            d.st->update = r;
-            if (r == 0) d.st->clock += 1;
-        }
-        if (d.st->rebuild != 0) return;  // uniform over the grid
-    }
+    const int64_t i = (int64_t)blk * 32 + b;
+    if (use_status && step_frozen(d.st, blk, lane)) return;  // uniform over the grid
     const bool live = i < p.N;
     const int64_t ii = live ? i : 0;
 
@@ -151,7 +147,6 @@ __global__ __launch_bounds__(64) void k_forces_mfma(const Params<float> p, const
 
     const int ti = live ? d.type[ii] : 0;
     const uint32_t tbit = 1u << ti;
-    const bool need_dd = T == 4;
     const bool contact = p.ncontact != 0;
 
     // ---- pass 1: perimeter and bending integrals over both halves
@@ -161,8 +156,11 @@ __global__ __launch_bounds__(64) void k_forces_mfma(const Params<float> p, const
         const float R = SLOT(j), dR = SLOT(NH + j);
         Qp += f_sqrt(R * R + dR * dR);
         if constexpr (T == 4) Qb += term_q<float>(FCC_TERM_BENDING, R, dR, SLOT(2 * NH + j));
-        sR[h * NH + j][b] = R;
+        sQ[0][h * NH + j][b] = R;
+        sQ[1][h * NH + j][b] = dR;
+        if constexpr (T == 4) sQ[2][h * NH + j][b] = SLOT(2 * NH + j);
     }
+#undef SLOT
     Qp += __shfl_xor(Qp, 32);
     if constexpr (T == 4) Qb += __shfl_xor(Qb, 32);
     float Wp = 0.f, Wb = 0.f, Et[MAXB];
@@ -283,17 +281,17 @@ __global__ __launch_bounds__(64) void k_forces_mfma(const Params<float> p, const
         planes_add(pl[c], other);
     }
 
-    // ---- pass 2: contour gradients of the lane's nodes (in place of R, R', R'')
+    // ---- pass 2: contour gradients of the lane's nodes (in place of R, R', R'' in LDS)
     float fx = 0.f, fy = 0.f, tt = 0.f;
-#pragma unroll
+#pragma unroll 4
     for (int j = 0; j < NH; j++) {
         const int k = h * NH + j;
-        const float R = SLOT(j), dR = SLOT(NH + j);
+        const float R = sQ[0][k][b], dR = sQ[1][k][b];
         const float rs = f_rsqrt(R * R + dR * dR);
         float GR = Wp * (R * rs), GD = Wp * (dR * rs), GDD = 0.f;
         if constexpr (T == 4) {
             float j0, j1, j2;
-            term_J<float>(FCC_TERM_BENDING, R, dR, SLOT(2 * NH + j), j0, j1, j2);
+            term_J<float>(FCC_TERM_BENDING, R, dR, sQ[T == 4 ? 2 : 0][k][b], j0, j1, j2);
             GR += Wb * j0;
             GD += Wb * j1;
             GDD = Wb * j2;
@@ -321,13 +319,10 @@ __global__ __launch_bounds__(64) void k_forces_mfma(const Params<float> p, const
             d.GDD[ii * NTH_M + k] = GDD;
         }
         const float sc = live ? PULL_SCALE : 0.f;
-        SLOT(j) = GR * sc;
-        SLOT(NH + j) = GD * sc;
-        if constexpr (T == 4) SLOT(2 * NH + j) = GDD * sc;
+        sQ[0][k][b] = GR * sc;
+        sQ[1][k][b] = GD * sc;
+        if constexpr (T == 4) sQ[T == 4 ? 2 : 0][k][b] = GDD * sc;
     }
-#pragma unroll
-    for (int sig = NH * (T == 4 ? 3 : 2); sig < 16 * T; sig++) SLOT(sig) = 0.f;
-#undef SLOT
     fx += __shfl_xor(fx, 32);
     fy += __shfl_xor(fy, 32);
     tt += __shfl_xor(tt, 32);
@@ -338,7 +333,11 @@ __global__ __launch_bounds__(64) void k_forces_mfma(const Params<float> p, const
     for (int c = 0; c < 2 * T; c++) {
         float x[8];
 #pragma unroll
-        for (int s = 0; s < 8; s++) x[s] = acc[c >> 1][8 * (c & 1) + s];
+        for (int s = 0; s < 8; s++) {
+            const int sig = 16 * (c >> 1) + 8 * (c & 1) + s;  // slot σ = 16t + r
+            const int qn = sig / NH, jn = sig % NH;
+            x[s] = qn < (T == 4 ? 3 : 2) ? sQ[qn < (T == 4 ? 3 : 2) ? qn : 0][h * NH + jn][b] : 0.f;
+        }
         v_t ah, al;
         split8<BF>(x, ah, al);
         const v_t b1 = load_op<BF>(bpull + ((c * 2 + 0) * 64 + lane) * 8);
@@ -353,7 +352,7 @@ __global__ __launch_bounds__(64) void k_forces_mfma(const Params<float> p, const
     if (n < NC) {
 #pragma unroll
         for (int r = 0; r < 16; r++) {
-            const int64_t body = (int64_t)blockIdx.x * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
+            const int64_t body = (int64_t)blk * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
             if (body < p.N) d.glab[body * p.ldc + n] = gacc[r] * (1.f / PULL_SCALE);
         }
     }
@@ -363,6 +362,21 @@ __global__ __launch_bounds__(64) void k_forces_mfma(const Params<float> p, const
         d.gth[ii] = tt;
         d.Ebody[ii] = E;
     }
+}
+
+template <int K, int T, int NCLS, bool BF>
+__global__ __launch_bounds__(64) void k_forces_mfma(const Params<float> p, const Ptrs<float> d,
+                                                    const uint16_t *asyn, const uint16_t *bpull, int use_status,
+                                                    int debug) {
+    k_forces_mfma_impl<K, T, NCLS, BF>(p, d, asyn, bpull, use_status, debug, blockIdx.x);
+}
+
+template <int K, int T, int NCLS, bool BF>
+__global__ __launch_bounds__(64) void k_forces_mfma_ens(const Params<float> *ps, const Ptrs<float> *ds,
+                                                    const uint16_t *asyn, const uint16_t *bpull, int use_status,
+                                                    int debug, const int32_t *brep, const int32_t *bfirst) {
+    const int r = brep[blockIdx.x];
+    k_forces_mfma_impl<K, T, NCLS, BF>(ps[r], ds[r], asyn, bpull, use_status, debug, (int64_t)blockIdx.x - bfirst[r]);
 }
 
 }  // namespace fcc

@@ -40,8 +40,8 @@ __device__ __forceinline__ void synth(const FCC_CONST T *S, int nth, int k, cons
 // gθ and Ebody. With use_status it also performs the status mirror and the
 // clock tick of a step (src/utils.jl:50-78).
 template <typename T, int NTH_, int KM>
-__global__ __launch_bounds__(64) void k_forces(const Params<T> p, const Ptrs<T> d, int use_status,
-                                               int debug) {
+__device__ __forceinline__ void k_forces_impl(const Params<T> &p, const Ptrs<T> &d, int use_status,
+                                               int debug, int64_t blk) {
     constexpr int NTHM = NTH_ ? NTH_ : FCC_MAX_THETA;
     constexpr int NCM = 2 * KM + 1;
     __shared__ T sR[NTHM][64];
@@ -55,15 +55,8 @@ __global__ __launch_bounds__(64) void k_forces(const Params<T> p, const Ptrs<T> 
         scs[1][k] = d.cs[2 * k + 1];
     }
     __syncthreads();
-    const int64_t i = (int64_t)blockIdx.x * 64 + lane;
-    if (use_status) {
-        if (i == 0) {
-            int r = d.st->rebuild;
-            d.st->update = r;
-            if (r == 0) d.st->clock += 1;
-        }
-        if (d.st->rebuild != 0) return;
-    }
+    const int64_t i = (int64_t)blk * 64 + lane;
+    if (use_status && step_frozen(d.st, blk, lane)) return;
     if (i >= p.N) return;
     const FCC_CONST T *S = as_const(d.S);
     T lab[NCM];
@@ -226,6 +219,19 @@ __global__ __launch_bounds__(64) void k_forces(const Params<T> p, const Ptrs<T> 
     d.Ebody[i] = E;
 }
 
+template <typename T, int NTH_, int KM>
+__global__ __launch_bounds__(64) void k_forces(const Params<T> p, const Ptrs<T> d, int use_status,
+                                               int debug) {
+    k_forces_impl<T, NTH_, KM>(p, d, use_status, debug, blockIdx.x);
+}
+
+template <typename T, int NTH_, int KM>
+__global__ __launch_bounds__(64) void k_forces_ens(const Params<T> *ps, const Ptrs<T> *ds, int use_status,
+                                               int debug, const int32_t *brep, const int32_t *bfirst) {
+    const int r = brep[blockIdx.x];
+    k_forces_impl<T, NTH_, KM>(ps[r], ds[r], use_status, debug, (int64_t)blockIdx.x - bfirst[r]);
+}
+
 // ----------------------------------------------------------- per-body math
 
 // (cos nθ, sin nθ) for n = 2..K+1: direct in FP64, by rotation from (cos θ,
@@ -382,8 +388,8 @@ __device__ __forceinline__ void substrate_disp(const Params<T> &p, const Ptrs<T>
 // displacement_kernel! (src/dynamics.jl:258-277), used when some body type
 // has parent drag: a child needs its parent's θ before the update.
 template <typename T>
-__global__ __launch_bounds__(256) void k_displacement(const Params<T> p, const Ptrs<T> d) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+__device__ __forceinline__ void k_displacement_impl(const Params<T> &p, const Ptrs<T> &d, int64_t blk) {
+    const int64_t i = (int64_t)blk * 256 + threadIdx.x;
     if (i >= p.N || d.st->update != 0) return;
     const uint64_t draw = (uint64_t)(d.st->clock - 1);
     const int ti = d.type[i];
@@ -408,16 +414,27 @@ __global__ __launch_bounds__(256) void k_displacement(const Params<T> p, const P
     d.disp[3 * i + 2] = sqrt(T(2) * p.Dr[ti] * p.dt) * x3;
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void k_displacement(const Params<T> p, const Ptrs<T> d) {
+    k_displacement_impl<T>(p, d, blockIdx.x);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_displacement_ens(const Params<T> *ps, const Ptrs<T> *ds, const int32_t *brep, const int32_t *bfirst) {
+    const int r = brep[blockIdx.x];
+    k_displacement_impl<T>(ps[r], ds[r], (int64_t)blockIdx.x - bfirst[r]);
+}
+
 // Fused Euler–Maruyama update (src/dynamics.jl:310-333) with the
 // displacement and Philox noise of src/dynamics.jl:84-99,258-277 (bodies
 // without parent drag), the body-frame gradient, the shape constraint (FP64
 // when F64S), position wrapping in FP64, the lab frame of the new state and
 // the Verlet skin check (src/neighbors.jl:409-416).
 template <typename T, int KM, bool EXACT, bool F64S>
-__global__ __launch_bounds__(256) void k_update(const Params<T> p, const Ptrs<T> d, int use_disp, int debug) {
+__device__ __forceinline__ void k_update_impl(const Params<T> &p, const Ptrs<T> &d, int use_disp, int debug, int64_t blk) {
     constexpr int NCM = 2 * KM + 1;
     constexpr int W = row_width<NCM, T>();
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t i = (int64_t)blk * 256 + threadIdx.x;
     if (i >= p.N || d.st->update != 0) return;
     const uint64_t draw = (uint64_t)(d.st->clock - 1);
     const int K = EXACT ? KM : p.K;
@@ -493,6 +510,7 @@ __global__ __launch_bounds__(256) void k_update(const Params<T> p, const Ptrs<T>
         store_row<T, KM>(l, d.lab + i * p.ldc, K);
         if (debug) store_row<T, KM>(gc, d.gcoef + i * p.ldc, K);
     }
+    if (i == 0) d.st->stop = d.st->clock >= d.st->target ? 1 : 0;
     if (p.skin_check) {
         const double dx = min_image(xw - d.x0[i], p.Lx);
         const double dy = min_image(yw - d.y0[i], p.Ly);
@@ -501,6 +519,17 @@ __global__ __launch_bounds__(256) void k_update(const Params<T> p, const Ptrs<T>
         if (2.0 * delta > (double)p.skin)
             __hip_atomic_store(&d.st->rebuild, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+}
+
+template <typename T, int KM, bool EXACT, bool F64S>
+__global__ __launch_bounds__(256) void k_update(const Params<T> p, const Ptrs<T> d, int use_disp, int debug) {
+    k_update_impl<T, KM, EXACT, F64S>(p, d, use_disp, debug, blockIdx.x);
+}
+
+template <typename T, int KM, bool EXACT, bool F64S>
+__global__ __launch_bounds__(256) void k_update_ens(const Params<T> *ps, const Ptrs<T> *ds, int use_disp, int debug, const int32_t *brep, const int32_t *bfirst) {
+    const int r = brep[blockIdx.x];
+    k_update_impl<T, KM, EXACT, F64S>(ps[r], ds[r], use_disp, debug, (int64_t)blockIdx.x - bfirst[r]);
 }
 
 // ------------------------------------------------------- cell / neighbours
@@ -633,6 +662,12 @@ __global__ void k_clear_status(DevStatus *st) {
     st->rebuild = 0;
     st->update = 0;
     st->overflow = 0;
+}
+
+// Step budget of an ensemble run: freeze once `clock` reaches `target`.
+__global__ void k_set_target(DevStatus *st, int64_t target) {
+    st->target = target;
+    st->stop = st->clock >= target ? 1 : 0;
 }
 
 template <typename T>

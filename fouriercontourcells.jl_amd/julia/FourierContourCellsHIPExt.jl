@@ -13,7 +13,7 @@
 # Terms or sizes without a native kernel (user BodyTerms, LinearRepulsion, bond
 # terms, nθ > 64, more than 4 body types) make `fcc_create` return
 # FCC_ERR_UNSUPPORTED; the extension then builds the reference Simulation on
-# the CPU backend instead (the north star's fallback to the reference path).
+# the CPU backend instead (a user-defined term runs on the reference path).
 module FourierContourCellsHIPExt
 
 using FourierContourCells

@@ -9,8 +9,9 @@ bodies without a flipped node, the per-body relative error of the force
 magnitude of the body's node contributions Σk |Pk| Δφ |nk| (nk the outward
 normal element (R cos + R' sin, R sin - R' cos), resp. |R R'| for the torque):
 the natural error scale of a sum whose terms cancel (a nucleus inside its cell
-feels -2 at every node and a net force ≈ 0). It must stay below 1e-4 (the
-north-star bound). Shape index and shape-mode gradients are checked too.
+feels -2 at every node and a net force ≈ 0). It must stay below 1e-4, the
+force tolerance of the mixed path. Shape index and shape-mode gradients are
+checked too.
 """
 import math
 import sys
