@@ -158,6 +158,7 @@ static bool row_phys(int t, int w, int T, int &q, int &k) {
 // ------------------------------------------------------------- simulation
 
 struct fcc_sim {
+    bool borrowed = false;  // owned by an ensemble
     virtual ~fcc_sim() {}
     virtual int set_bodies(const fcc_bodies *b) = 0;
     virtual int get_bodies(fcc_bodies *b) = 0;
@@ -184,6 +185,7 @@ struct Sim final : fcc_sim {
     Params<T> p{};
     Ptrs<T> d{};
     hipStream_t stream = nullptr;
+    bool own_stream = true;
     int64_t N = 0;
     int K = 0, nc = 0, ldc = 0, nth = 0, KM = 0;
     bool mixed = false, debug = false, has_contact = false, has_pdrag = false;
@@ -213,7 +215,7 @@ struct Sim final : fcc_sim {
         for (hipEvent_t e : tev)
             if (e) (void)hipEventDestroy(e);
         for (void *q : allocs) (void)hipFree(q);
-        if (stream) (void)hipStreamDestroy(stream);
+        if (stream && own_stream) (void)hipStreamDestroy(stream);
     }
 
     template <typename U>
@@ -232,7 +234,9 @@ struct Sim final : fcc_sim {
         (void)hipFree(q);
     }
 
-    int init(const fcc_config *c, const fcc_bodies *b) {
+    // `ext_stream` / `ext_st`: a stream and status slot owned by an ensemble.
+    int init(const fcc_config *c, const fcc_bodies *b, hipStream_t ext_stream = nullptr,
+             DevStatus *ext_st = nullptr) {
         cfg = *c;
         N = c->n_bodies;
         K = c->n_modes;
@@ -243,7 +247,12 @@ struct Sim final : fcc_sim {
         mixed = c->precision == FCC_MIXED_FP16 || c->precision == FCC_MIXED_BF16;
         debug = (c->flags & FCC_FLAG_DEBUG_BUFFERS) != 0;
         skin = c->skin;
-        HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        if (ext_stream) {
+            stream = ext_stream;
+            own_stream = false;
+        } else {
+            HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        }
 
         p.N = N;
         p.K = K;
@@ -331,7 +340,11 @@ struct Sim final : fcc_sim {
             (rc = alloc(&d.type, N)) || (rc = alloc(&d.parent, N)) || (rc = alloc(&d.lab, N * ldc)) ||
             (rc = alloc(&d.rmax, N)) || (rc = alloc(&d.rmin, N)) || (rc = alloc(&d.glab, N * ldc)) || (rc = alloc(&d.gx, N)) ||
             (rc = alloc(&d.gy, N)) || (rc = alloc(&d.gth, N)) || (rc = alloc(&d.Ebody, N)) ||
-            (rc = alloc(&d.gcoef, N * ldc)) || (rc = alloc(&d.st, 1)))
+            (rc = alloc(&d.gcoef, N * ldc)))
+            return rc;
+        if (ext_st)
+            d.st = ext_st;
+        else if ((rc = alloc(&d.st, 1)))
             return rc;
         HIPCHK(hipMemsetAsync(d.coef, 0, sizeof(T) * N * ldc, stream));
         HIPCHK(hipMemsetAsync(d.lab, 0, sizeof(T) * N * ldc, stream));
@@ -1018,6 +1031,8 @@ struct Sim final : fcc_sim {
     }
 };
 
+#include "fcc_ensemble.hpp"
+
 // ------------------------------------------------------------------ C ABI
 
 extern "C" {
@@ -1049,6 +1064,7 @@ int fcc_create(const fcc_config *cfg, const fcc_bodies *bodies, fcc_sim **out) {
 }
 
 int fcc_destroy(fcc_sim *s) {
+    if (s && s->borrowed) return fail(FCC_ERR_STATE, "replica handle belongs to an ensemble (fcc_ensemble_destroy)");
     delete s;
     return FCC_OK;
 }
@@ -1141,6 +1157,78 @@ int fcc_philox4x32(const uint32_t *ctr, const uint32_t *key, int64_t n, uint32_t
     (void)hipFree(dout);
     HIPCHK(e);
     return FCC_OK;
+}
+
+// ---- ensembles
+
+int fcc_ensemble_create(int32_t n, const fcc_config *cfgs, const fcc_bodies *bodies, fcc_ensemble **out) {
+    if (!out) return fail(FCC_ERR_ARGUMENT, "null output handle");
+    *out = nullptr;
+    if (n < 1 || !cfgs || !bodies) return fail(FCC_ERR_ARGUMENT, "an ensemble needs at least one replica");
+    for (int32_t r = 0; r < n; r++) {
+        int rc = validate(&cfgs[r], &bodies[r]);
+        if (rc) {
+            char msg[sizeof g_err];
+            std::snprintf(msg, sizeof msg, "%s", g_err);
+            return fail(rc, "replica %d: %s", r, msg);
+        }
+        const fcc_config &a = cfgs[0], &c = cfgs[r];
+        if (c.n_modes != a.n_modes || c.n_theta != a.n_theta || c.precision != a.precision || c.flags != a.flags)
+            return fail(FCC_ERR_UNSUPPORTED, "replica %d: n_modes, n_theta, precision and flags must equal replica 0's", r);
+    }
+    fcc_ensemble *e = nullptr;
+    int rc;
+    if (cfgs[0].precision == FCC_FP64) {
+        auto *q = new Ensemble<double>();
+        rc = q->init(n, cfgs, bodies);
+        e = q;
+    } else {
+        auto *q = new Ensemble<float>();
+        rc = q->init(n, cfgs, bodies);
+        e = q;
+    }
+    if (rc) {
+        delete e;
+        return rc;
+    }
+    *out = e;
+    return FCC_OK;
+}
+
+int fcc_ensemble_destroy(fcc_ensemble *e) {
+    delete e;
+    return FCC_OK;
+}
+
+#define FCC_ENS(e) \
+    if (!(e)) return fail(FCC_ERR_ARGUMENT, "null ensemble handle")
+
+int fcc_ensemble_member(fcc_ensemble *e, int32_t r, fcc_sim **member) {
+    FCC_ENS(e);
+    if (!member) return fail(FCC_ERR_ARGUMENT, "null output handle");
+    if (r < 0 || r >= e->size()) return fail(FCC_ERR_ARGUMENT, "replica %d outside 0:%d", r, e->size() - 1);
+    *member = e->member(r);
+    return FCC_OK;
+}
+
+int fcc_ensemble_run(fcc_ensemble *e, int64_t n) {
+    FCC_ENS(e);
+    if (n < 0) return fail(FCC_ERR_ARGUMENT, "steps must be non-negative");
+    return e->run(n);
+}
+
+int fcc_ensemble_run_timed(fcc_ensemble *e, int64_t n, double *fms, double *ums) {
+    FCC_ENS(e);
+    if (n < 0) return fail(FCC_ERR_ARGUMENT, "steps must be non-negative");
+    return e->run_timed(n, fms, ums);
+}
+
+int fcc_ensemble_get_launch(fcc_ensemble *e, fcc_launch *lc) { FCC_ENS(e); return e->get_launch(lc); }
+int fcc_ensemble_set_launch(fcc_ensemble *e, const fcc_launch *lc) { FCC_ENS(e); return e->set_launch(lc); }
+
+int fcc_ensemble_autotune(fcc_ensemble *e, int64_t steps, fcc_launch *best, double *ms) {
+    FCC_ENS(e);
+    return e->autotune(steps, best, ms);
 }
 
 }  // extern "C"

@@ -107,6 +107,25 @@ __device__ __forceinline__ const FCC_CONST T *as_const(const T *p) {
     return (const FCC_CONST T *)p;
 }
 
+// One workgroup of an ensemble launch: replica `rep` (-1 = padding of the
+// grid) and the workgroup's index within that replica's own grid.
+struct Tile {
+    int32_t rep, blk;
+};
+
+// Element r of a table the host writes between launches only, read through
+// the scalar cache (uniform address) word by word.
+template <typename P>
+__device__ __forceinline__ P load_uniform(const P *tab, int64_t r) {
+    static_assert(sizeof(P) % 4 == 0, "table entries are whole words");
+    P out;
+    const FCC_CONST uint32_t *src = (const FCC_CONST uint32_t *)(tab + r);
+    uint32_t *dst = reinterpret_cast<uint32_t *>(&out);
+#pragma unroll
+    for (size_t q = 0; q < sizeof(P) / 4; q++) dst[q] = src[q];
+    return out;
+}
+
 // ------------------------------------------------------------------ Philox
 
 // Philox4x32-10 (Salmon et al., SC'11), src/random.jl:10-36.

@@ -1,0 +1,499 @@
+// fcc_ensemble.hpp — ensembles of independent simulations (host side of
+// fcc_ensemble_* in include/fcc.h; included by fcc_api.hip after Sim).
+//
+// The reference advances independent simulations concurrently, one task and
+// one stream each (README.md:249-261, test/test_cuda.jl:95-104); a small
+// system of a few thousand bodies leaves most of the GPU idle. Here the
+// replicas of an ensemble are ordinary simulations whose buffers stay where
+// each Sim allocated them, sharing one stream and one contiguous array of
+// status slots. A step of all replicas is one launch per kernel: the grid is
+// a table of tiles (replica, workgroup of that replica) and every workgroup
+// reads its replica's Params / Ptrs from device tables, so replicas of any
+// size share one launch. The step is captured once in a hipGraph (the graph
+// reads the tables at run time, so new list capacities or dt need no new
+// capture). A replica whose Verlet skin trips is frozen by its own status
+// flag until the host rebuilds its list between graph chunks, and a
+// per-replica step budget freezes replicas that have finished, so every
+// replica executes exactly the kernels fcc_run would give it alone.
+
+struct fcc_ensemble {
+    virtual ~fcc_ensemble() {}
+    virtual int32_t size() const = 0;
+    virtual fcc_sim *member(int32_t r) = 0;
+    virtual int run(int64_t n) = 0;
+    virtual int run_timed(int64_t n, double *fms, double *ums) = 0;
+    virtual int get_launch(fcc_launch *lc) = 0;
+    virtual int set_launch(const fcc_launch *lc) = 0;
+    virtual int autotune(int64_t steps, fcc_launch *best, double *ms) = 0;
+};
+
+static int check_launch(const fcc_launch *lc) {
+    if (!lc) return fail(FCC_ERR_ARGUMENT, "null launch shape");
+    if (lc->update_block != 64 && lc->update_block != 128 && lc->update_block != 256)
+        return fail(FCC_ERR_ARGUMENT, "update_block must be 64, 128 or 256");
+    if (lc->xcd_order != 0 && lc->xcd_order != 1) return fail(FCC_ERR_ARGUMENT, "xcd_order must be 0 or 1");
+    if (lc->chunk < 0) return fail(FCC_ERR_ARGUMENT, "chunk must be non-negative");
+    return FCC_OK;
+}
+
+// Tiles of one launch: ceil(N_r / tile) workgroups per replica, in replica
+// order. With `xcd` the table is padded to a multiple of 8 and permuted so
+// that workgroup b, which the dispatcher places on XCD b mod 8, takes tile
+// (b mod 8)·G/8 + b/8: each XCD walks one contiguous range of tiles and a
+// replica's neighbour gathers stay within one XCD's L2.
+static std::vector<Tile> tile_table(const std::vector<int64_t> &sizes, int tile, bool xcd) {
+    std::vector<Tile> lin;
+    for (size_t r = 0; r < sizes.size(); r++) {
+        const int64_t nb = (sizes[r] + tile - 1) / tile;
+        for (int64_t b = 0; b < nb; b++) lin.push_back(Tile{(int32_t)r, (int32_t)b});
+    }
+    if (!xcd) return lin;
+    const size_t G = (lin.size() + 7) / 8 * 8, seg = G / 8;
+    std::vector<Tile> out(G);
+    for (size_t b = 0; b < G; b++) {
+        const size_t tau = (b % 8) * seg + b / 8;
+        out[b] = tau < lin.size() ? lin[tau] : Tile{-1, 0};
+    }
+    return out;
+}
+
+template <typename T>
+struct Ensemble final : fcc_ensemble {
+    std::vector<Sim<T> *> m;
+    hipStream_t stream = nullptr;
+    DevStatus *st = nullptr;  // one slot per replica
+    Params<T> *dP = nullptr;
+    Ptrs<T> *dD = nullptr;
+    Tile *tf = nullptr, *tu = nullptr;  // tiles of the forces / update launches
+    size_t cap_f = 0, cap_u = 0;
+    unsigned gf = 0, gu = 0;
+    std::vector<Params<T>> hP, upP;
+    std::vector<Ptrs<T>> hD, upD;
+    bool uploaded = false;
+    fcc_launch lc{256, 1, 0, 0};
+    fcc_launch tiles_lc{0, 0, 0, 0};
+    int var = -1, tiles_var = -1, ncls = 1;  // forces kernel: 2 MFMA, 1 fast, 0 generic
+    hipGraphExec_t gexec = nullptr;
+    bool graph_ok = false;
+    std::vector<double> interval;  // mean steps between rebuilds, per replica
+    std::vector<int64_t> since;
+    bool want_timed = false;
+    hipEvent_t tev[3] = {nullptr, nullptr, nullptr};
+    double acc_f = 0.0, acc_u = 0.0;
+    int64_t acc_n = 0;
+
+    ~Ensemble() override {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        for (hipEvent_t e : tev)
+            if (e) (void)hipEventDestroy(e);
+        for (Sim<T> *s : m) delete s;
+        for (void *q : {(void *)st, (void *)dP, (void *)dD, (void *)tf, (void *)tu})
+            if (q) (void)hipFree(q);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    int32_t size() const override { return (int32_t)m.size(); }
+    fcc_sim *member(int32_t r) override { return m[r]; }
+
+    int init(int32_t n, const fcc_config *cfgs, const fcc_bodies *bodies) {
+        HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        HIPCHK(hipMalloc(&st, sizeof(DevStatus) * n));
+        HIPCHK(hipMalloc(&dP, sizeof(Params<T>) * n));
+        HIPCHK(hipMalloc(&dD, sizeof(Ptrs<T>) * n));
+        for (int32_t r = 0; r < n; r++) {
+            auto *s = new Sim<T>();
+            s->borrowed = true;
+            m.push_back(s);
+            int rc = s->init(&cfgs[r], &bodies[r], stream, st + r);
+            if (rc) return rc;
+        }
+        const Sim<T> &a = *m[0];
+        for (int32_t r = 0; r < n; r++) {
+            const Sim<T> &s = *m[r];
+            if (s.has_pdrag != a.has_pdrag)
+                return fail(FCC_ERR_UNSUPPORTED, "replica %d: parent drag must be present in all replicas or none", r);
+            if (s.mfma_T != a.mfma_T)
+                return fail(FCC_ERR_UNSUPPORTED, "replica %d: Bending must be present in all replicas or none", r);
+            if (s.has_contact && s.skin <= 0.0)
+                return fail(FCC_ERR_UNSUPPORTED, "replica %d: contact without a Verlet skin has no graph step", r);
+        }
+        hP.resize(n);
+        hD.resize(n);
+        interval.assign(n, 64.0);
+        since.assign(n, 0);
+        HIPCHK(hipStreamSynchronize(stream));
+        return FCC_OK;
+    }
+
+    // ------------------------------------------------------------ launches
+
+    int launch_forces() {
+        const Sim<T> &a = *m[0];
+        const int dbg = a.debug ? 1 : 0;
+        if constexpr (sizeof(T) == 4) {
+            if (var == 2) {
+                const bool bf = a.cfg.precision == FCC_MIXED_BF16;
+#define FCC_MFE(KV, TV, NCV)                                                                                  \
+    if (bf) hipLaunchKernelGGL((k_forces_mfma_ens<KV, TV, NCV, true>), gf, 64, 0, stream, dP, dD, tf, a.asyn,  \
+                               a.bpull, 1, dbg);                                                               \
+    else hipLaunchKernelGGL((k_forces_mfma_ens<KV, TV, NCV, false>), gf, 64, 0, stream, dP, dD, tf, a.asyn,   \
+                            a.bpull, 1, dbg);
+#define FCC_MFE_T(KV, NCV) if (a.mfma_T == 3) { FCC_MFE(KV, 3, NCV) } else { FCC_MFE(KV, 4, NCV) }
+                if (a.K == 5) {
+                    if (ncls <= 1) { FCC_MFE_T(5, 1) } else { FCC_MFE_T(5, 2) }
+                } else {
+                    if (ncls <= 1) { FCC_MFE_T(7, 1) } else { FCC_MFE_T(7, 2) }
+                }
+#undef FCC_MFE_T
+#undef FCC_MFE
+                HIPCHK(hipGetLastError());
+                return FCC_OK;
+            }
+        }
+        if (var == 1) {
+#define FCC_FE(KV, NC)                                                                                        \
+    if (a.nth == 40) hipLaunchKernelGGL((k_forces_fast_ens<T, 40, KV, NC>), gf, 64, 0, stream, dP, dD, tf, 1, dbg); \
+    else hipLaunchKernelGGL((k_forces_fast_ens<T, 0, KV, NC>), gf, 64, 0, stream, dP, dD, tf, 1, dbg);
+            if (a.K == 5) {
+                if (ncls <= 1) { FCC_FE(5, 1) } else { FCC_FE(5, 2) }
+            } else {
+                if (ncls <= 1) { FCC_FE(7, 1) } else { FCC_FE(7, 2) }
+            }
+#undef FCC_FE
+            HIPCHK(hipGetLastError());
+            return FCC_OK;
+        }
+#define FCC_GE(KMV)                                                                                           \
+    if (a.nth == 40) hipLaunchKernelGGL((k_forces_ens<T, 40, KMV>), gf, 64, 0, stream, dP, dD, tf, 1, dbg);   \
+    else hipLaunchKernelGGL((k_forces_ens<T, 0, KMV>), gf, 64, 0, stream, dP, dD, tf, 1, dbg);
+        switch (a.KM) {
+        case 5: FCC_GE(5) break;
+        case 7: FCC_GE(7) break;
+        default: FCC_GE(15)
+        }
+#undef FCC_GE
+        HIPCHK(hipGetLastError());
+        return FCC_OK;
+    }
+
+    int launch_update() {
+        const Sim<T> &a = *m[0];
+        const int ud = a.has_pdrag ? 1 : 0, dbg = a.debug ? 1 : 0;
+        const int ub = lc.update_block;
+        if (ud) hipLaunchKernelGGL((k_displacement_ens<T>), gu, ub, 0, stream, dP, dD, tu);
+#define FCC_UE(KMV, EX)                                                                                       \
+    if (a.mixed) hipLaunchKernelGGL((k_update_ens<T, KMV, EX, true>), gu, ub, 0, stream, dP, dD, tu, ud, dbg); \
+    else hipLaunchKernelGGL((k_update_ens<T, KMV, EX, false>), gu, ub, 0, stream, dP, dD, tu, ud, dbg);
+        if (a.K == 5) {
+            FCC_UE(5, true)
+        } else if (a.K == 7) {
+            FCC_UE(7, true)
+        } else if (a.KM == 5) {
+            FCC_UE(5, false)
+        } else if (a.KM == 7) {
+            FCC_UE(7, false)
+        } else {
+            FCC_UE(15, false)
+        }
+#undef FCC_UE
+        HIPCHK(hipGetLastError());
+        return FCC_OK;
+    }
+
+    // Tables, tiles and the captured step, brought up to date with the
+    // replicas (kernel variant, list capacities, dt) and the launch shape.
+    int prepare() {
+        bool fast = true;
+        int nc = 1;
+        for (Sim<T> *s : m) {
+            fast = fast && s->fast_ok();
+            nc = std::max(nc, s->p.ncls);
+        }
+        const int v = fast ? ((sizeof(T) == 4 && m[0]->mfma_T) ? 2 : 1) : 0;
+        if (v != var || nc != ncls) graph_ok = false;
+        var = v;
+        ncls = nc;
+        const size_t R = m.size();
+        for (size_t r = 0; r < R; r++) {
+            Sim<T> &s = *m[r];
+            s.p.skin_check = (s.has_contact && s.skin > 0.0) ? 1 : 0;
+            hP[r] = s.p;
+            hD[r] = s.d;
+        }
+        if (!uploaded || std::memcmp(hP.data(), upP.data(), sizeof(Params<T>) * R) != 0 ||
+            std::memcmp(hD.data(), upD.data(), sizeof(Ptrs<T>) * R) != 0) {
+            upP = hP;
+            upD = hD;
+            HIPCHK(hipMemcpyAsync(dP, upP.data(), sizeof(Params<T>) * R, hipMemcpyHostToDevice, stream));
+            HIPCHK(hipMemcpyAsync(dD, upD.data(), sizeof(Ptrs<T>) * R, hipMemcpyHostToDevice, stream));
+            uploaded = true;
+        }
+        if (var != tiles_var || lc.update_block != tiles_lc.update_block || lc.xcd_order != tiles_lc.xcd_order) {
+            std::vector<int64_t> sizes(R);
+            for (size_t r = 0; r < R; r++) sizes[r] = m[r]->N;
+            const std::vector<Tile> a = tile_table(sizes, var == 2 ? 32 : 64, lc.xcd_order != 0);
+            const std::vector<Tile> b = tile_table(sizes, lc.update_block, lc.xcd_order != 0);
+            HIPCHK(hipStreamSynchronize(stream));
+            if (a.size() > cap_f) {
+                if (tf) HIPCHK(hipFree(tf));
+                tf = nullptr;
+                HIPCHK(hipMalloc(&tf, sizeof(Tile) * a.size()));
+                cap_f = a.size();
+            }
+            if (b.size() > cap_u) {
+                if (tu) HIPCHK(hipFree(tu));
+                tu = nullptr;
+                HIPCHK(hipMalloc(&tu, sizeof(Tile) * b.size()));
+                cap_u = b.size();
+            }
+            HIPCHK(hipMemcpy(tf, a.data(), sizeof(Tile) * a.size(), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(tu, b.data(), sizeof(Tile) * b.size(), hipMemcpyHostToDevice));
+            gf = (unsigned)a.size();
+            gu = (unsigned)b.size();
+            tiles_var = var;
+            tiles_lc = lc;
+            graph_ok = false;
+        }
+        if (!graph_ok) {
+            std::lock_guard<std::mutex> lk(g_capture_lock);
+            if (gexec) {
+                HIPCHK(hipGraphExecDestroy(gexec));
+                gexec = nullptr;
+            }
+            hipGraph_t graph;
+            HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            int rc = launch_forces();
+            if (!rc) rc = launch_update();
+            hipError_t e = hipStreamEndCapture(stream, &graph);
+            if (rc) return rc;
+            HIPCHK(e);
+            HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+            HIPCHK(hipGraphDestroy(graph));
+            graph_ok = true;
+        }
+        return FCC_OK;
+    }
+
+    int read_status(std::vector<DevStatus> &hs) {
+        hs.resize(m.size());
+        HIPCHK(hipMemcpyAsync(hs.data(), st, sizeof(DevStatus) * m.size(), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        return FCC_OK;
+    }
+
+    int set_budgets(int64_t steps) {
+        const int R = (int)m.size();
+        hipLaunchKernelGGL(k_ensemble_targets, blocks(R, 64), 64, 0, stream, st, R, steps);
+        HIPCHK(hipGetLastError());
+        return FCC_OK;
+    }
+
+    // update_neighbors! of every replica whose list is stale or whose skin
+    // tripped on its last step (run_steps! begins with it).
+    int refresh_lists(std::vector<DevStatus> &hs) {
+        int rc = read_status(hs);
+        if (rc) return rc;
+        for (size_t r = 0; r < m.size(); r++) {
+            Sim<T> &s = *m[r];
+            if (s.has_contact && (!s.valid || hs[r].rebuild != 0) && (rc = s.rebuild())) return rc;
+        }
+        return FCC_OK;
+    }
+
+    int64_t chunk_len() const {
+        if (lc.chunk > 0) return lc.chunk;
+        double mi = 1e300;
+        for (size_t r = 0; r < m.size(); r++)
+            if (m[r]->has_contact) mi = std::min(mi, interval[r]);
+        if (mi >= 1e300) return 256;
+        return std::min<int64_t>(64, std::max<int64_t>(4, (int64_t)std::llround(mi / 8)));
+    }
+
+    int advance(int64_t n) {
+        std::vector<DevStatus> hs;
+        int rc = refresh_lists(hs);
+        if (rc) return rc;
+        if ((rc = set_budgets(n)) || (rc = read_status(hs))) return rc;
+        const size_t R = m.size();
+        std::vector<int64_t> clock(R), target(R);
+        for (size_t r = 0; r < R; r++) {
+            clock[r] = hs[r].clock;
+            target[r] = hs[r].target;
+        }
+        for (;;) {
+            int64_t remaining = 0;
+            for (size_t r = 0; r < R; r++) remaining = std::max(remaining, target[r] - clock[r]);
+            if (remaining <= 0) break;
+            if ((rc = prepare())) return rc;
+            const int64_t k = std::min(chunk_len(), remaining);
+            if (!want_timed) {
+                for (int64_t q = 0; q < k; q++) HIPCHK(hipGraphLaunch(gexec, stream));
+            } else {
+                // the chunk's last step eagerly between events: the kernels
+                // and arguments of the captured step
+                if (!tev[0])
+                    for (auto &ev : tev) HIPCHK(hipEventCreate(&ev));
+                for (int64_t q = 0; q + 1 < k; q++) HIPCHK(hipGraphLaunch(gexec, stream));
+                HIPCHK(hipEventRecord(tev[0], stream));
+                if ((rc = launch_forces())) return rc;
+                HIPCHK(hipEventRecord(tev[1], stream));
+                if ((rc = launch_update())) return rc;
+                HIPCHK(hipEventRecord(tev[2], stream));
+            }
+            if ((rc = read_status(hs))) return rc;
+            if (want_timed) {
+                float a = 0.f, b = 0.f;
+                HIPCHK(hipEventElapsedTime(&a, tev[0], tev[1]));
+                HIPCHK(hipEventElapsedTime(&b, tev[1], tev[2]));
+                acc_f += a;
+                acc_u += b;
+                acc_n++;
+            }
+            for (size_t r = 0; r < R; r++) {
+                Sim<T> &s = *m[r];
+                const int64_t ex = hs[r].clock - clock[r];
+                clock[r] = hs[r].clock;
+                for (int64_t q = 0; q < ex; q++) s.t += s.cfg.dt;
+                s.nstep += ex;
+                since[r] += ex;
+                if (hs[r].rebuild != 0 && clock[r] < target[r]) {
+                    if ((rc = s.rebuild())) return rc;
+                    interval[r] = (interval[r] + (double)since[r]) / 2;
+                    since[r] = 0;
+                } else if ((double)since[r] > interval[r]) {
+                    interval[r] = std::min((double)since[r], 4.0 * 256);
+                }
+            }
+        }
+        return FCC_OK;
+    }
+
+    int run(int64_t n) override {
+        if (n <= 0) return FCC_OK;
+        const int rc = advance(n);
+        const int rc2 = set_budgets(-1);  // no budget outside ensemble runs
+        hipError_t e = hipStreamSynchronize(stream);
+        if (rc) return rc;
+        if (rc2) return rc2;
+        HIPCHK(e);
+        return FCC_OK;
+    }
+
+    int run_timed(int64_t n, double *fms, double *ums) override {
+        want_timed = true;
+        acc_f = acc_u = 0.0;
+        acc_n = 0;
+        const int rc = run(n);
+        want_timed = false;
+        if (rc) return rc;
+        if (fms) *fms = acc_n ? acc_f / acc_n : 0.0;
+        if (ums) *ums = acc_n ? acc_u / acc_n : 0.0;
+        return FCC_OK;
+    }
+
+    int get_launch(fcc_launch *out) override {
+        if (!out) return fail(FCC_ERR_ARGUMENT, "null launch shape");
+        *out = lc;
+        return FCC_OK;
+    }
+
+    int set_launch(const fcc_launch *in) override {
+        int rc = check_launch(in);
+        if (rc) return rc;
+        lc = *in;
+        return FCC_OK;
+    }
+
+    // ----------------------------------------------------------- autotuner
+
+    // Device regions a step writes (state, geometry, gradients, status).
+    std::vector<std::pair<void *, size_t>> step_regions() {
+        std::vector<std::pair<void *, size_t>> v;
+        for (Sim<T> *s : m) {
+            const size_t N = (size_t)s->N, rows = N * (size_t)s->ldc;
+            Ptrs<T> &d = s->d;
+            v.push_back({d.x, 8 * N});
+            v.push_back({d.y, 8 * N});
+            v.push_back({d.imgx, 4 * N});
+            v.push_back({d.imgy, 4 * N});
+            v.push_back({d.th, sizeof(T) * N});
+            v.push_back({d.coef, sizeof(T) * rows});
+            v.push_back({d.lab, sizeof(T) * rows});
+            v.push_back({d.rmax, sizeof(T) * N});
+            v.push_back({d.rmin, sizeof(T) * N});
+            v.push_back({d.glab, sizeof(T) * rows});
+            v.push_back({d.gx, sizeof(T) * N});
+            v.push_back({d.gy, sizeof(T) * N});
+            v.push_back({d.gth, sizeof(T) * N});
+            v.push_back({d.Ebody, sizeof(T) * N});
+        }
+        v.push_back({st, sizeof(DevStatus) * m.size()});
+        return v;
+    }
+
+    int copy_regions(const std::vector<std::pair<void *, size_t>> &v, char *buf, bool save) {
+        size_t off = 0;
+        for (const auto &q : v) {
+            if (save)
+                HIPCHK(hipMemcpyAsync(buf + off, q.first, q.second, hipMemcpyDeviceToDevice, stream));
+            else
+                HIPCHK(hipMemcpyAsync(q.first, buf + off, q.second, hipMemcpyDeviceToDevice, stream));
+            off += (q.second + 255) / 256 * 256;
+        }
+        return FCC_OK;
+    }
+
+    int autotune(int64_t steps, fcc_launch *best, double *ms) override {
+        if (steps <= 0) return fail(FCC_ERR_ARGUMENT, "steps must be positive");
+        std::vector<DevStatus> hs;
+        int rc = refresh_lists(hs);
+        if (rc) return rc;
+        const auto regions = step_regions();
+        size_t bytes = 0;
+        for (const auto &q : regions) bytes += (q.second + 255) / 256 * 256;
+        char *snap = nullptr;
+        HIPCHK(hipMalloc(&snap, bytes));
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        fcc_launch keep = lc, win = lc;
+        double best_ms = 1e300;
+        rc = copy_regions(regions, snap, true);
+        if (!rc && hipEventCreate(&e0) != hipSuccess) rc = fail(FCC_ERR_DEVICE, "hipEventCreate failed");
+        if (!rc && hipEventCreate(&e1) != hipSuccess) rc = fail(FCC_ERR_DEVICE, "hipEventCreate failed");
+        const int ubs[3] = {64, 128, 256};
+        for (int xo = 0; xo < 2 && !rc; xo++) {
+            for (int u = 0; u < 3 && !rc; u++) {
+                lc = fcc_launch{ubs[u], xo, keep.chunk, 0};
+                if ((rc = prepare()) || (rc = copy_regions(regions, snap, false))) break;
+                hipError_t e = hipSuccess;
+                for (int q = 0; q < 2 && e == hipSuccess; q++) e = hipGraphLaunch(gexec, stream);
+                if (e == hipSuccess) e = hipEventRecord(e0, stream);
+                for (int64_t q = 0; q < steps && e == hipSuccess; q++) e = hipGraphLaunch(gexec, stream);
+                if (e == hipSuccess) e = hipEventRecord(e1, stream);
+                if (e == hipSuccess) e = hipEventSynchronize(e1);
+                float t = 0.f;
+                if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+                if (e != hipSuccess) {
+                    rc = fail(FCC_ERR_DEVICE, "autotune trial: %s", hipGetErrorString(e));
+                    break;
+                }
+                const double per = (double)t / (double)steps;
+                if (per < best_ms) {
+                    best_ms = per;
+                    win = lc;
+                }
+            }
+        }
+        if (!rc) rc = copy_regions(regions, snap, false);
+        lc = rc ? keep : win;
+        if (!rc) rc = prepare();
+        (void)hipStreamSynchronize(stream);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        (void)hipFree(snap);
+        if (rc) return rc;
+        if (best) *best = lc;
+        if (ms) *ms = best_ms;
+        return FCC_OK;
+    }
+};

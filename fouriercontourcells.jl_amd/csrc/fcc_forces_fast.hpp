@@ -122,8 +122,8 @@ __device__ __forceinline__ void synth_k(const FCC_CONST T *S, int nth, int ldc, 
     }
 }
 
-template <typename T, int NTH_, int K, int NCLS>
-__device__ __forceinline__ void k_forces_fast_impl(const Params<T> &p, const Ptrs<T> &d, int use_status,
+template <typename T, int NTH_, int K, int NCLS, typename PP, typename DP>
+__device__ __forceinline__ void k_forces_fast_impl(const PP &p, const DP &d, int use_status,
                                                     int debug, int64_t blk) {
     constexpr int NC = 2 * K + 1;
     constexpr int NTHM = NTH_ ? NTH_ : FCC_MAX_THETA;
@@ -348,10 +348,13 @@ __global__ __launch_bounds__(64) void k_forces_fast(const Params<T> p, const Ptr
 }
 
 template <typename T, int NTH_, int K, int NCLS>
-__global__ __launch_bounds__(64) void k_forces_fast_ens(const Params<T> *ps, const Ptrs<T> *ds, int use_status,
-                                                    int debug, const int32_t *brep, const int32_t *bfirst) {
-    const int r = brep[blockIdx.x];
-    k_forces_fast_impl<T, NTH_, K, NCLS>(ps[r], ds[r], use_status, debug, (int64_t)blockIdx.x - bfirst[r]);
+__global__ __launch_bounds__(64) void k_forces_fast_ens(const Params<T> *ps, const Ptrs<T> *ds, const Tile *tiles,
+                                                        int use_status, int debug) {
+    const Tile tl = load_uniform(tiles, blockIdx.x);
+    if (tl.rep < 0) return;
+    const FCC_CONST Params<T> &p = as_const(ps)[tl.rep];
+    const FCC_CONST Ptrs<T> &d = as_const(ds)[tl.rep];
+    k_forces_fast_impl<T, NTH_, K, NCLS>(p, d, use_status, debug, tl.blk);
 }
 
 }  // namespace fcc

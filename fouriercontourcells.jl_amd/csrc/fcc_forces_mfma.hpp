@@ -91,8 +91,8 @@ __device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
 // Operand tables (built on the host, 16-bit values):
 //  asyn[(t * 2 + part) * 64 + lane][8]   synthesis A (S rows), part 0 hi / 1 lo
 //  bpull[(c * 2 + v) * 64 + lane][8]     pullback B, v 0 = [Sh | Sl], 1 = [Sh | 0]
-template <int K, int T, int NCLS, bool BF>
-__device__ __forceinline__ void k_forces_mfma_impl(const Params<float> &p, const Ptrs<float> &d,
+template <int K, int T, int NCLS, bool BF, typename PP, typename DP>
+__device__ __forceinline__ void k_forces_mfma_impl(const PP &p, const DP &d,
                                                     const uint16_t *asyn, const uint16_t *bpull, int use_status,
                                                     int debug, int64_t blk) {
     typedef MfmaType<BF> MT;
@@ -373,10 +373,13 @@ __global__ __launch_bounds__(64) void k_forces_mfma(const Params<float> p, const
 
 template <int K, int T, int NCLS, bool BF>
 __global__ __launch_bounds__(64) void k_forces_mfma_ens(const Params<float> *ps, const Ptrs<float> *ds,
-                                                    const uint16_t *asyn, const uint16_t *bpull, int use_status,
-                                                    int debug, const int32_t *brep, const int32_t *bfirst) {
-    const int r = brep[blockIdx.x];
-    k_forces_mfma_impl<K, T, NCLS, BF>(ps[r], ds[r], asyn, bpull, use_status, debug, (int64_t)blockIdx.x - bfirst[r]);
+                                                        const Tile *tiles, const uint16_t *asyn,
+                                                        const uint16_t *bpull, int use_status, int debug) {
+    const Tile tl = load_uniform(tiles, blockIdx.x);
+    if (tl.rep < 0) return;
+    const FCC_CONST Params<float> &p = as_const(ps)[tl.rep];
+    const FCC_CONST Ptrs<float> &d = as_const(ds)[tl.rep];
+    k_forces_mfma_impl<K, T, NCLS, BF>(p, d, asyn, bpull, use_status, debug, tl.blk);
 }
 
 }  // namespace fcc

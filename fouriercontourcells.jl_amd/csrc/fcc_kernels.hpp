@@ -39,8 +39,8 @@ __device__ __forceinline__ void synth(const FCC_CONST T *S, int nth, int k, cons
 // (src/gradients.jl:374-386), fused. Writes glab (∂E/∂(R0·lab)), gx, gy,
 // gθ and Ebody. With use_status it also performs the status mirror and the
 // clock tick of a step (src/utils.jl:50-78).
-template <typename T, int NTH_, int KM>
-__device__ __forceinline__ void k_forces_impl(const Params<T> &p, const Ptrs<T> &d, int use_status,
+template <typename T, int NTH_, int KM, typename PP, typename DP>
+__device__ __forceinline__ void k_forces_impl(const PP &p, const DP &d, int use_status,
                                                int debug, int64_t blk) {
     constexpr int NTHM = NTH_ ? NTH_ : FCC_MAX_THETA;
     constexpr int NCM = 2 * KM + 1;
@@ -226,10 +226,13 @@ __global__ __launch_bounds__(64) void k_forces(const Params<T> p, const Ptrs<T> 
 }
 
 template <typename T, int NTH_, int KM>
-__global__ __launch_bounds__(64) void k_forces_ens(const Params<T> *ps, const Ptrs<T> *ds, int use_status,
-                                               int debug, const int32_t *brep, const int32_t *bfirst) {
-    const int r = brep[blockIdx.x];
-    k_forces_impl<T, NTH_, KM>(ps[r], ds[r], use_status, debug, (int64_t)blockIdx.x - bfirst[r]);
+__global__ __launch_bounds__(64) void k_forces_ens(const Params<T> *ps, const Ptrs<T> *ds, const Tile *tiles,
+                                                   int use_status, int debug) {
+    const Tile tl = load_uniform(tiles, blockIdx.x);
+    if (tl.rep < 0) return;
+    const FCC_CONST Params<T> &p = as_const(ps)[tl.rep];
+    const FCC_CONST Ptrs<T> &d = as_const(ds)[tl.rep];
+    k_forces_impl<T, NTH_, KM>(p, d, use_status, debug, tl.blk);
 }
 
 // ----------------------------------------------------------- per-body math
@@ -363,9 +366,9 @@ __global__ __launch_bounds__(256) void k_body_frame(const Params<T> p, const Ptr
 }
 
 // substrate_displacement (src/dynamics.jl:84-99) of body c.
-template <typename T>
-__device__ __forceinline__ void substrate_disp(const Params<T> &p, const Ptrs<T> &d, int64_t c,
-                                               uint64_t draw, T &ux, T &uy) {
+template <typename T, typename PP, typename DP>
+__device__ __forceinline__ void substrate_disp(const PP &p, const DP &d, int64_t c, uint64_t draw, T &ux,
+                                               T &uy) {
     const int tc = d.type[c];
     T fx = d.gx[c], fy = d.gy[c];
     const int nch = d.ccount[c];
@@ -387,9 +390,9 @@ __device__ __forceinline__ void substrate_disp(const Params<T> &p, const Ptrs<T>
 
 // displacement_kernel! (src/dynamics.jl:258-277), used when some body type
 // has parent drag: a child needs its parent's θ before the update.
-template <typename T>
-__device__ __forceinline__ void k_displacement_impl(const Params<T> &p, const Ptrs<T> &d, int64_t blk) {
-    const int64_t i = (int64_t)blk * 256 + threadIdx.x;
+template <typename T, int UB, typename PP, typename DP>
+__device__ __forceinline__ void k_displacement_impl(const PP &p, const DP &d, int64_t blk) {
+    const int64_t i = (int64_t)blk * (UB ? UB : (int)blockDim.x) + threadIdx.x;
     if (i >= p.N || d.st->update != 0) return;
     const uint64_t draw = (uint64_t)(d.st->clock - 1);
     const int ti = d.type[i];
@@ -416,13 +419,17 @@ __device__ __forceinline__ void k_displacement_impl(const Params<T> &p, const Pt
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_displacement(const Params<T> p, const Ptrs<T> d) {
-    k_displacement_impl<T>(p, d, blockIdx.x);
+    k_displacement_impl<T, 256>(p, d, blockIdx.x);
 }
 
+// Ensemble launch: any workgroup size up to 256 (the tile size of the table).
 template <typename T>
-__global__ __launch_bounds__(256) void k_displacement_ens(const Params<T> *ps, const Ptrs<T> *ds, const int32_t *brep, const int32_t *bfirst) {
-    const int r = brep[blockIdx.x];
-    k_displacement_impl<T>(ps[r], ds[r], (int64_t)blockIdx.x - bfirst[r]);
+__global__ __launch_bounds__(256) void k_displacement_ens(const Params<T> *ps, const Ptrs<T> *ds, const Tile *tiles) {
+    const Tile tl = load_uniform(tiles, blockIdx.x);
+    if (tl.rep < 0) return;
+    const FCC_CONST Params<T> &p = as_const(ps)[tl.rep];
+    const FCC_CONST Ptrs<T> &d = as_const(ds)[tl.rep];
+    k_displacement_impl<T, 0>(p, d, tl.blk);
 }
 
 // Fused Euler–Maruyama update (src/dynamics.jl:310-333) with the
@@ -430,11 +437,11 @@ __global__ __launch_bounds__(256) void k_displacement_ens(const Params<T> *ps, c
 // without parent drag), the body-frame gradient, the shape constraint (FP64
 // when F64S), position wrapping in FP64, the lab frame of the new state and
 // the Verlet skin check (src/neighbors.jl:409-416).
-template <typename T, int KM, bool EXACT, bool F64S>
-__device__ __forceinline__ void k_update_impl(const Params<T> &p, const Ptrs<T> &d, int use_disp, int debug, int64_t blk) {
+template <typename T, int KM, bool EXACT, bool F64S, int UB, typename PP, typename DP>
+__device__ __forceinline__ void k_update_impl(const PP &p, const DP &d, int use_disp, int debug, int64_t blk) {
     constexpr int NCM = 2 * KM + 1;
     constexpr int W = row_width<NCM, T>();
-    const int64_t i = (int64_t)blk * 256 + threadIdx.x;
+    const int64_t i = (int64_t)blk * (UB ? UB : (int)blockDim.x) + threadIdx.x;
     if (i >= p.N || d.st->update != 0) return;
     const uint64_t draw = (uint64_t)(d.st->clock - 1);
     const int K = EXACT ? KM : p.K;
@@ -523,13 +530,18 @@ __device__ __forceinline__ void k_update_impl(const Params<T> &p, const Ptrs<T> 
 
 template <typename T, int KM, bool EXACT, bool F64S>
 __global__ __launch_bounds__(256) void k_update(const Params<T> p, const Ptrs<T> d, int use_disp, int debug) {
-    k_update_impl<T, KM, EXACT, F64S>(p, d, use_disp, debug, blockIdx.x);
+    k_update_impl<T, KM, EXACT, F64S, 256>(p, d, use_disp, debug, blockIdx.x);
 }
 
+// Ensemble launch: any workgroup size up to 256 (the tile size of the table).
 template <typename T, int KM, bool EXACT, bool F64S>
-__global__ __launch_bounds__(256) void k_update_ens(const Params<T> *ps, const Ptrs<T> *ds, int use_disp, int debug, const int32_t *brep, const int32_t *bfirst) {
-    const int r = brep[blockIdx.x];
-    k_update_impl<T, KM, EXACT, F64S>(ps[r], ds[r], use_disp, debug, (int64_t)blockIdx.x - bfirst[r]);
+__global__ __launch_bounds__(256) void k_update_ens(const Params<T> *ps, const Ptrs<T> *ds, const Tile *tiles,
+                                                    int use_disp, int debug) {
+    const Tile tl = load_uniform(tiles, blockIdx.x);
+    if (tl.rep < 0) return;
+    const FCC_CONST Params<T> &p = as_const(ps)[tl.rep];
+    const FCC_CONST Ptrs<T> &d = as_const(ds)[tl.rep];
+    k_update_impl<T, KM, EXACT, F64S, 0>(p, d, use_disp, debug, tl.blk);
 }
 
 // ------------------------------------------------------- cell / neighbours
@@ -664,10 +676,20 @@ __global__ void k_clear_status(DevStatus *st) {
     st->overflow = 0;
 }
 
-// Step budget of an ensemble run: freeze once `clock` reaches `target`.
+// Step budget of a simulation: freeze once `clock` reaches `target`.
 __global__ void k_set_target(DevStatus *st, int64_t target) {
     st->target = target;
     st->stop = st->clock >= target ? 1 : 0;
+}
+
+// Step budgets of the n replicas of an ensemble run: target = clock + steps
+// (steps < 0: none) and the matching `stop` flag.
+__global__ void k_ensemble_targets(DevStatus *st, int n, int64_t steps) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int64_t tg = steps < 0 ? INT64_MAX : st[r].clock + steps;
+    st[r].target = tg;
+    st[r].stop = st[r].clock >= tg ? 1 : 0;
 }
 
 template <typename T>

@@ -1,8 +1,8 @@
 """fcc_hip — host mirror of the FourierContourCells.jl time-step API over
 libfcc_hip.so (MI355X / gfx950)."""
-from .api import (ArgumentError, Bending, BodyType, Bodies, DeviceError, DimensionMismatch,
+from .api import (ArgumentError, Bending, BodyType, Bodies, DeviceError, DimensionMismatch, Ensemble,
                   LinearRepulsion, OnTypes, Overlap, PerimeterElasticity, PeriodicBox,
-                  Simulation, Tension, UnsupportedError, areas, body_energy, compute_gradients,
+                  Replica, Simulation, Tension, UnsupportedError, areas, body_energy, compute_gradients,
                   hexagonal_monolayer, perimeters, philox_noise, philox_uniforms, positions,
                   refresh, run, shape_index, step, unwrapped_positions, update_neighbors, vcat,
                   with_nuclei)

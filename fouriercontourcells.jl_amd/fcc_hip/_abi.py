@@ -84,6 +84,15 @@ class FccConfig(ctypes.Structure):
     ]
 
 
+class FccLaunch(ctypes.Structure):
+    _fields_ = [
+        ("update_block", ctypes.c_int32),
+        ("xcd_order", ctypes.c_int32),
+        ("chunk", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int32)
 
@@ -135,6 +144,14 @@ _DEVICE_ONLY = {
     "abi_version": [],
     "time_step_kernels": [_vp, _i64, _dp, _dp],
     "run_timed": [_vp, _i64, _dp, _dp],
+    "ensemble_create": [_i32, ctypes.POINTER(FccConfig), ctypes.POINTER(FccBodies), ctypes.POINTER(_vp)],
+    "ensemble_destroy": [_vp],
+    "ensemble_member": [_vp, _i32, ctypes.POINTER(_vp)],
+    "ensemble_run": [_vp, _i64],
+    "ensemble_run_timed": [_vp, _i64, _dp, _dp],
+    "ensemble_get_launch": [_vp, ctypes.POINTER(FccLaunch)],
+    "ensemble_set_launch": [_vp, ctypes.POINTER(FccLaunch)],
+    "ensemble_autotune": [_vp, _i64, ctypes.POINTER(FccLaunch), _dp],
 }
 
 # Every symbol include/fcc.h declares.

@@ -344,7 +344,15 @@ class Simulation:
 
     def __init__(self, bodies, box, types, energies=(), dt=0.005, ntheta=40, precision="fp64",
                  seed=1, skin=0.3, debug=False, engine=None):
-        self.engine = engine if engine is not None else _abi.hip_engine()
+        self._setup(engine if engine is not None else _abi.hip_engine(), bodies, box, types, energies,
+                    dt, ntheta, precision, seed, skin, debug)
+        h = ctypes.c_void_p()
+        _check(self.engine, self.engine.create(ctypes.byref(self._cfg),
+                                               ctypes.byref(self._bodies.struct()), ctypes.byref(h)))
+        self._h = h
+
+    def _setup(self, engine, bodies, box, types, energies, dt, ntheta, precision, seed, skin, debug):
+        self.engine = engine
         self.box = box
         self.types = tuple(types)
         self.energies = tuple(energies)
@@ -353,14 +361,14 @@ class Simulation:
         self.ntheta = int(ntheta)
         self.precision = precision
         self._bodies = bodies.copy()
+        self._owner = None
         flags = _abi.FCC_FLAG_DEBUG_BUFFERS if debug else 0
         self._cfg = _config(bodies, box, types, energies, dt, ntheta, precision, seed, skin, flags)
-        h = ctypes.c_void_p()
-        _check(self.engine, self.engine.create(ctypes.byref(self._cfg),
-                                               ctypes.byref(self._bodies.struct()), ctypes.byref(h)))
-        self._h = h
 
     def close(self):
+        if getattr(self, "_owner", None) is not None:  # a replica: the ensemble frees it
+            self._h = None
+            return
         if getattr(self, "_h", None):
             self.engine.destroy(self._h)
             self._h = None
@@ -460,6 +468,104 @@ class Simulation:
                                                      ip(cellid), ip(perm), ip(start), ip(stop)))
         return SimpleNamespace(ncx=ncx.value, ncy=ncy.value, cellid=cellid, perm=perm,
                                start=start, stop=stop)
+
+
+class Replica:
+    """The arguments of Simulation(...) that an ensemble replica keeps as
+    its own (bodies, box, types, energies, dt, seed, skin)."""
+
+    def __init__(self, bodies, box, types, energies=(), dt=0.005, seed=1, skin=0.3):
+        self.bodies, self.box, self.types, self.energies = bodies, box, tuple(types), tuple(energies)
+        self.dt, self.seed, self.skin = dt, seed, skin
+
+
+class Ensemble:
+    """Independent simulations advanced together: the reference runs one
+    Simulation per task and stream (README.md:249-261); here every step of
+    all replicas is one launch per kernel, captured in a hipGraph
+    (fcc_ensemble_* of include/fcc.h). Replicas share `precision`, `ntheta`
+    and the shape-mode count; sizes, boxes, parameters and seeds differ.
+    `members` are Simulation handles of the replicas (every function of
+    this module applies to them); each replica ends bitwise where `run`
+    would take it alone. `tune` runs the launch autotuner once.
+    """
+
+    def __init__(self, replicas, ntheta=40, precision="fp32", debug=False, tune=False, tune_steps=20,
+                 engine=None):
+        self.engine = engine if engine is not None else _abi.hip_engine()
+        replicas = list(replicas)
+        if not replicas:
+            raise ArgumentError("an ensemble needs at least one replica")
+        n = len(replicas)
+        self.members = []
+        cfgs = (FccConfig * n)()
+        structs = (FccBodies * n)()
+        for r, rp in enumerate(replicas):
+            sim = Simulation.__new__(Simulation)
+            sim._setup(self.engine, rp.bodies, rp.box, rp.types, rp.energies, rp.dt, ntheta, precision,
+                       rp.seed, rp.skin, debug)
+            cfgs[r] = sim._cfg
+            structs[r] = sim._bodies.struct()
+            self.members.append(sim)
+        h = ctypes.c_void_p()
+        _check(self.engine, self.engine.ensemble_create(n, cfgs, structs, ctypes.byref(h)))
+        self._h = h
+        for r, sim in enumerate(self.members):
+            mh = ctypes.c_void_p()
+            _check(self.engine, self.engine.ensemble_member(self._h, r, ctypes.byref(mh)))
+            sim._h = mh
+            sim._owner = self
+        if tune:
+            self.autotune(tune_steps)
+
+    def __len__(self):
+        return len(self.members)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            for sim in self.members:
+                sim._h = None
+            self.engine.ensemble_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, nsteps):
+        """run!(sim, nsteps) on every replica."""
+        _check(self.engine, self.engine.ensemble_run(self._h, int(nsteps)))
+        return self
+
+    def run_timed(self, nsteps):
+        """`run` with HIP-event timings of the chunks' last steps: (forces ms,
+        update ms) per ensemble step."""
+        f, u = ctypes.c_double(), ctypes.c_double()
+        _check(self.engine, self.engine.ensemble_run_timed(self._h, int(nsteps), ctypes.byref(f),
+                                                           ctypes.byref(u)))
+        return f.value, u.value
+
+    @property
+    def launch(self):
+        lc = _abi.FccLaunch()
+        _check(self.engine, self.engine.ensemble_get_launch(self._h, ctypes.byref(lc)))
+        return {"update_block": lc.update_block, "xcd_order": lc.xcd_order, "chunk": lc.chunk}
+
+    @launch.setter
+    def launch(self, v):
+        lc = _abi.FccLaunch(int(v.get("update_block", 256)), int(v.get("xcd_order", 1)),
+                            int(v.get("chunk", 0)), 0)
+        _check(self.engine, self.engine.ensemble_set_launch(self._h, ctypes.byref(lc)))
+
+    def autotune(self, steps=20):
+        """Time every candidate launch shape over `steps` steps from the same
+        state (restored afterwards), keep the fastest; returns (launch, ms per step)."""
+        lc, ms = _abi.FccLaunch(), ctypes.c_double()
+        _check(self.engine, self.engine.ensemble_autotune(self._h, int(steps), ctypes.byref(lc),
+                                                          ctypes.byref(ms)))
+        return self.launch, ms.value
 
 
 # ---------------------------------------------------------------- functions

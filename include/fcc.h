@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define FCC_ABI_VERSION 1
+#define FCC_ABI_VERSION 2
 
 #define FCC_MAX_TYPES 4        /* body types per simulation */
 #define FCC_MAX_TERMS 8        /* energy terms per simulation */
@@ -196,6 +196,49 @@ int fcc_philox_noise(uint64_t seed, uint64_t draw, int64_t n, int32_t precision,
  * key [2 x n] -> out [4 x n]. */
 int fcc_philox4x32(const uint32_t *ctr, const uint32_t *key, int64_t n,
                    uint32_t *out);
+
+/* ---- ensembles of independent simulations
+ *
+ * Independent simulations advanced together, in place of one task and stream
+ * per simulation (README.md:249-261, test/test_cuda.jl:95-104). The replicas
+ * share precision, n_modes, n_theta, parent drag (present or not) and flags;
+ * N, box, body types, energy parameters, dt, skin and seed are their own.
+ * Each step of all replicas is one launch per kernel over a table of
+ * (replica, workgroup) tiles, captured once in a hipGraph; a replica whose
+ * Verlet skin trips is frozen until its list is rebuilt between graph
+ * chunks. Every replica ends bitwise where fcc_run would take it alone. */
+typedef struct fcc_ensemble fcc_ensemble;
+
+/* Launch shape of the ensemble step (set by hand or by the autotuner). */
+typedef struct {
+    int32_t update_block; /* bodies per workgroup of the update kernel: 64, 128 or 256 */
+    int32_t xcd_order;    /* 1: each XCD runs one contiguous range of tiles (8 XCDs) */
+    int32_t chunk;        /* graph replays between host status reads; 0 = adaptive */
+    int32_t reserved;
+} fcc_launch;
+
+/* Create n replicas from cfgs[n] and bodies[n] (validated like fcc_create).
+ * FCC_ERR_UNSUPPORTED if the replicas differ in what must be shared. */
+int fcc_ensemble_create(int32_t n, const fcc_config *cfgs, const fcc_bodies *bodies,
+                        fcc_ensemble **out);
+int fcc_ensemble_destroy(fcc_ensemble *ens);
+
+/* Borrowed handle of replica r (0-based), valid until fcc_ensemble_destroy:
+ * every fcc_* call applies to it except fcc_destroy. */
+int fcc_ensemble_member(fcc_ensemble *ens, int32_t r, fcc_sim **member);
+
+/* Advance every replica by n steps (run!(sim, n) on each). */
+int fcc_ensemble_run(fcc_ensemble *ens, int64_t n);
+
+int fcc_ensemble_get_launch(fcc_ensemble *ens, fcc_launch *launch);
+int fcc_ensemble_set_launch(fcc_ensemble *ens, const fcc_launch *launch);
+
+/* Autotuner: replays `steps` ensemble steps for every candidate launch shape
+ * (update_block x xcd_order) from the same state, restored after each trial
+ * and at the end (the run state is unchanged), keeps the fastest and returns
+ * it with its device time per step in milliseconds. */
+int fcc_ensemble_autotune(fcc_ensemble *ens, int64_t steps, fcc_launch *best,
+                          double *ms_per_step);
 
 #ifdef __cplusplus
 }

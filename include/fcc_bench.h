@@ -27,6 +27,12 @@ int fcc_time_step_kernels(fcc_sim *sim, int64_t steps, double *forces_ms,
  * durations over those steps, in milliseconds. Results equal fcc_run's. */
 int fcc_run_timed(fcc_sim *sim, int64_t n, double *forces_ms, double *update_ms);
 
+/* fcc_ensemble_run with the same eager last step per chunk between HIP
+ * events; average forces and update kernel durations of an ensemble step
+ * (all replicas), in milliseconds. Results equal fcc_ensemble_run's. */
+int fcc_ensemble_run_timed(fcc_ensemble *ens, int64_t n, double *forces_ms,
+                           double *update_ms);
+
 #ifdef __cplusplus
 }
 #endif

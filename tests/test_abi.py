@@ -32,7 +32,9 @@ def test_library_exports_header_symbols():
     for sym in header_symbols():
         assert hasattr(lib, sym), sym
     lib.fcc_abi_version.restype = ctypes.c_int
-    assert lib.fcc_abi_version() == 1
+    version = int(re.search(r"#define FCC_ABI_VERSION (\d+)",
+                            open(os.path.join(ROOT, "include", "fcc.h")).read()).group(1))
+    assert lib.fcc_abi_version() == version
 
 
 def test_struct_layout():
@@ -40,6 +42,7 @@ def test_struct_layout():
     assert ctypes.sizeof(_abi.FccBodyType) == 56
     assert ctypes.sizeof(_abi.FccTerm) == 8 + 16 + 16 + 2 * 8 * 16
     assert ctypes.sizeof(_abi.FccBodies) == 9 * 8
+    assert ctypes.sizeof(_abi.FccLaunch) == 16
 
 
 def test_host_validation_errors():
