@@ -119,6 +119,79 @@ def forces_flops(sim, tested, inr):
     return N * 2 * (2 * 3 * nth * nc) + 12 * tested + (7 + 8 * K) * inr + 10 * nth * N
 
 
+def ensemble_replicas(nrep=64, seed=4):
+    """Batched-ensemble workload: `nrep` replicas with N log-uniform in
+    [1024, 16384] (nx ≈ ny ≈ √N, ny even), packing U[0.9, 1.0], v0 linear in
+    0.02..0.2, Dr cycling over {0.005, 0.01, 0.02}, Tension(η ~ U[0.005, 0.05])
+    + Overlap(1.0), modes 2..6, nθ = 40, dt = 0.005, Philox key = replica id
+    (sizes and η from numpy's generator with `seed`)."""
+    rng = np.random.default_rng(seed)
+    reps = []
+    for r in range(nrep):
+        n = math.exp(rng.uniform(math.log(1024), math.log(16384)))
+        nx = max(2, round(math.sqrt(n)))
+        ny = max(2, 2 * round(n / nx / 2))
+        packing = rng.uniform(0.9, 1.0)
+        eta = rng.uniform(0.005, 0.05)
+        v0 = 0.02 + (0.2 - 0.02) * r / max(1, nrep - 1)
+        Dr = (0.005, 0.01, 0.02)[r % 3]
+        b, box = F.hexagonal_monolayer(nx, ny, packing=packing, maxmode=6, seed=r)
+        reps.append(F.Replica(b, box, types=(F.BodyType(v0=v0, Dr=Dr, muab=0.1),),
+                              energies=(F.Tension(eta), F.Overlap(1.0)), dt=0.005, seed=r, skin=0.3))
+    return reps
+
+
+def bench_ensemble(steps, warmup, independent_steps, workers=16):
+    """Ensemble throughput (all replicas in one launch per kernel and step,
+    graph mode, launch shape from the autotuner) next to the same replicas
+    as independent simulations, each run by its own host thread on its own
+    stream (the reference's one-task-per-simulation pattern)."""
+    import torch
+    from concurrent.futures import ThreadPoolExecutor
+    reps = ensemble_replicas()
+    total = sum(len(rp.bodies) for rp in reps)
+    ens = F.Ensemble(reps, precision="fp32")
+    ens.run(warmup)
+    launch, tune_ms = ens.autotune(20)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fms, ums = ens.run_timed(steps)
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    value = total * steps / sec
+    builds = sum(m.counters().builds for m in ens.members)
+    ens.close()
+    print(f"[bench] ensemble: {len(reps)} replicas, {total} bodies, {value / 1e6:.1f} M body-steps/s "
+          f"({sec / steps * 1e3:.4f} ms/step; forces {fms * 1e3:.1f} us, update {ums * 1e3:.1f} us), "
+          f"launch {launch}, {builds} rebuilds", file=sys.stderr, flush=True)
+    sims = [F.Simulation(rp.bodies, rp.box, types=rp.types, energies=rp.energies, dt=rp.dt, ntheta=40,
+                         precision="fp32", seed=rp.seed, skin=rp.skin) for rp in reps]
+    with ThreadPoolExecutor(max_workers=workers) as pool:
+        list(pool.map(lambda s: F.run(s, warmup), sims))
+        for s in sims:
+            s.counters()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        list(pool.map(lambda s: F.run(s, independent_steps), sims))
+        for s in sims:
+            s.counters()
+        torch.cuda.synchronize()
+        isec = time.perf_counter() - t0
+    ivalue = total * independent_steps / isec
+    for s in sims:
+        s.close()
+    print(f"[bench] independent simulations ({workers} host threads, one stream each): "
+          f"{ivalue / 1e6:.1f} M body-steps/s", file=sys.stderr, flush=True)
+    return {"value": value, "unit": "body-steps/s", "replicas": len(reps), "bodies": total,
+            "steps": steps, "ms_per_step": sec / steps * 1e3, "forces_ms": fms, "update_ms": ums,
+            "launch": launch, "autotune_ms_per_step": tune_ms, "rebuilds": builds,
+            "independent_value": ivalue, "independent_steps": independent_steps,
+            "independent_threads": workers, "speedup_vs_independent": value / ivalue,
+            "workload": "64 replicas, N log-uniform [1024,16384], packing U[0.9,1], v0 0.02..0.2, "
+                        "Dr {0.005,0.01,0.02}, Tension(U[0.005,0.05])+Overlap(1), modes 2..6, FP32, "
+                        "dt=0.005, key = replica id"}
+
+
 def cpu_baseline(n, seconds=15.0):
     """FP64 oracle (C restatement of the reference CPU path, OpenMP over
     bodies) on the same workload: 1 warm-up step, then steps until about
@@ -151,6 +224,10 @@ def main():
     ap.add_argument("--precision", default="mixed")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ensemble-steps", type=int, default=2000)
+    ap.add_argument("--ensemble-only", action="store_true",
+                    help="only the ensemble leg (its own JSON line)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration)")
@@ -161,6 +238,11 @@ def main():
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy()
+        return
+
+    if args.ensemble_only:
+        print(json.dumps(bench_ensemble(args.ensemble_steps, args.warmup, max(1, args.ensemble_steps // 4))),
+              flush=True)
         return
 
     print(f"[bench] workload N={args.n ** 2} precision={args.precision}", file=sys.stderr, flush=True)
@@ -190,6 +272,10 @@ def main():
     upd_bytes = 340 * sim.N  # per body-step, 188 B read + 152 B written
     print(f"[bench] forces {fms * 1e3:.1f} us, update {ums * 1e3:.1f} us, pairs tested {tested}, "
           f"in range {inr}", file=sys.stderr, flush=True)
+
+    ensemble = None
+    if not args.no_ensemble:
+        ensemble = bench_ensemble(args.ensemble_steps, min(args.warmup, 200), max(1, args.ensemble_steps // 4))
 
     cpu = None if args.no_cpu else cpu_baseline(args.n, args.cpu_seconds)
 
@@ -222,6 +308,7 @@ def main():
                             "unit": "GB/s", "frac": upd_bytes / (ums * 1e-3) / 1e9 / PEAK_HBM_GBS,
                             "launch_ms": ums},
         "cpu_baseline": cpu,
+        "ensemble": ensemble,
     }
     print(json.dumps(line), flush=True)
 

@@ -642,6 +642,26 @@ struct Sim final : fcc_sim {
         return FCC_OK;
     }
 
+    // The same build enqueued on a side stream `sb` without waiting for it
+    // (ensemble runs): the replica stays frozen by its rebuild flag, which
+    // k_finish_build releases if the list fits its capacity and `limit`.
+    int rebuild_async(hipStream_t sb, int32_t limit) {
+        const unsigned gN = blocks(N, 256), gC = blocks(nce, 256);
+        hipLaunchKernelGGL(k_fill_i32, gC, 256, 0, sb, nce, ccnt, 0);
+        hipLaunchKernelGGL(k_cell_index, gN, 256, 0, sb, N, d.x, d.y, ncx, ncy, wx, wy, cellid, ccnt);
+        hipLaunchKernelGGL(k_scan, 1, 1024, 0, sb, nce, ccnt, cstart, cfill);
+        hipLaunchKernelGGL(k_scatter, gN, 256, 0, sb, N, cellid, cfill, perm);
+        hipLaunchKernelGGL(k_cell_sort, gC, 256, 0, sb, nce, cstart, ccnt, perm);
+        hipLaunchKernelGGL(k_begin_build, 1, 1, 0, sb, d.st);
+        hipLaunchKernelGGL((k_neighbors<T>), gN, 256, 0, sb, N, d.x, d.y, d.rmax, perm, cstart, ccnt, p.Lx, p.Ly,
+                           ncx, ncy, wx, wy, range + skin, d.cap, d.nidx, d.ncount, d.st);
+        if (skin > 0.0)
+            hipLaunchKernelGGL((k_copy_skin_ref<T>), gN, 256, 0, sb, N, d.x, d.y, d.rmax, d.x0, d.y0, d.rmax0);
+        hipLaunchKernelGGL(k_finish_build, 1, 1, 0, sb, d.st, d.cap, limit);
+        HIPCHK(hipGetLastError());
+        return FCC_OK;
+    }
+
     // update_neighbors!(sim; force) (src/simulation.jl:132-142). The skin
     // check of the current state is the flag raised by the fused check of the
     // last update (the same test on the same state).

@@ -19,11 +19,13 @@ constexpr int MAXB = 4;  // native body terms per simulation
 // counts the executed steps and is the Philox draw index. `stop` is raised by
 // the update kernel once `clock` reaches `target` (ensemble runs give every
 // replica its own step budget); a raised `rebuild` or `stop` freezes the
-// following steps.
+// following steps. `builds` counts the list builds run on the device side of
+// an ensemble (the host compares it with the builds it has issued).
 struct DevStatus {
     int32_t rebuild, update, overflow, stop;
     int64_t clock;
     int64_t target;
+    int32_t builds, pad;
 };
 
 // Status prologue of the first kernel of a step: block 0 mirrors the freeze
@@ -36,6 +38,15 @@ __device__ __forceinline__ bool step_frozen(DevStatus *st, int64_t blk, int lane
         if (fr == 0) st->clock += 1;
     }
     return fr != 0;
+}
+
+// Status check of the forces kernel: use_status 1 = the kernel opens the step
+// (step_frozen); 2 = a gate kernel already took the step's freeze decision
+// into `update` (ensemble steps, whose lists are rebuilt concurrently).
+__device__ __forceinline__ bool forces_frozen(int use_status, DevStatus *st, int64_t blk, int lane) {
+    if (use_status == 1) return step_frozen(st, blk, lane);
+    if (use_status == 2) return st->update != 0;
+    return false;
 }
 
 // Scalar parameters of one simulation, passed by value to every kernel.

@@ -78,18 +78,30 @@ struct Ensemble final : fcc_ensemble {
     std::vector<double> interval;  // mean steps between rebuilds, per replica
     std::vector<int64_t> since;
     bool want_timed = false;
-    hipEvent_t tev[3] = {nullptr, nullptr, nullptr};
     double acc_f = 0.0, acc_u = 0.0;
     int64_t acc_n = 0;
+    // pipelined runs: list builds on a side stream, status snapshots of the
+    // chunks in flight in pinned host memory
+    static constexpr int NB = 4;
+    hipStream_t side = nullptr;
+    DevStatus *snap = nullptr;
+    hipEvent_t sev[NB] = {};
+    hipEvent_t tev[NB][3] = {};
 
     ~Ensemble() override {
         if (stream) (void)hipStreamSynchronize(stream);
+        if (side) (void)hipStreamSynchronize(side);
         if (gexec) (void)hipGraphExecDestroy(gexec);
-        for (hipEvent_t e : tev)
-            if (e) (void)hipEventDestroy(e);
+        for (int q = 0; q < NB; q++) {
+            if (sev[q]) (void)hipEventDestroy(sev[q]);
+            for (hipEvent_t e : tev[q])
+                if (e) (void)hipEventDestroy(e);
+        }
+        if (snap) (void)hipHostFree(snap);
         for (Sim<T> *s : m) delete s;
         for (void *q : {(void *)st, (void *)dP, (void *)dD, (void *)tf, (void *)tu})
             if (q) (void)hipFree(q);
+        if (side) (void)hipStreamDestroy(side);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -128,6 +140,13 @@ struct Ensemble final : fcc_ensemble {
 
     // ------------------------------------------------------------ launches
 
+    int launch_gate() {
+        const int R = (int)m.size();
+        hipLaunchKernelGGL(k_gate, blocks(R, 64), 64, 0, stream, st, R);
+        HIPCHK(hipGetLastError());
+        return FCC_OK;
+    }
+
     int launch_forces() {
         const Sim<T> &a = *m[0];
         const int dbg = a.debug ? 1 : 0;
@@ -136,9 +155,9 @@ struct Ensemble final : fcc_ensemble {
                 const bool bf = a.cfg.precision == FCC_MIXED_BF16;
 #define FCC_MFE(KV, TV, NCV)                                                                                  \
     if (bf) hipLaunchKernelGGL((k_forces_mfma_ens<KV, TV, NCV, true>), gf, 64, 0, stream, dP, dD, tf, a.asyn,  \
-                               a.bpull, 1, dbg);                                                               \
+                               a.bpull, 2, dbg);                                                               \
     else hipLaunchKernelGGL((k_forces_mfma_ens<KV, TV, NCV, false>), gf, 64, 0, stream, dP, dD, tf, a.asyn,   \
-                            a.bpull, 1, dbg);
+                            a.bpull, 2, dbg);
 #define FCC_MFE_T(KV, NCV) if (a.mfma_T == 3) { FCC_MFE(KV, 3, NCV) } else { FCC_MFE(KV, 4, NCV) }
                 if (a.K == 5) {
                     if (ncls <= 1) { FCC_MFE_T(5, 1) } else { FCC_MFE_T(5, 2) }
@@ -153,8 +172,8 @@ struct Ensemble final : fcc_ensemble {
         }
         if (var == 1) {
 #define FCC_FE(KV, NC)                                                                                        \
-    if (a.nth == 40) hipLaunchKernelGGL((k_forces_fast_ens<T, 40, KV, NC>), gf, 64, 0, stream, dP, dD, tf, 1, dbg); \
-    else hipLaunchKernelGGL((k_forces_fast_ens<T, 0, KV, NC>), gf, 64, 0, stream, dP, dD, tf, 1, dbg);
+    if (a.nth == 40) hipLaunchKernelGGL((k_forces_fast_ens<T, 40, KV, NC>), gf, 64, 0, stream, dP, dD, tf, 2, dbg); \
+    else hipLaunchKernelGGL((k_forces_fast_ens<T, 0, KV, NC>), gf, 64, 0, stream, dP, dD, tf, 2, dbg);
             if (a.K == 5) {
                 if (ncls <= 1) { FCC_FE(5, 1) } else { FCC_FE(5, 2) }
             } else {
@@ -165,8 +184,8 @@ struct Ensemble final : fcc_ensemble {
             return FCC_OK;
         }
 #define FCC_GE(KMV)                                                                                           \
-    if (a.nth == 40) hipLaunchKernelGGL((k_forces_ens<T, 40, KMV>), gf, 64, 0, stream, dP, dD, tf, 1, dbg);   \
-    else hipLaunchKernelGGL((k_forces_ens<T, 0, KMV>), gf, 64, 0, stream, dP, dD, tf, 1, dbg);
+    if (a.nth == 40) hipLaunchKernelGGL((k_forces_ens<T, 40, KMV>), gf, 64, 0, stream, dP, dD, tf, 2, dbg);   \
+    else hipLaunchKernelGGL((k_forces_ens<T, 0, KMV>), gf, 64, 0, stream, dP, dD, tf, 2, dbg);
         switch (a.KM) {
         case 5: FCC_GE(5) break;
         case 7: FCC_GE(7) break;
@@ -263,7 +282,8 @@ struct Ensemble final : fcc_ensemble {
             }
             hipGraph_t graph;
             HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-            int rc = launch_forces();
+            int rc = launch_gate();
+            if (!rc) rc = launch_forces();
             if (!rc) rc = launch_update();
             hipError_t e = hipStreamEndCapture(stream, &graph);
             if (rc) return rc;
@@ -306,66 +326,135 @@ struct Ensemble final : fcc_ensemble {
         double mi = 1e300;
         for (size_t r = 0; r < m.size(); r++)
             if (m[r]->has_contact) mi = std::min(mi, interval[r]);
-        if (mi >= 1e300) return 256;
-        return std::min<int64_t>(64, std::max<int64_t>(4, (int64_t)std::llround(mi / 8)));
+        if (mi >= 1e300) return 64;
+        return std::min<int64_t>(16, std::max<int64_t>(2, (int64_t)std::llround(mi / 16)));
     }
 
+    int ensure_async() {
+        if (side) return FCC_OK;
+        HIPCHK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+        HIPCHK(hipHostMalloc((void **)&snap, sizeof(DevStatus) * NB * m.size(), hipHostMallocDefault));
+        for (int q = 0; q < NB; q++) {
+            HIPCHK(hipEventCreateWithFlags(&sev[q], hipEventDisableTiming));
+            for (auto &e : tev[q]) HIPCHK(hipEventCreate(&e));
+        }
+        return FCC_OK;
+    }
+
+    // k steps on the main stream, then a snapshot of every status slot.
+    int issue(int64_t k, int slot) {
+        const size_t R = m.size();
+        if (!want_timed) {
+            for (int64_t q = 0; q < k; q++) HIPCHK(hipGraphLaunch(gexec, stream));
+        } else {
+            // the chunk's last step eagerly between events: the kernels and
+            // arguments of the captured step
+            for (int64_t q = 0; q + 1 < k; q++) HIPCHK(hipGraphLaunch(gexec, stream));
+            int rc = launch_gate();
+            if (rc) return rc;
+            HIPCHK(hipEventRecord(tev[slot][0], stream));
+            if ((rc = launch_forces())) return rc;
+            HIPCHK(hipEventRecord(tev[slot][1], stream));
+            if ((rc = launch_update())) return rc;
+            HIPCHK(hipEventRecord(tev[slot][2], stream));
+        }
+        HIPCHK(hipMemcpyAsync(snap + (size_t)slot * R, st, sizeof(DevStatus) * R, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipEventRecord(sev[slot], stream));
+        return FCC_OK;
+    }
+
+    // run! of every replica, pipelined: the host keeps two chunks queued on
+    // the main stream and reads each chunk's status snapshot while the next
+    // one runs. A replica whose skin tripped stays frozen (its workgroups
+    // return at once) while its list is rebuilt on the side stream; the gate
+    // kernel of the first step after the build's release lets it go on, so
+    // it executes the same steps and builds as alone, and the device never
+    // waits for the host. Capacity overflows and kernel switches (rare) are
+    // handled synchronously.
     int advance(int64_t n) {
         std::vector<DevStatus> hs;
         int rc = refresh_lists(hs);
         if (rc) return rc;
-        if ((rc = set_budgets(n)) || (rc = read_status(hs))) return rc;
+        if ((rc = set_budgets(n)) || (rc = read_status(hs)) || (rc = ensure_async()) || (rc = prepare())) return rc;
         const size_t R = m.size();
         std::vector<int64_t> clock(R), target(R);
+        std::vector<int64_t> trip(R);  // clock at the last build issued
+        std::vector<int32_t> seen(R), issued_builds(R);
         for (size_t r = 0; r < R; r++) {
-            clock[r] = hs[r].clock;
+            clock[r] = trip[r] = hs[r].clock;
             target[r] = hs[r].target;
+            seen[r] = issued_builds[r] = hs[r].builds;
         }
-        for (;;) {
-            int64_t remaining = 0;
-            for (size_t r = 0; r < R; r++) remaining = std::max(remaining, target[r] - clock[r]);
-            if (remaining <= 0) break;
-            if ((rc = prepare())) return rc;
-            const int64_t k = std::min(chunk_len(), remaining);
-            if (!want_timed) {
-                for (int64_t q = 0; q < k; q++) HIPCHK(hipGraphLaunch(gexec, stream));
-            } else {
-                // the chunk's last step eagerly between events: the kernels
-                // and arguments of the captured step
-                if (!tev[0])
-                    for (auto &ev : tev) HIPCHK(hipEventCreate(&ev));
-                for (int64_t q = 0; q + 1 < k; q++) HIPCHK(hipGraphLaunch(gexec, stream));
-                HIPCHK(hipEventRecord(tev[0], stream));
-                if ((rc = launch_forces())) return rc;
-                HIPCHK(hipEventRecord(tev[1], stream));
-                if ((rc = launch_update())) return rc;
-                HIPCHK(hipEventRecord(tev[2], stream));
+        constexpr int32_t NLIM = 1 << NPLANE;  // neighbour limit of the fast kernels
+        int64_t issued = 0, read = 0, stale_before = 0;
+        bool finished = false;
+        while (!finished || read < issued) {
+            while (!finished && issued - read < 2) {
+                if ((rc = issue(chunk_len(), (int)(issued % NB)))) return rc;
+                issued++;
             }
-            if ((rc = read_status(hs))) return rc;
+            const int slot = (int)(read % NB);
+            HIPCHK(hipEventSynchronize(sev[slot]));
+            const DevStatus *s = snap + (size_t)slot * R;
+            const bool stale = read < stale_before;
+            read++;
             if (want_timed) {
                 float a = 0.f, b = 0.f;
-                HIPCHK(hipEventElapsedTime(&a, tev[0], tev[1]));
-                HIPCHK(hipEventElapsedTime(&b, tev[1], tev[2]));
+                HIPCHK(hipEventElapsedTime(&a, tev[slot][0], tev[slot][1]));
+                HIPCHK(hipEventElapsedTime(&b, tev[slot][1], tev[slot][2]));
                 acc_f += a;
                 acc_u += b;
                 acc_n++;
             }
+            bool all = true;
+            std::vector<size_t> special;
             for (size_t r = 0; r < R; r++) {
-                Sim<T> &s = *m[r];
-                const int64_t ex = hs[r].clock - clock[r];
-                clock[r] = hs[r].clock;
-                for (int64_t q = 0; q < ex; q++) s.t += s.cfg.dt;
-                s.nstep += ex;
+                Sim<T> &sim = *m[r];
+                const int64_t ex = s[r].clock - clock[r];
+                clock[r] = s[r].clock;
+                for (int64_t q = 0; q < ex; q++) sim.t += sim.cfg.dt;
+                sim.nstep += ex;
                 since[r] += ex;
-                if (hs[r].rebuild != 0 && clock[r] < target[r]) {
-                    if ((rc = s.rebuild())) return rc;
-                    interval[r] = (interval[r] + (double)since[r]) / 2;
-                    since[r] = 0;
-                } else if ((double)since[r] > interval[r]) {
-                    interval[r] = std::min((double)since[r], 4.0 * 256);
+                if (s[r].builds != seen[r]) {  // builds completed on the side stream
+                    sim.builds += s[r].builds - seen[r];
+                    seen[r] = s[r].builds;
+                    sim.maxcount = s[r].overflow;
                 }
+                if (clock[r] < target[r]) all = false;
+                if (stale || seen[r] != issued_builds[r] || s[r].rebuild == 0 || clock[r] >= target[r]) continue;
+                if (clock[r] == trip[r]) {
+                    // no step since the last build: either that build kept the
+                    // replica frozen (it did not fit) or its release is not in
+                    // this snapshot yet
+                    if (s[r].overflow > sim.d.cap || (var != 0 && s[r].overflow >= NLIM)) special.push_back(r);
+                    continue;
+                }
+                if ((rc = sim.rebuild_async(side, var != 0 ? NLIM : INT32_MAX))) return rc;
+                issued_builds[r]++;
+                trip[r] = clock[r];
+                interval[r] = (interval[r] + (double)since[r]) / 2;
+                since[r] = 0;
             }
+            if (!special.empty()) {
+                HIPCHK(hipStreamSynchronize(stream));
+                HIPCHK(hipStreamSynchronize(side));
+                for (size_t r : special) {
+                    Sim<T> &sim = *m[r];
+                    if (s[r].overflow > sim.d.cap) {
+                        sim.builds--;  // the overflowed build is redone, as the regrow loop of Sim::rebuild
+                        if ((rc = sim.rebuild())) return rc;
+                    } else {
+                        sim.maxcount = s[r].overflow;
+                        hipLaunchKernelGGL(k_clear_status, 1, 1, 0, stream, sim.d.st);
+                        HIPCHK(hipGetLastError());
+                    }
+                }
+                if ((rc = prepare())) return rc;
+                stale_before = issued;  // snapshots already queued predate these fixes
+            }
+            finished = all;
         }
+        HIPCHK(hipStreamSynchronize(side));
         return FCC_OK;
     }
 
@@ -384,6 +473,10 @@ struct Ensemble final : fcc_ensemble {
         want_timed = true;
         acc_f = acc_u = 0.0;
         acc_n = 0;
+        if (int rc = ensure_async()) {
+            want_timed = false;
+            return rc;
+        }
         const int rc = run(n);
         want_timed = false;
         if (rc) return rc;

@@ -137,7 +137,7 @@ __device__ __forceinline__ void k_forces_fast_impl(const PP &p, const DP &d, int
     }
     __syncthreads();
     const int64_t i = (int64_t)blk * 64 + lane;
-    if (use_status && step_frozen(d.st, blk, lane)) return;
+    if (forces_frozen(use_status, d.st, blk, lane)) return;
     if (i >= p.N) return;
     const FCC_CONST T *S = as_const(d.Sc);
     const FCC_CONST T *CS = as_const(d.cs);

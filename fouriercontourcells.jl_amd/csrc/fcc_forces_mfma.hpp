@@ -109,7 +109,7 @@ __device__ __forceinline__ void k_forces_mfma_impl(const PP &p, const DP &d,
         scs[1][k] = d.cs[2 * k + 1];
     }
     const int64_t i = (int64_t)blk * 32 + b;
-    if (use_status && step_frozen(d.st, blk, lane)) return;  // uniform over the grid
+    if (forces_frozen(use_status, d.st, blk, lane)) return;  // uniform over the grid
     const bool live = i < p.N;
     const int64_t ii = live ? i : 0;
 

@@ -56,7 +56,7 @@ __device__ __forceinline__ void k_forces_impl(const PP &p, const DP &d, int use_
     }
     __syncthreads();
     const int64_t i = (int64_t)blk * 64 + lane;
-    if (use_status && step_frozen(d.st, blk, lane)) return;
+    if (forces_frozen(use_status, d.st, blk, lane)) return;
     if (i >= p.N) return;
     const FCC_CONST T *S = as_const(d.S);
     T lab[NCM];
@@ -680,6 +680,32 @@ __global__ void k_clear_status(DevStatus *st) {
 __global__ void k_set_target(DevStatus *st, int64_t target) {
     st->target = target;
     st->stop = st->clock >= target ? 1 : 0;
+}
+
+// Step opening of an ensemble: the freeze decision of every replica (its
+// rebuild flag read with acquire order, so that lists completed by a build on
+// another stream are visible to the kernels that follow) and the clock tick.
+__global__ void k_gate(DevStatus *st, int n) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int32_t rb = __hip_atomic_load(&st[r].rebuild, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    const int32_t fr = rb | st[r].stop;
+    st[r].update = fr;
+    if (fr == 0) st[r].clock += 1;
+}
+
+// Start of a list build on the side stream: the neighbour kernel's maximum.
+__global__ void k_begin_build(DevStatus *st) { st->overflow = 0; }
+
+// End of a list build on the side stream: count it and release the replica
+// (rebuild = 0, release order) unless the list overflowed its capacity or
+// holds more neighbours than the forces kernel in use accepts (`limit`); the
+// host then regrows or switches kernels.
+__global__ void k_finish_build(DevStatus *st, int32_t cap, int32_t limit) {
+    const int32_t n = st->overflow;
+    st->builds += 1;
+    const int32_t keep = (n > cap || n >= limit) ? 1 : 0;
+    __hip_atomic_store(&st->rebuild, keep, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Step budgets of the n replicas of an ensemble run: target = clock + steps
