@@ -263,6 +263,8 @@ struct Sim final : fcc_sim {
         p.dphi = (T)dphi;
         p.Lx = c->Lx;
         p.Ly = c->Ly;
+        p.invLx = 1.0 / c->Lx;
+        p.invLy = 1.0 / c->Ly;
         // terms (src/simulation.jl:73-76)
         int nb = 0;
         int overlaps[FCC_MAX_TERMS];
@@ -314,6 +316,11 @@ struct Sim final : fcc_sim {
                     }
             p.ncls = (int)vals.size();
             for (size_t q = 0; q < vals.size(); q++) p.Jcls[q] = (T)vals[q];
+            p.cls_uniform = 1;
+            for (int rel = 0; rel < 2; rel++)
+                for (int ti = 0; ti < c->n_types; ti++)
+                    for (int tj = 0; tj < c->n_types; tj++)
+                        if (p.ccls[rel][ti][tj] != 0) p.cls_uniform = 0;
         }
         range = 0.0;  // interaction_range(Overlap) = 0
         p.range = (T)range;

@@ -80,6 +80,8 @@ struct Params {
     int ccls[2][FCC_MAX_TYPES][FCC_MAX_TYPES];
     T Jcls[2 * FCC_MAX_TYPES * FCC_MAX_TYPES];
     int small_box;  // a node offset may need a second minimum image
+    double invLx, invLy;  // 1/L: minimum image of neighbour offsets outside small boxes
+    int cls_uniform;      // every pair of types (related or not) couples in class 0
 };
 
 // Device buffers of one simulation. Per-body coefficient rows are stored
@@ -194,6 +196,13 @@ __device__ __forceinline__ void philox_pair(uint32_t k0, uint32_t k1, uint32_t i
 template <typename T>
 __device__ __forceinline__ T min_image(T d, T L) {
     return d - L * rint(d / L);
+}
+
+// minimum_image of a neighbour offset |d| < L/2 - cutoff or > L/2 + cutoff
+// (boxes that are not small): d·(1/L) rounds to the integer d/L rounds to, so
+// the result equals min_image's.
+__device__ __forceinline__ double min_image_nb(double d, double L, double invL, bool small) {
+    return small ? min_image(d, L) : d - L * rint(d * invL);
 }
 
 // wrap (src/box.jl:21-30).

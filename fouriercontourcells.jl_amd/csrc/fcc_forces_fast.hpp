@@ -201,10 +201,13 @@ __device__ __forceinline__ void k_forces_fast_impl(const PP &p, const DP &d, int
         if (cnt > d.cap) cnt = d.cap;
         for (int q = 0; q < cnt; q++) {
             const int j = d.nidx[(int64_t)q * p.N + i];
-            const int cls = p.ccls[pi == d.parent[j] ? 1 : 0][ti][d.type[j]];
-            if (cls < 0) continue;  // zero coupling: no contribution
-            const T dxij = (T)min_image(xi - d.x[j], p.Lx);
-            const T dyij = (T)min_image(yi - d.y[j], p.Ly);
+            int cls = 0;
+            if (!p.cls_uniform) {
+                cls = p.ccls[pi == d.parent[j] ? 1 : 0][ti][d.type[j]];
+                if (cls < 0) continue;  // zero coupling: no contribution
+            }
+            const T dxij = (T)min_image_nb(xi - d.x[j], p.Lx, p.invLx, wrap2);
+            const T dyij = (T)min_image_nb(yi - d.y[j], p.Ly, p.invLy, wrap2);
             const T lim = d.rmax[j] + p.range;
             const T lim2 = lim * lim;
             const T D2 = dxij * dxij + dyij * dyij;

@@ -9,11 +9,12 @@
 // 8(r/4) + 4h + r%4 (r = 0..15) of column l % 32 of each 32-row tile; the basis
 // rows are permuted on the host so that the slots σ = 16t + r of lane half h
 // are R (σ < 20), R' (σ < 40) and R'' (σ < 60, with Bending) at its nodes.
-// After the synthesis the lane stores its R, R' (R'') in LDS ([node][body],
-// 5 KB each per wave), which frees the accumulators during the contact pass;
-// pass 2 overwrites them with the contour gradients, which the pullback reads
-// back as its A operand (the row slots of a K-chunk are exactly the lane's 8
-// rows of that chunk).
+// Only R goes to LDS ([node][body], 5 KB per wave) for the contact pass,
+// which reads the whole contour of the body; after the contact pass the
+// synthesis is run a second time (a few MFMAs), and the contour gradients
+// replace R, R', R'' in the same registers, where the pullback takes its A
+// operand (the row slots of a K-chunk are exactly the lane's 8 rows of that
+// chunk). The small LDS footprint keeps five waves per SIMD.
 //
 // hi+lo: x = hi + lo with hi = rn(x), lo = rn(x - hi) in the 16-bit type;
 // A·B ≈ Ah·Bh + Ah·Bl + Al·Bh. lab is pre-scaled by 2^10 and G by 2^8 (powers
@@ -88,6 +89,38 @@ __device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
     return ((uint64_t)hi << 32) | lo;
 }
 
+// Synthesis on MFMA: lane (b, h) supplies lab coefficients 8h..8h+7 of body
+// b (pre-scaled by SYN_SCALE); acc[t] receives tile t of [R; R'; R''] (slots
+// σ = 16t + r), scaled back to R0·(S·lab).
+template <int K, int T, bool BF>
+__device__ __forceinline__ void synth_tiles(const float *lab_row, bool live, int h, int lane, float unsc,
+                                            const uint16_t *asyn, f16v_t (&acc)[T]) {
+    typedef MfmaType<BF> MT;
+    typedef typename MT::v_t v_t;
+    constexpr int NC = 2 * K + 1;
+    float x[8];
+    const float4 *row = reinterpret_cast<const float4 *>(lab_row + 8 * h);
+    const float4 u0 = row[0], u1 = row[1];
+    x[0] = u0.x; x[1] = u0.y; x[2] = u0.z; x[3] = u0.w;
+    x[4] = u1.x; x[5] = u1.y; x[6] = u1.z; x[7] = u1.w;
+#pragma unroll
+    for (int s = 0; s < 8; s++) x[s] = (live && 8 * h + s < NC) ? x[s] * SYN_SCALE : 0.f;
+    v_t bh, bl;
+    split8<BF>(x, bh, bl);
+#pragma unroll
+    for (int t = 0; t < T; t++) {
+        const v_t ah = load_op<BF>(asyn + ((t * 2 + 0) * 64 + lane) * 8);
+        const v_t al = load_op<BF>(asyn + ((t * 2 + 1) * 64 + lane) * 8);
+        f16v_t c = {};
+        c = MT::mfma(ah, bh, c);
+        c = MT::mfma(ah, bl, c);
+        c = MT::mfma(al, bh, c);
+#pragma unroll
+        for (int r = 0; r < 16; r++) c[r] *= unsc;
+        acc[t] = c;
+    }
+}
+
 // Operand tables (built on the host, 16-bit values):
 //  asyn[(t * 2 + part) * 64 + lane][8]   synthesis A (S rows), part 0 hi / 1 lo
 //  bpull[(c * 2 + v) * 64 + lane][8]     pullback B, v 0 = [Sh | Sl], 1 = [Sh | 0]
@@ -99,9 +132,8 @@ __device__ __forceinline__ void k_forces_mfma_impl(const PP &p, const DP &d,
     typedef typename MT::v_t v_t;
     constexpr int NC = 2 * K + 1;
     constexpr int W = row_width<NC, float>();
-    __shared__ float sQ[T == 4 ? 3 : 2][NTH_M][32];  // R, R' (R''); then GR, GD (GDD)
+    __shared__ float sR[NTH_M][32];  // R of the wave's 32 bodies for the contact pass
     __shared__ float scs[2][NTH_M];
-    float(&sR)[NTH_M][32] = sQ[0];
     const int lane = threadIdx.x;
     const int b = lane & 31, h = lane >> 5;
     for (int k = lane; k < NTH_M; k += 64) {
@@ -112,38 +144,15 @@ __device__ __forceinline__ void k_forces_mfma_impl(const PP &p, const DP &d,
     if (forces_frozen(use_status, d.st, blk, lane)) return;  // uniform over the grid
     const bool live = i < p.N;
     const int64_t ii = live ? i : 0;
-
-    // ---- synthesis on MFMA: lane (b, h) supplies lab coefficients 8h..8h+7 of body b
-    f16v_t acc[T];
-    {
-        float x[8];
-        const float4 *row = reinterpret_cast<const float4 *>(d.lab + ii * p.ldc + 8 * h);
-        const float4 u0 = row[0], u1 = row[1];
-        x[0] = u0.x; x[1] = u0.y; x[2] = u0.z; x[3] = u0.w;
-        x[4] = u1.x; x[5] = u1.y; x[6] = u1.z; x[7] = u1.w;
-#pragma unroll
-        for (int s = 0; s < 8; s++) x[s] = (live && 8 * h + s < NC) ? x[s] * SYN_SCALE : 0.f;
-        v_t bh, bl;
-        split8<BF>(x, bh, bl);
-#pragma unroll
-        for (int t = 0; t < T; t++) {
-            const v_t ah = load_op<BF>(asyn + ((t * 2 + 0) * 64 + lane) * 8);
-            const v_t al = load_op<BF>(asyn + ((t * 2 + 1) * 64 + lane) * 8);
-            f16v_t c = {};
-            c = MT::mfma(ah, bh, c);
-            c = MT::mfma(ah, bl, c);
-            c = MT::mfma(al, bh, c);
-            acc[t] = c;
-        }
-    }
     const float r0 = live ? d.R0[ii] : 1.f;
     const float unsc = r0 * (1.f / SYN_SCALE);
+    const float *lab_row = d.lab + ii * p.ldc;
+
+    // ---- synthesis on MFMA
+    f16v_t acc[T];
+    synth_tiles<K, T, BF>(lab_row, live, h, lane, unsc, asyn, acc);
     // slot σ = 16t + r → acc[t][r]; R at σ = j, R' at 20 + j, R'' at 40 + j
 #define SLOT(sig) acc[(sig) >> 4][(sig)&15]
-#pragma unroll
-    for (int t = 0; t < T; t++)
-#pragma unroll
-        for (int r = 0; r < 16; r++) acc[t][r] *= unsc;
 
     const int ti = live ? d.type[ii] : 0;
     const uint32_t tbit = 1u << ti;
@@ -156,11 +165,8 @@ __device__ __forceinline__ void k_forces_mfma_impl(const PP &p, const DP &d,
         const float R = SLOT(j), dR = SLOT(NH + j);
         Qp += f_sqrt(R * R + dR * dR);
         if constexpr (T == 4) Qb += term_q<float>(FCC_TERM_BENDING, R, dR, SLOT(2 * NH + j));
-        sQ[0][h * NH + j][b] = R;
-        sQ[1][h * NH + j][b] = dR;
-        if constexpr (T == 4) sQ[2][h * NH + j][b] = SLOT(2 * NH + j);
+        sR[h * NH + j][b] = R;
     }
-#undef SLOT
     Qp += __shfl_xor(Qp, 32);
     if constexpr (T == 4) Qb += __shfl_xor(Qb, 32);
     float Wp = 0.f, Wb = 0.f, Et[MAXB];
@@ -200,10 +206,13 @@ __device__ __forceinline__ void k_forces_mfma_impl(const PP &p, const DP &d,
         if (cnt > d.cap) cnt = d.cap;
         for (int q = h; q < cnt; q += 2) {
             const int j = d.nidx[(int64_t)q * p.N + ii];
-            const int cls = p.ccls[pi == d.parent[j] ? 1 : 0][ti][d.type[j]];
-            if (cls < 0) continue;
-            const float dxij = (float)min_image(xi - d.x[j], p.Lx);
-            const float dyij = (float)min_image(yi - d.y[j], p.Ly);
+            int cls = 0;
+            if (!p.cls_uniform) {
+                cls = p.ccls[pi == d.parent[j] ? 1 : 0][ti][d.type[j]];
+                if (cls < 0) continue;
+            }
+            const float dxij = (float)min_image_nb(xi - d.x[j], p.Lx, p.invLx, wrap2);
+            const float dyij = (float)min_image_nb(yi - d.y[j], p.Ly, p.invLy, wrap2);
             const float lim = d.rmax[j] + p.range;
             const float lim2 = lim * lim;
             const float D2 = dxij * dxij + dyij * dyij;
@@ -281,17 +290,20 @@ __device__ __forceinline__ void k_forces_mfma_impl(const PP &p, const DP &d,
         planes_add(pl[c], other);
     }
 
-    // ---- pass 2: contour gradients of the lane's nodes (in place of R, R', R'' in LDS)
+    // ---- pass 2: synthesis again; the contour gradients of the lane's nodes
+    // replace R, R', R'' in their register slots
+    synth_tiles<K, T, BF>(lab_row, live, h, lane, unsc, asyn, acc);
     float fx = 0.f, fy = 0.f, tt = 0.f;
-#pragma unroll 4
+    const float sc = live ? PULL_SCALE : 0.f;
+#pragma unroll
     for (int j = 0; j < NH; j++) {
         const int k = h * NH + j;
-        const float R = sQ[0][k][b], dR = sQ[1][k][b];
+        const float R = SLOT(j), dR = SLOT(NH + j);
         const float rs = f_rsqrt(R * R + dR * dR);
         float GR = Wp * (R * rs), GD = Wp * (dR * rs), GDD = 0.f;
         if constexpr (T == 4) {
             float j0, j1, j2;
-            term_J<float>(FCC_TERM_BENDING, R, dR, sQ[T == 4 ? 2 : 0][k][b], j0, j1, j2);
+            term_J<float>(FCC_TERM_BENDING, R, dR, SLOT(2 * NH + j), j0, j1, j2);
             GR += Wb * j0;
             GD += Wb * j1;
             GDD = Wb * j2;
@@ -318,10 +330,9 @@ __device__ __forceinline__ void k_forces_mfma_impl(const PP &p, const DP &d,
             d.GD[ii * NTH_M + k] = GD;
             d.GDD[ii * NTH_M + k] = GDD;
         }
-        const float sc = live ? PULL_SCALE : 0.f;
-        sQ[0][k][b] = GR * sc;
-        sQ[1][k][b] = GD * sc;
-        if constexpr (T == 4) sQ[T == 4 ? 2 : 0][k][b] = GDD * sc;
+        SLOT(j) = GR * sc;
+        SLOT(NH + j) = GD * sc;
+        if constexpr (T == 4) SLOT(2 * NH + j) = GDD * sc;
     }
     fx += __shfl_xor(fx, 32);
     fy += __shfl_xor(fy, 32);
@@ -335,8 +346,7 @@ __device__ __forceinline__ void k_forces_mfma_impl(const PP &p, const DP &d,
 #pragma unroll
         for (int s = 0; s < 8; s++) {
             const int sig = 16 * (c >> 1) + 8 * (c & 1) + s;  // slot σ = 16t + r
-            const int qn = sig / NH, jn = sig % NH;
-            x[s] = qn < (T == 4 ? 3 : 2) ? sQ[qn < (T == 4 ? 3 : 2) ? qn : 0][h * NH + jn][b] : 0.f;
+            x[s] = sig < (T == 4 ? 3 : 2) * NH ? SLOT(sig) : 0.f;
         }
         v_t ah, al;
         split8<BF>(x, ah, al);
@@ -345,6 +355,7 @@ __device__ __forceinline__ void k_forces_mfma_impl(const PP &p, const DP &d,
         gacc = MT::mfma(ah, b1, gacc);
         gacc = MT::mfma(al, b2, gacc);
     }
+#undef SLOT
     // column n of lane l: n < 16 hi·hi + lo·hi of coefficient n, n ≥ 16 hi·lo of n - 16
 #pragma unroll
     for (int r = 0; r < 16; r++) gacc[r] += __shfl_down(gacc[r], 16);
