@@ -9,9 +9,12 @@ step over all bodies; steps run in graph mode (captured hipGraph replayed in
 chunks, one host sync per chunk) with the state resident in HBM.
 
 Prints one JSON line: the mixed-precision path's throughput (`value`), the
-plain FP32 path's for comparison, the roofline of the dominant kernel (forces,
-FP32 VALU) from HIP-event timings taken here, and the FP64 CPU baseline (the
-oracle, OpenMP over bodies) on a bounded sample of the same workload.
+plain FP32 path's for comparison, the roofline of the dominant kernel (forces)
+and of the update kernel from HIP-event timings taken here (back-to-back
+launches, fcc_time_kernels), the FP64 CPU baseline (the oracle, OpenMP over
+bodies) on a bounded sample of the same workload, and the batched-ensemble
+workload (64 replicas of uneven size in one launch per kernel and step) next
+to the same replicas run as independent simulations.
 """
 import argparse
 import json
@@ -59,6 +62,17 @@ def timed_run(sim, steps):
     if rc != 0:
         raise RuntimeError(sim.engine.last_error())
     return sec, f.value, u.value
+
+
+def kernel_durations(sim, reps=50):
+    """Average forces / update kernel durations (ms) from `reps` back-to-back
+    launches of each between HIP events on the simulation's stream
+    (fcc_time_kernels; the state is restored afterwards)."""
+    f, u = ctypes.c_double(), ctypes.c_double()
+    rc = sim.engine.time_kernels(sim._h, int(reps), ctypes.byref(f), ctypes.byref(u))
+    if rc != 0:
+        raise RuntimeError(sim.engine.last_error())
+    return f.value, u.value
 
 
 def calibrate_copy(nbytes=1 << 30, reps=5):
@@ -141,7 +155,7 @@ def ensemble_replicas(nrep=64, seed=4):
     return reps
 
 
-def bench_ensemble(steps, warmup, independent_steps, workers=16):
+def bench_ensemble(steps, warmup, independent_steps, workers=16, independent=True):
     """Ensemble throughput (all replicas in one launch per kernel and step,
     graph mode, launch shape from the autotuner) next to the same replicas
     as independent simulations, each run by its own host thread on its own
@@ -164,6 +178,14 @@ def bench_ensemble(steps, warmup, independent_steps, workers=16):
     print(f"[bench] ensemble: {len(reps)} replicas, {total} bodies, {value / 1e6:.1f} M body-steps/s "
           f"({sec / steps * 1e3:.4f} ms/step; forces {fms * 1e3:.1f} us, update {ums * 1e3:.1f} us), "
           f"launch {launch}, {builds} rebuilds", file=sys.stderr, flush=True)
+    out = {"value": value, "unit": "body-steps/s", "replicas": len(reps), "bodies": total,
+           "steps": steps, "ms_per_step": sec / steps * 1e3, "forces_ms": fms, "update_ms": ums,
+           "launch": launch, "autotune_ms_per_step": tune_ms, "rebuilds": builds,
+           "workload": "64 replicas, N log-uniform [1024,16384], packing U[0.9,1], v0 0.02..0.2, "
+                       "Dr {0.005,0.01,0.02}, Tension(U[0.005,0.05])+Overlap(1), modes 2..6, FP32, "
+                       "dt=0.005, key = replica id"}
+    if not independent:
+        return out
     sims = [F.Simulation(rp.bodies, rp.box, types=rp.types, energies=rp.energies, dt=rp.dt, ntheta=40,
                          precision="fp32", seed=rp.seed, skin=rp.skin) for rp in reps]
     with ThreadPoolExecutor(max_workers=workers) as pool:
@@ -182,14 +204,9 @@ def bench_ensemble(steps, warmup, independent_steps, workers=16):
         s.close()
     print(f"[bench] independent simulations ({workers} host threads, one stream each): "
           f"{ivalue / 1e6:.1f} M body-steps/s", file=sys.stderr, flush=True)
-    return {"value": value, "unit": "body-steps/s", "replicas": len(reps), "bodies": total,
-            "steps": steps, "ms_per_step": sec / steps * 1e3, "forces_ms": fms, "update_ms": ums,
-            "launch": launch, "autotune_ms_per_step": tune_ms, "rebuilds": builds,
-            "independent_value": ivalue, "independent_steps": independent_steps,
-            "independent_threads": workers, "speedup_vs_independent": value / ivalue,
-            "workload": "64 replicas, N log-uniform [1024,16384], packing U[0.9,1], v0 0.02..0.2, "
-                        "Dr {0.005,0.01,0.02}, Tension(U[0.005,0.05])+Overlap(1), modes 2..6, FP32, "
-                        "dt=0.005, key = replica id"}
+    out.update({"independent_value": ivalue, "independent_steps": independent_steps,
+                "independent_threads": workers, "speedup_vs_independent": value / ivalue})
+    return out
 
 
 def cpu_baseline(n, seconds=15.0):
@@ -228,6 +245,8 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=2000)
     ap.add_argument("--ensemble-only", action="store_true",
                     help="only the ensemble leg (its own JSON line)")
+    ap.add_argument("--no-independent", action="store_true",
+                    help="ensemble leg without the independent-simulations comparison")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration)")
@@ -241,15 +260,16 @@ def main():
         return
 
     if args.ensemble_only:
-        print(json.dumps(bench_ensemble(args.ensemble_steps, args.warmup, max(1, args.ensemble_steps // 4))),
-              flush=True)
+        print(json.dumps(bench_ensemble(args.ensemble_steps, args.warmup, max(1, args.ensemble_steps // 4),
+                                        independent=not args.no_independent)), flush=True)
         return
 
     print(f"[bench] workload N={args.n ** 2} precision={args.precision}", file=sys.stderr, flush=True)
     sim = make_sim(args.n, args.precision)
     F.run(sim, args.warmup)
     tested0, inr0 = contact_pairs(sim)
-    sec, fms, ums = timed_run(sim, args.steps)
+    sec, run_fms, run_ums = timed_run(sim, args.steps)
+    fms, ums = kernel_durations(sim)
     tested1, inr1 = contact_pairs(sim)
     tested, inr = (tested0 + tested1) / 2, (inr0 + inr1) / 2
     c = sim.counters()
@@ -270,8 +290,9 @@ def main():
     fkernel = "k_forces_mfma" if args.precision.startswith("mixed") else "k_forces_fast"
     achieved = flops / (fms * 1e-3) / 1e12
     upd_bytes = 340 * sim.N  # per body-step, 188 B read + 152 B written
-    print(f"[bench] forces {fms * 1e3:.1f} us, update {ums * 1e3:.1f} us, pairs tested {tested}, "
-          f"in range {inr}", file=sys.stderr, flush=True)
+    print(f"[bench] forces {fms * 1e3:.1f} us, update {ums * 1e3:.1f} us (back-to-back launches; "
+          f"last step of each chunk in the run: {run_fms * 1e3:.1f} / {run_ums * 1e3:.1f} us), "
+          f"pairs tested {tested}, in range {inr}", file=sys.stderr, flush=True)
 
     ensemble = None
     if not args.no_ensemble:
@@ -301,12 +322,12 @@ def main():
         "roofline": {"kernel": fkernel, "bound": "mfma", "achieved": achieved,
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                      "traffic": measured_traffic(fkernel) if sim.N == 262144 else None,
-                     "launch_ms": fms, "flops_per_launch": flops},
+                     "launch_ms": fms, "launch_ms_in_run": run_fms, "flops_per_launch": flops},
         "roofline_update": {"kernel": "k_update", "bound": "hbm",
                             "traffic": measured_traffic("k_update") if sim.N == 262144 else None,
                             "achieved": upd_bytes / (ums * 1e-3) / 1e9, "peak": PEAK_HBM_GBS,
                             "unit": "GB/s", "frac": upd_bytes / (ums * 1e-3) / 1e9 / PEAK_HBM_GBS,
-                            "launch_ms": ums},
+                            "launch_ms": ums, "launch_ms_in_run": run_ums},
         "cpu_baseline": cpu,
         "ensemble": ensemble,
     }

@@ -6,6 +6,7 @@
 // host read of the rebuild flag per step, like step! in src/simulation.jl:165-173)
 // or replayed from a captured hipGraph in chunks (fcc_run, like
 // run_steps! in ext/FourierContourCellsCUDAExt.jl:144-180).
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -39,6 +40,17 @@ static int fail(int code, const char *fmt, ...) {
     } while (0)
 
 static inline unsigned blocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+// Kernel launch; given timing events (EB at the start, EE at the end of the
+// kernel) the dispatch itself records them (hipExtLaunchKernelGGL), with no
+// marker packets between kernels, so the interval is the kernel's duration.
+#define FCC_LAUNCH(KERN, G, B, SH, S, EB, EE, ...)                                                    \
+    do {                                                                                             \
+        if ((EB) || (EE))                                                                            \
+            hipExtLaunchKernelGGL(KERN, dim3(G), dim3(B), SH, S, EB, EE, 0, __VA_ARGS__);            \
+        else                                                                                         \
+            hipLaunchKernelGGL(KERN, dim3(G), dim3(B), SH, S, __VA_ARGS__);                          \
+    } while (0)
 
 // Graph capture is serialised (ext/FourierContourCellsCUDAExt.jl:80-84).
 static std::mutex g_capture_lock;
@@ -177,6 +189,7 @@ struct fcc_sim {
     virtual int get_counters(int64_t *, double *, int64_t *, int64_t *) = 0;
     virtual int time_step_kernels(int64_t, double *, double *) = 0;
     virtual int run_timed(int64_t, double *, double *) = 0;
+    virtual int time_kernels(int64_t, double *, double *) = 0;
 };
 
 template <typename T>
@@ -205,7 +218,7 @@ struct Sim final : fcc_sim {
     hipGraphExec_t gexec = nullptr;
     int gen = 0, gexec_gen = -1;
     bool want_timed = false;
-    hipEvent_t tev[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t tev[4] = {nullptr, nullptr, nullptr, nullptr};
     double interval = 16.0;
     int64_t since_rebuild = 0, gbuilds = 0;
 
@@ -528,7 +541,7 @@ struct Sim final : fcc_sim {
         return (K == 5 || K == 7) && p.ncls <= 2 && (!has_contact || maxcount < (1 << NPLANE));
     }
 
-    int launch_forces(int use_status) {
+    int launch_forces(int use_status, hipEvent_t eb = nullptr, hipEvent_t ee = nullptr) {
         const unsigned g = blocks(N, 64);
         const int dbg = debug ? 1 : 0;
         if constexpr (sizeof(T) == 4) {
@@ -536,8 +549,8 @@ struct Sim final : fcc_sim {
                 const unsigned gm = blocks(N, 32);
                 const bool bf = cfg.precision == FCC_MIXED_BF16;
 #define FCC_MF(KV, TV, NCV)                                                                             \
-    if (bf) hipLaunchKernelGGL((k_forces_mfma<KV, TV, NCV, true>), gm, 64, 0, stream, p, d, asyn, bpull, use_status, dbg); \
-    else hipLaunchKernelGGL((k_forces_mfma<KV, TV, NCV, false>), gm, 64, 0, stream, p, d, asyn, bpull, use_status, dbg);
+    if (bf) FCC_LAUNCH((k_forces_mfma<KV, TV, NCV, true>), gm, 64, 0, stream, eb, ee, p, d, asyn, bpull, use_status, dbg); \
+    else FCC_LAUNCH((k_forces_mfma<KV, TV, NCV, false>), gm, 64, 0, stream, eb, ee, p, d, asyn, bpull, use_status, dbg);
 #define FCC_MF_T(KV, NCV) if (mfma_T == 3) { FCC_MF(KV, 3, NCV) } else { FCC_MF(KV, 4, NCV) }
                 if (K == 5) {
                     if (p.ncls <= 1) { FCC_MF_T(5, 1) } else { FCC_MF_T(5, 2) }
@@ -552,8 +565,8 @@ struct Sim final : fcc_sim {
         }
         if (fast_ok()) {
 #define FCC_FAST(KV, NC)                                                                                 \
-    if (nth == 40) hipLaunchKernelGGL((k_forces_fast<T, 40, KV, NC>), g, 64, 0, stream, p, d, use_status, dbg); \
-    else hipLaunchKernelGGL((k_forces_fast<T, 0, KV, NC>), g, 64, 0, stream, p, d, use_status, dbg);
+    if (nth == 40) FCC_LAUNCH((k_forces_fast<T, 40, KV, NC>), g, 64, 0, stream, eb, ee, p, d, use_status, dbg); \
+    else FCC_LAUNCH((k_forces_fast<T, 0, KV, NC>), g, 64, 0, stream, eb, ee, p, d, use_status, dbg);
             if (K == 5) {
                 if (p.ncls <= 1) { FCC_FAST(5, 1) } else { FCC_FAST(5, 2) }
             } else {
@@ -564,8 +577,8 @@ struct Sim final : fcc_sim {
             return FCC_OK;
         }
 #define FCC_FORCES(KMV)                                                                                  \
-    if (nth == 40) hipLaunchKernelGGL((k_forces<T, 40, KMV>), g, 64, 0, stream, p, d, use_status, dbg);   \
-    else hipLaunchKernelGGL((k_forces<T, 0, KMV>), g, 64, 0, stream, p, d, use_status, dbg);
+    if (nth == 40) FCC_LAUNCH((k_forces<T, 40, KMV>), g, 64, 0, stream, eb, ee, p, d, use_status, dbg);   \
+    else FCC_LAUNCH((k_forces<T, 0, KMV>), g, 64, 0, stream, eb, ee, p, d, use_status, dbg);
         switch (KM) {
         case 5: FCC_FORCES(5) break;
         case 7: FCC_FORCES(7) break;
@@ -576,13 +589,14 @@ struct Sim final : fcc_sim {
         return FCC_OK;
     }
 
-    int launch_update() {
+    int launch_update(hipEvent_t eb = nullptr, hipEvent_t ee = nullptr) {
         const unsigned g = blocks(N, 256);
         const int ud = has_pdrag ? 1 : 0, dbg = debug ? 1 : 0;
-        if (has_pdrag) hipLaunchKernelGGL((k_displacement<T>), g, 256, 0, stream, p, d);
+        if (has_pdrag) FCC_LAUNCH((k_displacement<T>), g, 256, 0, stream, eb, nullptr, p, d);
+        hipEvent_t eu = has_pdrag ? nullptr : eb;
 #define FCC_UPD(KMV, EX)                                                                                   \
-    if (mixed) hipLaunchKernelGGL((k_update<T, KMV, EX, true>), g, 256, 0, stream, p, d, ud, dbg);          \
-    else hipLaunchKernelGGL((k_update<T, KMV, EX, false>), g, 256, 0, stream, p, d, ud, dbg);
+    if (mixed) FCC_LAUNCH((k_update<T, KMV, EX, true>), g, 256, 0, stream, eu, ee, p, d, ud, dbg);          \
+    else FCC_LAUNCH((k_update<T, KMV, EX, false>), g, 256, 0, stream, eu, ee, p, d, ud, dbg);
         if (K == 5) {
             FCC_UPD(5, true)
         } else if (K == 7) {
@@ -758,11 +772,8 @@ struct Sim final : fcc_sim {
                     for (auto &ev : tev) HIPCHK(hipEventCreate(&ev));
                 for (int64_t q = 0; q + 1 < k; q++) HIPCHK(hipGraphLaunch(gexec, stream));
                 p.skin_check = (has_contact && skin > 0.0) ? 1 : 0;
-                HIPCHK(hipEventRecord(tev[0], stream));
-                if ((rc = launch_forces(1))) return rc;
-                HIPCHK(hipEventRecord(tev[1], stream));
-                if ((rc = launch_update())) return rc;
-                HIPCHK(hipEventRecord(tev[2], stream));
+                if ((rc = launch_forces(1, tev[0], tev[1]))) return rc;
+                if ((rc = launch_update(tev[2], tev[3]))) return rc;
             }
             HIPCHK(hipMemcpyAsync(&st, d.st, sizeof st, hipMemcpyDeviceToHost, stream));
             HIPCHK(hipStreamSynchronize(stream));
@@ -771,7 +782,7 @@ struct Sim final : fcc_sim {
             if (want_timed && executed == k) {  // the chunk's last step ran (not frozen)
                 float a = 0.f, b = 0.f;
                 HIPCHK(hipEventElapsedTime(&a, tev[0], tev[1]));
-                HIPCHK(hipEventElapsedTime(&b, tev[1], tev[2]));
+                HIPCHK(hipEventElapsedTime(&b, tev[2], tev[3]));
                 acc_f += a;
                 acc_u += b;
                 acc_n++;
@@ -1012,25 +1023,86 @@ struct Sim final : fcc_sim {
         return FCC_OK;
     }
 
+    // Device regions a step writes (state, geometry, gradients, status).
+    std::vector<std::pair<void *, size_t>> step_regions() {
+        const size_t n = (size_t)N, rows = n * (size_t)ldc;
+        return {{d.x, 8 * n},       {d.y, 8 * n},           {d.imgx, 4 * n},
+                {d.imgy, 4 * n},    {d.th, sizeof(T) * n},  {d.coef, sizeof(T) * rows},
+                {d.lab, sizeof(T) * rows}, {d.rmax, sizeof(T) * n}, {d.rmin, sizeof(T) * n},
+                {d.glab, sizeof(T) * rows}, {d.gx, sizeof(T) * n}, {d.gy, sizeof(T) * n},
+                {d.gth, sizeof(T) * n},     {d.Ebody, sizeof(T) * n}, {d.st, sizeof(DevStatus)}};
+    }
+
+    // Back-to-back launches of the forces kernel, then of the update
+    // kernel(s), each series between two HIP events; the state is saved
+    // before and restored after, so the simulation is unchanged.
+    int time_kernels(int64_t reps, double *fms, double *ums) override {
+        if (has_contact && !valid) return fail(FCC_ERR_STATE, "neighbour list not built (call fcc_update_neighbors)");
+        const auto regions = step_regions();
+        size_t bytes = 0;
+        for (const auto &q : regions) bytes += (q.second + 255) / 256 * 256;
+        char *snap = nullptr;
+        HIPCHK(hipMalloc(&snap, bytes));
+        hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+        int rc = FCC_OK;
+        auto copy = [&](bool save) -> int {
+            size_t off = 0;
+            for (const auto &q : regions) {
+                if (save)
+                    HIPCHK(hipMemcpyAsync(snap + off, q.first, q.second, hipMemcpyDeviceToDevice, stream));
+                else
+                    HIPCHK(hipMemcpyAsync(q.first, snap + off, q.second, hipMemcpyDeviceToDevice, stream));
+                off += (q.second + 255) / 256 * 256;
+            }
+            return FCC_OK;
+        };
+        for (auto &e : ev)
+            if (!rc && hipEventCreate(&e) != hipSuccess) rc = fail(FCC_ERR_DEVICE, "hipEventCreate failed");
+        if (!rc) rc = copy(true);
+        if (!rc) {
+            hipLaunchKernelGGL(k_clear_status, 1, 1, 0, stream, d.st);  // update = 0: the update kernel runs
+            p.skin_check = (has_contact && skin > 0.0) ? 1 : 0;
+            hipError_t e = hipEventRecord(ev[0], stream);
+            for (int64_t q = 0; q < reps && !rc && e == hipSuccess; q++) rc = launch_forces(0);
+            if (!rc && e == hipSuccess) e = hipEventRecord(ev[1], stream);
+            for (int64_t q = 0; q < reps && !rc && e == hipSuccess; q++) rc = launch_update();
+            if (!rc && e == hipSuccess) e = hipEventRecord(ev[2], stream);
+            if (!rc && e == hipSuccess) e = hipEventSynchronize(ev[2]);
+            float a = 0.f, b = 0.f;
+            if (!rc && e == hipSuccess) e = hipEventElapsedTime(&a, ev[0], ev[1]);
+            if (!rc && e == hipSuccess) e = hipEventElapsedTime(&b, ev[1], ev[2]);
+            if (!rc && e != hipSuccess) rc = fail(FCC_ERR_DEVICE, "kernel timing: %s", hipGetErrorString(e));
+            if (!rc) {
+                if (fms) *fms = reps > 0 ? a / (double)reps : 0.0;
+                if (ums) *ums = reps > 0 ? b / (double)reps : 0.0;
+            }
+            const int rc2 = copy(false);
+            if (!rc) rc = rc2;
+        }
+        (void)hipStreamSynchronize(stream);
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+        (void)hipFree(snap);
+        return rc;
+    }
+
     int time_step_kernels(int64_t steps, double *fms, double *ums) override {
-        hipEvent_t e0, e1, e2;
+        hipEvent_t e0, e1, e2, e3;
         HIPCHK(hipEventCreate(&e0));
         HIPCHK(hipEventCreate(&e1));
         HIPCHK(hipEventCreate(&e2));
+        HIPCHK(hipEventCreate(&e3));
         double sf = 0.0, su = 0.0;
         int rc = FCC_OK;
         for (int64_t q = 0; q < steps && !rc; q++) {
             if (has_contact && (rc = update_neighbors(0))) break;
             p.skin_check = (has_contact && valid && skin > 0.0) ? 1 : 0;
-            hipError_t e = hipEventRecord(e0, stream);
-            if (e == hipSuccess && !(rc = launch_forces(1))) {
-                e = hipEventRecord(e1, stream);
-                if (e == hipSuccess && !(rc = launch_update())) e = hipEventRecord(e2, stream);
-            }
-            if (!rc && e == hipSuccess) e = hipEventSynchronize(e2);
+            hipError_t e = hipSuccess;
+            if (!(rc = launch_forces(1, e0, e1))) rc = launch_update(e2, e3);
+            if (!rc) e = hipEventSynchronize(e3);
             float a = 0.f, b = 0.f;
             if (!rc && e == hipSuccess) e = hipEventElapsedTime(&a, e0, e1);
-            if (!rc && e == hipSuccess) e = hipEventElapsedTime(&b, e1, e2);
+            if (!rc && e == hipSuccess) e = hipEventElapsedTime(&b, e2, e3);
             if (!rc && e != hipSuccess) rc = fail(FCC_ERR_DEVICE, "event timing: %s", hipGetErrorString(e));
             sf += a;
             su += b;
@@ -1040,6 +1112,7 @@ struct Sim final : fcc_sim {
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         (void)hipEventDestroy(e2);
+        (void)hipEventDestroy(e3);
         if (rc) return rc;
         if (fms) *fms = steps > 0 ? sf / (double)steps : 0.0;
         if (ums) *ums = steps > 0 ? su / (double)steps : 0.0;
@@ -1138,6 +1211,12 @@ int fcc_time_step_kernels(fcc_sim *s, int64_t steps, double *fms, double *ums) {
     FCC_HANDLE(s);
     if (steps < 0) return fail(FCC_ERR_ARGUMENT, "steps must be non-negative");
     return s->time_step_kernels(steps, fms, ums);
+}
+
+int fcc_time_kernels(fcc_sim *s, int64_t reps, double *fms, double *ums) {
+    FCC_HANDLE(s);
+    if (reps < 1) return fail(FCC_ERR_ARGUMENT, "reps must be positive");
+    return s->time_kernels(reps, fms, ums);
 }
 
 int fcc_run_timed(fcc_sim *s, int64_t n, double *fms, double *ums) {

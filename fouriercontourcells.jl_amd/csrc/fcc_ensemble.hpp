@@ -86,7 +86,7 @@ struct Ensemble final : fcc_ensemble {
     hipStream_t side = nullptr;
     DevStatus *snap = nullptr;
     hipEvent_t sev[NB] = {};
-    hipEvent_t tev[NB][3] = {};
+    hipEvent_t tev[NB][4] = {};
 
     ~Ensemble() override {
         if (stream) (void)hipStreamSynchronize(stream);
@@ -147,17 +147,17 @@ struct Ensemble final : fcc_ensemble {
         return FCC_OK;
     }
 
-    int launch_forces() {
+    int launch_forces(hipEvent_t eb = nullptr, hipEvent_t ee = nullptr) {
         const Sim<T> &a = *m[0];
         const int dbg = a.debug ? 1 : 0;
         if constexpr (sizeof(T) == 4) {
             if (var == 2) {
                 const bool bf = a.cfg.precision == FCC_MIXED_BF16;
 #define FCC_MFE(KV, TV, NCV)                                                                                  \
-    if (bf) hipLaunchKernelGGL((k_forces_mfma_ens<KV, TV, NCV, true>), gf, 64, 0, stream, dP, dD, tf, a.asyn,  \
-                               a.bpull, 2, dbg);                                                               \
-    else hipLaunchKernelGGL((k_forces_mfma_ens<KV, TV, NCV, false>), gf, 64, 0, stream, dP, dD, tf, a.asyn,   \
-                            a.bpull, 2, dbg);
+    if (bf) FCC_LAUNCH((k_forces_mfma_ens<KV, TV, NCV, true>), gf, 64, 0, stream, eb, ee, dP, dD, tf, a.asyn,  \
+                       a.bpull, 2, dbg);                                                               \
+    else FCC_LAUNCH((k_forces_mfma_ens<KV, TV, NCV, false>), gf, 64, 0, stream, eb, ee, dP, dD, tf, a.asyn,   \
+                    a.bpull, 2, dbg);
 #define FCC_MFE_T(KV, NCV) if (a.mfma_T == 3) { FCC_MFE(KV, 3, NCV) } else { FCC_MFE(KV, 4, NCV) }
                 if (a.K == 5) {
                     if (ncls <= 1) { FCC_MFE_T(5, 1) } else { FCC_MFE_T(5, 2) }
@@ -172,8 +172,8 @@ struct Ensemble final : fcc_ensemble {
         }
         if (var == 1) {
 #define FCC_FE(KV, NC)                                                                                        \
-    if (a.nth == 40) hipLaunchKernelGGL((k_forces_fast_ens<T, 40, KV, NC>), gf, 64, 0, stream, dP, dD, tf, 2, dbg); \
-    else hipLaunchKernelGGL((k_forces_fast_ens<T, 0, KV, NC>), gf, 64, 0, stream, dP, dD, tf, 2, dbg);
+    if (a.nth == 40) FCC_LAUNCH((k_forces_fast_ens<T, 40, KV, NC>), gf, 64, 0, stream, eb, ee, dP, dD, tf, 2, dbg); \
+    else FCC_LAUNCH((k_forces_fast_ens<T, 0, KV, NC>), gf, 64, 0, stream, eb, ee, dP, dD, tf, 2, dbg);
             if (a.K == 5) {
                 if (ncls <= 1) { FCC_FE(5, 1) } else { FCC_FE(5, 2) }
             } else {
@@ -184,8 +184,8 @@ struct Ensemble final : fcc_ensemble {
             return FCC_OK;
         }
 #define FCC_GE(KMV)                                                                                           \
-    if (a.nth == 40) hipLaunchKernelGGL((k_forces_ens<T, 40, KMV>), gf, 64, 0, stream, dP, dD, tf, 2, dbg);   \
-    else hipLaunchKernelGGL((k_forces_ens<T, 0, KMV>), gf, 64, 0, stream, dP, dD, tf, 2, dbg);
+    if (a.nth == 40) FCC_LAUNCH((k_forces_ens<T, 40, KMV>), gf, 64, 0, stream, eb, ee, dP, dD, tf, 2, dbg);   \
+    else FCC_LAUNCH((k_forces_ens<T, 0, KMV>), gf, 64, 0, stream, eb, ee, dP, dD, tf, 2, dbg);
         switch (a.KM) {
         case 5: FCC_GE(5) break;
         case 7: FCC_GE(7) break;
@@ -196,14 +196,15 @@ struct Ensemble final : fcc_ensemble {
         return FCC_OK;
     }
 
-    int launch_update() {
+    int launch_update(hipEvent_t eb = nullptr, hipEvent_t ee = nullptr) {
         const Sim<T> &a = *m[0];
         const int ud = a.has_pdrag ? 1 : 0, dbg = a.debug ? 1 : 0;
         const int ub = lc.update_block;
-        if (ud) hipLaunchKernelGGL((k_displacement_ens<T>), gu, ub, 0, stream, dP, dD, tu);
+        if (ud) FCC_LAUNCH((k_displacement_ens<T>), gu, ub, 0, stream, eb, nullptr, dP, dD, tu);
+        hipEvent_t eu = ud ? nullptr : eb;
 #define FCC_UE(KMV, EX)                                                                                       \
-    if (a.mixed) hipLaunchKernelGGL((k_update_ens<T, KMV, EX, true>), gu, ub, 0, stream, dP, dD, tu, ud, dbg); \
-    else hipLaunchKernelGGL((k_update_ens<T, KMV, EX, false>), gu, ub, 0, stream, dP, dD, tu, ud, dbg);
+    if (a.mixed) FCC_LAUNCH((k_update_ens<T, KMV, EX, true>), gu, ub, 0, stream, eu, ee, dP, dD, tu, ud, dbg); \
+    else FCC_LAUNCH((k_update_ens<T, KMV, EX, false>), gu, ub, 0, stream, eu, ee, dP, dD, tu, ud, dbg);
         if (a.K == 5) {
             FCC_UE(5, true)
         } else if (a.K == 7) {
@@ -352,11 +353,8 @@ struct Ensemble final : fcc_ensemble {
             for (int64_t q = 0; q + 1 < k; q++) HIPCHK(hipGraphLaunch(gexec, stream));
             int rc = launch_gate();
             if (rc) return rc;
-            HIPCHK(hipEventRecord(tev[slot][0], stream));
-            if ((rc = launch_forces())) return rc;
-            HIPCHK(hipEventRecord(tev[slot][1], stream));
-            if ((rc = launch_update())) return rc;
-            HIPCHK(hipEventRecord(tev[slot][2], stream));
+            if ((rc = launch_forces(tev[slot][0], tev[slot][1]))) return rc;
+            if ((rc = launch_update(tev[slot][2], tev[slot][3]))) return rc;
         }
         HIPCHK(hipMemcpyAsync(snap + (size_t)slot * R, st, sizeof(DevStatus) * R, hipMemcpyDeviceToHost, stream));
         HIPCHK(hipEventRecord(sev[slot], stream));
@@ -401,7 +399,7 @@ struct Ensemble final : fcc_ensemble {
             if (want_timed) {
                 float a = 0.f, b = 0.f;
                 HIPCHK(hipEventElapsedTime(&a, tev[slot][0], tev[slot][1]));
-                HIPCHK(hipEventElapsedTime(&b, tev[slot][1], tev[slot][2]));
+                HIPCHK(hipEventElapsedTime(&b, tev[slot][2], tev[slot][3]));
                 acc_f += a;
                 acc_u += b;
                 acc_n++;

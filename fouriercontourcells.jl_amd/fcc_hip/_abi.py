@@ -144,6 +144,7 @@ _DEVICE_ONLY = {
     "abi_version": [],
     "time_step_kernels": [_vp, _i64, _dp, _dp],
     "run_timed": [_vp, _i64, _dp, _dp],
+    "time_kernels": [_vp, _i64, _dp, _dp],
     "ensemble_create": [_i32, ctypes.POINTER(FccConfig), ctypes.POINTER(FccBodies), ctypes.POINTER(_vp)],
     "ensemble_destroy": [_vp],
     "ensemble_member": [_vp, _i32, ctypes.POINTER(_vp)],

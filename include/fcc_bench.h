@@ -27,6 +27,13 @@ int fcc_time_step_kernels(fcc_sim *sim, int64_t steps, double *forces_ms,
  * durations over those steps, in milliseconds. Results equal fcc_run's. */
 int fcc_run_timed(fcc_sim *sim, int64_t n, double *forces_ms, double *update_ms);
 
+/* Average duration of the forces kernel and of the update kernel(s) of a
+ * step, in milliseconds: `reps` back-to-back launches of each between two
+ * HIP events on the simulation's stream (the forces kernel evaluates the
+ * current state; the update launches start from it). The state is saved
+ * before and restored afterwards: the simulation is left unchanged. */
+int fcc_time_kernels(fcc_sim *sim, int64_t reps, double *forces_ms, double *update_ms);
+
 /* fcc_ensemble_run with the same eager last step per chunk between HIP
  * events; average forces and update kernel durations of an ensemble step
  * (all replicas), in milliseconds. Results equal fcc_ensemble_run's. */
