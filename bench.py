@@ -65,9 +65,9 @@ def timed_run(sim, steps):
 
 
 def kernel_durations(sim, reps=50):
-    """Average forces / update kernel durations (ms) from `reps` back-to-back
-    launches of each between HIP events on the simulation's stream
-    (fcc_time_kernels; the state is restored afterwards)."""
+    """Average forces / update kernel durations (ms) from HIP events on the
+    simulation's stream around `reps` back-to-back forces launches and `reps`
+    back-to-back steps (fcc_time_kernels; the state is restored afterwards)."""
     f, u = ctypes.c_double(), ctypes.c_double()
     rc = sim.engine.time_kernels(sim._h, int(reps), ctypes.byref(f), ctypes.byref(u))
     if rc != 0:

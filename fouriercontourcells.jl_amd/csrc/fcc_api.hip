@@ -1033,9 +1033,12 @@ struct Sim final : fcc_sim {
                 {d.gth, sizeof(T) * n},     {d.Ebody, sizeof(T) * n}, {d.st, sizeof(DevStatus)}};
     }
 
-    // Back-to-back launches of the forces kernel, then of the update
-    // kernel(s), each series between two HIP events; the state is saved
-    // before and restored after, so the simulation is unchanged.
+    // Two series between HIP events: `reps` back-to-back forces launches,
+    // then `reps` steps (forces + update kernels, which advance the state:
+    // an update launched on its own again and again would integrate stale
+    // forces and time a state the model never reaches). Forces = series 1 /
+    // reps, update = (series 2 - series 1) / reps. The state is saved before
+    // and restored after, so the simulation is unchanged.
     int time_kernels(int64_t reps, double *fms, double *ums) override {
         if (has_contact && !valid) return fail(FCC_ERR_STATE, "neighbour list not built (call fcc_update_neighbors)");
         const auto regions = step_regions();
@@ -1065,7 +1068,8 @@ struct Sim final : fcc_sim {
             hipError_t e = hipEventRecord(ev[0], stream);
             for (int64_t q = 0; q < reps && !rc && e == hipSuccess; q++) rc = launch_forces(0);
             if (!rc && e == hipSuccess) e = hipEventRecord(ev[1], stream);
-            for (int64_t q = 0; q < reps && !rc && e == hipSuccess; q++) rc = launch_update();
+            for (int64_t q = 0; q < reps && !rc && e == hipSuccess; q++)
+                if (!(rc = launch_forces(0))) rc = launch_update();
             if (!rc && e == hipSuccess) e = hipEventRecord(ev[2], stream);
             if (!rc && e == hipSuccess) e = hipEventSynchronize(ev[2]);
             float a = 0.f, b = 0.f;
@@ -1073,8 +1077,8 @@ struct Sim final : fcc_sim {
             if (!rc && e == hipSuccess) e = hipEventElapsedTime(&b, ev[1], ev[2]);
             if (!rc && e != hipSuccess) rc = fail(FCC_ERR_DEVICE, "kernel timing: %s", hipGetErrorString(e));
             if (!rc) {
-                if (fms) *fms = reps > 0 ? a / (double)reps : 0.0;
-                if (ums) *ums = reps > 0 ? b / (double)reps : 0.0;
+                if (fms) *fms = a / (double)reps;
+                if (ums) *ums = (b - a) / (double)reps;
             }
             const int rc2 = copy(false);
             if (!rc) rc = rc2;

@@ -523,7 +523,9 @@ __device__ __forceinline__ void k_update_impl(const PP &p, const DP &d, int use_
         const double dy = min_image(yw - d.y0[i], p.Ly);
         const double dr = (double)rm - (double)d.rmax0[i];
         const double delta = sqrt(dx * dx + dy * dy) + (dr > 0.0 ? dr : 0.0);
-        if (2.0 * delta > (double)p.skin)
+        // one store per wave whose bodies tripped (not one per body)
+        const uint64_t trips = __ballot(2.0 * delta > (double)p.skin);
+        if (trips != 0 && (int)(threadIdx.x & 63) == __ffsll((long long)trips) - 1)
             __hip_atomic_store(&d.st->rebuild, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }

@@ -28,10 +28,11 @@ int fcc_time_step_kernels(fcc_sim *sim, int64_t steps, double *forces_ms,
 int fcc_run_timed(fcc_sim *sim, int64_t n, double *forces_ms, double *update_ms);
 
 /* Average duration of the forces kernel and of the update kernel(s) of a
- * step, in milliseconds: `reps` back-to-back launches of each between two
- * HIP events on the simulation's stream (the forces kernel evaluates the
- * current state; the update launches start from it). The state is saved
- * before and restored afterwards: the simulation is left unchanged. */
+ * step, in milliseconds, from two series between HIP events on the
+ * simulation's stream: `reps` back-to-back forces launches (evaluating the
+ * current state), then `reps` steps (forces + update); update = (steps -
+ * forces) / reps. The state is saved before and restored afterwards: the
+ * simulation is left unchanged. */
 int fcc_time_kernels(fcc_sim *sim, int64_t reps, double *forces_ms, double *update_ms);
 
 /* fcc_ensemble_run with the same eager last step per chunk between HIP

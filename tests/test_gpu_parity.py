@@ -225,6 +225,22 @@ def test_timed_run_equals_run(hip):
     assert same(a, b)
 
 
+@pytest.mark.parametrize("precision", ["fp32", "mixed_fp16"])
+def test_kernel_timing_leaves_state_unchanged(hip, precision):
+    # fcc_time_kernels saves and restores the state around its launches
+    import ctypes
+    say("kernel timing", precision)
+    a, b = soft_fluid(precision), soft_fluid(precision)
+    F.run(a, 60)
+    F.run(b, 60)
+    f, u = ctypes.c_double(), ctypes.c_double()
+    assert hip.time_kernels(b._h, 7, ctypes.byref(f), ctypes.byref(u)) == 0
+    assert f.value > 0 and u.value > 0
+    F.run(a, 60)
+    F.run(b, 60)
+    assert same(a, b)
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_generic_kernel_matches_oracle(hip, oracle_engine, precision):
     # K = 4 (maxmode 5) and three Overlap coupling values take the generic
