@@ -43,6 +43,95 @@ def make_sim(n, precision, seed=1234):
                         precision=precision, seed=seed, skin=0.3)
 
 
+def polydisperse_sim(precision, n=256, seed=7):
+    """Polydisperse monolayer: hexagonal_monolayer(256, 256; packing 0.95),
+    N = 65536, lognormal R0 (σ_log 0.1, mean 1), modes 2..6 with aₙ, bₙ
+    uniform in [-1, 1] rescaled to Σ√(aₙ² + bₙ²) = 0.3 (a₀ from the area
+    constraint at construction), v0 = 0.06, Dr = 0.01, μab = 0.1,
+    Tension(0.01) + Overlap(1.0), dt = 0.005, seed 7 (numpy's generator with
+    the same seed for R0 and the shapes)."""
+    b, box = F.hexagonal_monolayer(n, n, packing=0.95, maxmode=6, seed=seed)
+    rng = np.random.default_rng(seed)
+    N = len(b)
+    sig = 0.1
+    b.R0 = np.exp(sig * rng.standard_normal(N) - sig * sig / 2)
+    K = (b.coef.shape[0] - 1) // 2
+    ab = rng.uniform(-1.0, 1.0, size=(2 * K, N))
+    amp = np.sqrt(ab[:K] ** 2 + ab[K:] ** 2).sum(axis=0)
+    b.coef[1:, :] = ab * (0.3 / amp)
+    return F.Simulation(b, box, types=(F.BodyType(v0=0.06, Dr=0.01, muab=0.1),),
+                        energies=(F.Tension(0.01), F.Overlap(1.0)), dt=0.005, ntheta=40,
+                        precision=precision, seed=seed, skin=0.3)
+
+
+def nuclei_sim(precision, nx=256, ny=128, seed=1):
+    """Typed bodies with nuclei: hexagonal_monolayer(256, 128; packing 0.95)
+    cells + one nucleus each (with_nuclei, ratio 0.4), 65536 bodies, modes
+    2..8; cells: Tension(0.01), PerimeterElasticity(0.1, 3.7), v0 = 0.06;
+    nuclei: Bending(0.01), drag = parent; Overlap(J = 1, related = [0 -2; -2 0])
+    on all; Dr = 0.01, dt = 0.002."""
+    cells, box = F.hexagonal_monolayer(nx, ny, packing=0.95, maxmode=8, seed=seed)
+    b = F.with_nuclei(cells, ratio=0.4)
+    types = (F.BodyType(v0=0.06, Dr=0.01, muab=0.1), F.BodyType(Dr=0.01, muab=0.1, drag="parent"))
+    energies = (F.OnTypes(F.Tension(0.01), 1), F.OnTypes(F.PerimeterElasticity(0.1, 3.7), 1),
+                F.OnTypes(F.Bending(0.01), 2),
+                F.Overlap([[1.0, 1.0], [1.0, 1.0]], related=[[0.0, -2.0], [-2.0, 0.0]]))
+    return F.Simulation(b, box, types=types, energies=energies, dt=0.002, ntheta=40,
+                        precision=precision, seed=seed, skin=0.3)
+
+
+WORKLOADS = {
+    "polydisperse65536": (polydisperse_sim, "fp32"),
+    "nuclei65536": (nuclei_sim, "mixed"),
+}
+
+
+def bench_workload(name, precision, steps, warmup):
+    """Throughput of one of the other workloads (its own JSON line), with the
+    plain FP32 path beside a mixed one."""
+    maker, default = WORKLOADS[name]
+    precision = precision or default
+    out = {"workload": name, "doc": " ".join(maker.__doc__.split()), "precision": precision}
+    for prec in [precision] + (["fp32"] if precision != "fp32" else []):
+        sim = maker(prec)
+        F.run(sim, warmup)
+        sec, _, _ = timed_run(sim, steps)
+        fms, ums = kernel_durations(sim)
+        v = sim.N * steps / sec
+        print(f"[bench] {name} {prec}: {v / 1e6:.1f} M body-steps/s ({sec / steps * 1e3:.4f} ms/step; "
+              f"forces {fms * 1e3:.1f} us, update {ums * 1e3:.1f} us), N={sim.N}, "
+              f"builds {sim.counters().builds}", file=sys.stderr, flush=True)
+        out[prec] = {"value": v, "unit": "body-steps/s", "N": sim.N, "steps": steps,
+                     "ms_per_step": sec / steps * 1e3, "forces_ms": fms, "update_ms": ums}
+        sim.close()
+    return out
+
+
+def cpu_small(steps=1000):
+    """FP64 oracle on the small CPU workload: hexagonal_monolayer(32, 32;
+    packing 0.95), N = 1024, modes 2..6, BodyType(v0 = 0.06, Dr = 0.01,
+    μab = 0.1), Tension(0.01) + Overlap(1.0), dt = 0.005, θ from Philox seed
+    1234, noise seed 1, `steps` steps on all host cores."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    cores = max(1, min(cores, len(os.sched_getaffinity(0))))
+    out = {}
+    for c in sorted({1, cores}):
+        oracle.set_threads(c)
+        b, box = F.hexagonal_monolayer(32, 32, packing=0.95, maxmode=6, seed=1234)
+        sim = F.Simulation(b, box, types=(F.BodyType(v0=0.06, Dr=0.01, muab=0.1),),
+                           energies=(F.Tension(0.01), F.Overlap(1.0)), dt=0.005, ntheta=40, seed=1,
+                           skin=0.3, engine=oracle.engine())
+        t0 = time.perf_counter()
+        F.step(sim, steps)
+        dt = time.perf_counter() - t0
+        out[f"threads_{c}"] = {"value": sim.N * steps / dt, "ms_per_step": dt / steps * 1e3}
+        print(f"[bench] CPU oracle N=1024, {c} threads: {sim.N * steps / dt / 1e6:.3f} M body-steps/s "
+              f"({dt / steps * 1e3:.3f} ms/step)", file=sys.stderr, flush=True)
+    return out
+
+
 def sync(sim):
     import torch
     torch.cuda.synchronize()
@@ -248,6 +337,10 @@ def main():
     ap.add_argument("--no-independent", action="store_true",
                     help="ensemble leg without the independent-simulations comparison")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS),
+                    help="measure another workload instead (its own JSON line)")
+    ap.add_argument("--cpu-small", action="store_true",
+                    help="only the FP64 oracle on the N=1024 CPU workload (its own JSON line)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -257,6 +350,13 @@ def main():
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.cpu_small:
+        print(json.dumps({"cpu_small": cpu_small()}), flush=True)
+        return
+    if args.workload:
+        prec = None if args.precision == "mixed" else args.precision
+        print(json.dumps(bench_workload(args.workload, prec, args.steps, args.warmup)), flush=True)
         return
 
     if args.ensemble_only:
