@@ -14,6 +14,7 @@
 # terms, nθ > 64, more than 4 body types) make `fcc_create` return
 # FCC_ERR_UNSUPPORTED; the extension then builds the reference Simulation on
 # the CPU backend instead (a user-defined term runs on the reference path).
+# `HIPEnsemble` runs independent replicas by one launch per kernel and step.
 module FourierContourCellsHIPExt
 
 using FourierContourCells
@@ -23,6 +24,7 @@ using FourierContourCells: Bodies, PeriodicBox, BodyType, EnergyTerm, OnTypes, T
 import FourierContourCells: step!, run!, compute_gradients!, update_neighbors!, shape_index,
     perimeters, areas, positions, unwrapped_positions, body_energy
 using FCCHIP: HIPBackend, libfcc_hip, Bending
+import FCCHIP: HIPEnsemble, autotune!
 
 const FCC_OK = Cint(0)
 const FCC_ERR_UNSUPPORTED = Cint(-3)
@@ -213,5 +215,74 @@ function unwrapped_positions(sim::HIPSimulation)
     return permutedims(hcat(b.x .+ b.imgx .* sim.box.Lx, b.y .+ b.imgy .* sim.box.Ly))
 end
 body_energy(sim::HIPSimulation) = sum(compute_gradients!(sim).Ebody)
+
+# ---- ensembles: independent simulations advanced by one launch per kernel
+# and step (README.md:249-261 runs them as concurrent tasks instead)
+
+struct FccLaunch
+    update_block::Int32; xcd_order::Int32; chunk::Int32; reserved::Int32
+end
+
+mutable struct HIPEnsembleHandle
+    handle::Ptr{Cvoid}
+    members::Vector{HIPSimulation}
+end
+
+"""
+    HIPEnsemble(specs; T = Float32, nθ = 40)
+
+`specs` is a vector of named tuples `(bodies, box, types, energies, dt = 0.005,
+seed = 1, skin = 0.3)`, one per replica. Replicas share `T`, `nθ` and the mode
+count. `members` are `HIPSimulation`s (borrowed handles).
+"""
+function HIPEnsemble(specs::AbstractVector; T = Float32, nθ::Integer = 40)
+    hosts = Bodies[]
+    cfgs = FccConfig[]
+    for sp in specs
+        types = map(BodyType{Float64}, Tuple(sp.types))
+        terms = map(fcc_term, Tuple(sp.energies))
+        all(!isnothing, terms) || throw(ArgumentError("ensembles need native energy terms"))
+        host = to_host(sp.bodies)
+        host = Bodies(map(f -> f isa AbstractArray{<:AbstractFloat} ? Float64.(f) : f,
+            (getfield(host, f) for f in fieldnames(Bodies)))...)
+        check_parents(host, types)
+        K = size(host.coef, 1) ÷ 2
+        tt = ntuple(q -> q <= length(types) ? fcc_type(types[q]) : fcc_type(BodyType()), 4)
+        te = ntuple(q -> q <= length(terms) ? terms[q] : EMPTY_TERM, 8)
+        push!(cfgs, FccConfig(nbodies(host), K, nθ, length(types), length(terms), tt, te,
+            get(sp, :dt, 0.005), sp.box.Lx, sp.box.Ly, get(sp, :skin, 0.3), UInt64(get(sp, :seed, 1)),
+            PRECISION[T], 0))
+        push!(hosts, host)
+    end
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    GC.@preserve hosts begin
+        bs = [bodies_struct(b) for b in hosts]
+        check(ccall((:fcc_ensemble_create, libfcc_hip), Cint,
+            (Int32, Ptr{FccConfig}, Ptr{FccBodies}, Ref{Ptr{Cvoid}}), length(cfgs), cfgs, bs, h))
+    end
+    members = HIPSimulation[]
+    for (r, (sp, host)) in enumerate(zip(specs, hosts))
+        m = Ref{Ptr{Cvoid}}(C_NULL)
+        check(ccall((:fcc_ensemble_member, libfcc_hip), Cint, (Ptr{Cvoid}, Int32, Ref{Ptr{Cvoid}}),
+            h[], r - 1, m))
+        push!(members, HIPSimulation{T}(m[], host, PeriodicBox{Float64}(sp.box.Lx, sp.box.Ly), nθ,
+            get(sp, :dt, 0.005)))
+    end
+    ens = HIPEnsembleHandle(h[], members)
+    finalizer(e -> ccall((:fcc_ensemble_destroy, libfcc_hip), Cint, (Ptr{Cvoid},), e.handle), ens)
+    return ens
+end
+
+# run!(sim, n) of every replica
+run!(e::HIPEnsembleHandle, nsteps::Integer) =
+    (check(ccall((:fcc_ensemble_run, libfcc_hip), Cint, (Ptr{Cvoid}, Int64), e.handle, nsteps)); e)
+
+function autotune!(e::HIPEnsembleHandle; steps::Integer = 20)
+    best = Ref(FccLaunch(0, 0, 0, 0))
+    ms = Ref{Cdouble}(0.0)
+    check(ccall((:fcc_ensemble_autotune, libfcc_hip), Cint, (Ptr{Cvoid}, Int64, Ref{FccLaunch}, Ref{Cdouble}),
+        e.handle, steps, best, ms))
+    return best[], ms[]
+end
 
 end
