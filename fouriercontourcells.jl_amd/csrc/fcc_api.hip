@@ -52,6 +52,10 @@ static inline unsigned blocks(int64_t n, int b) { return (unsigned)((n + b - 1) 
             hipLaunchKernelGGL(KERN, dim3(G), dim3(B), SH, S, __VA_ARGS__);                          \
     } while (0)
 
+// Steps per long graph (fcc_run and ensemble runs replay it for all but the
+// last few steps of a chunk).
+constexpr int GRAPH_STEPS = 8;
+
 // Graph capture is serialised (ext/FourierContourCellsCUDAExt.jl:80-84).
 static std::mutex g_capture_lock;
 
@@ -215,7 +219,7 @@ struct Sim final : fcc_sim {
     double t = 0.0;
     std::vector<void *> allocs;
     // graph mode (ext/FourierContourCellsCUDAExt.jl:52-71)
-    hipGraphExec_t gexec = nullptr;
+    hipGraphExec_t gexec = nullptr, gexec_n = nullptr;  // one step; GRAPH_STEPS steps
     int gen = 0, gexec_gen = -1;
     bool want_timed = false;
     hipEvent_t tev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -225,6 +229,7 @@ struct Sim final : fcc_sim {
     ~Sim() override {
         if (stream) (void)hipStreamSynchronize(stream);
         if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (gexec_n) (void)hipGraphExecDestroy(gexec_n);
         for (hipEvent_t e : tev)
             if (e) (void)hipEventDestroy(e);
         for (void *q : allocs) (void)hipFree(q);
@@ -709,23 +714,42 @@ struct Sim final : fcc_sim {
     }
 
     // run_steps! in graph mode (ext/FourierContourCellsCUDAExt.jl:65-71,133-180).
-    int capture() {
-        std::lock_guard<std::mutex> lk(g_capture_lock);
-        if (gexec) {
-            HIPCHK(hipGraphExecDestroy(gexec));
-            gexec = nullptr;
-        }
-        p.skin_check = (has_contact && skin > 0.0) ? 1 : 0;
+    // Two graphs: one step, and GRAPH_STEPS steps in a row (a graph launch
+    // costs several µs between replays; inside a graph the kernels follow
+    // each other without a gap).
+    int capture_steps(int steps, hipGraphExec_t *out) {
         hipGraph_t graph;
         HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        int rc = launch_forces(1);
-        if (!rc) rc = launch_update();
+        int rc = FCC_OK;
+        for (int q = 0; q < steps && !rc; q++)
+            if (!(rc = launch_forces(1))) rc = launch_update();
         hipError_t e = hipStreamEndCapture(stream, &graph);
         if (rc) return rc;
         HIPCHK(e);
-        HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+        HIPCHK(hipGraphInstantiate(out, graph, nullptr, nullptr, 0));
         HIPCHK(hipGraphDestroy(graph));
+        return FCC_OK;
+    }
+
+    int capture() {
+        std::lock_guard<std::mutex> lk(g_capture_lock);
+        for (hipGraphExec_t *g : {&gexec, &gexec_n})
+            if (*g) {
+                HIPCHK(hipGraphExecDestroy(*g));
+                *g = nullptr;
+            }
+        p.skin_check = (has_contact && skin > 0.0) ? 1 : 0;
+        int rc = capture_steps(1, &gexec);
+        if (!rc) rc = capture_steps(GRAPH_STEPS, &gexec_n);
+        if (rc) return rc;
         gexec_gen = gen;
+        return FCC_OK;
+    }
+
+    // k graph steps: k / GRAPH_STEPS launches of the long graph, the rest one by one.
+    int replay(int64_t k) {
+        for (int64_t q = 0; q < k / GRAPH_STEPS; q++) HIPCHK(hipGraphLaunch(gexec_n, stream));
+        for (int64_t q = 0; q < k % GRAPH_STEPS; q++) HIPCHK(hipGraphLaunch(gexec, stream));
         return FCC_OK;
     }
 
@@ -764,13 +788,13 @@ struct Sim final : fcc_sim {
             const int64_t chunk = has_contact ? std::min<int64_t>(256, std::max<int64_t>(4, (int64_t)std::llround(interval / 4))) : 256;
             const int64_t k = std::min(chunk, n - done);
             if (!want_timed) {
-                for (int64_t q = 0; q < k; q++) HIPCHK(hipGraphLaunch(gexec, stream));
+                if ((rc = replay(k))) return rc;
             } else {
                 // the chunk's last step is launched eagerly between events:
                 // the same kernels and arguments as the graph
                 if (!tev[0])
                     for (auto &ev : tev) HIPCHK(hipEventCreate(&ev));
-                for (int64_t q = 0; q + 1 < k; q++) HIPCHK(hipGraphLaunch(gexec, stream));
+                if ((rc = replay(k - 1))) return rc;
                 p.skin_check = (has_contact && skin > 0.0) ? 1 : 0;
                 if ((rc = launch_forces(1, tev[0], tev[1]))) return rc;
                 if ((rc = launch_update(tev[2], tev[3]))) return rc;

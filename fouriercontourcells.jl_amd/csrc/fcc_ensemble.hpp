@@ -73,7 +73,7 @@ struct Ensemble final : fcc_ensemble {
     fcc_launch lc{256, 1, 0, 0};
     fcc_launch tiles_lc{0, 0, 0, 0};
     int var = -1, tiles_var = -1, ncls = 1;  // forces kernel: 2 MFMA, 1 fast, 0 generic
-    hipGraphExec_t gexec = nullptr;
+    hipGraphExec_t gexec = nullptr, gexec_n = nullptr;  // one step; GRAPH_STEPS steps
     bool graph_ok = false;
     std::vector<double> interval;  // mean steps between rebuilds, per replica
     std::vector<int64_t> since;
@@ -92,6 +92,7 @@ struct Ensemble final : fcc_ensemble {
         if (stream) (void)hipStreamSynchronize(stream);
         if (side) (void)hipStreamSynchronize(side);
         if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (gexec_n) (void)hipGraphExecDestroy(gexec_n);
         for (int q = 0; q < NB; q++) {
             if (sev[q]) (void)hipEventDestroy(sev[q]);
             for (hipEvent_t e : tev[q])
@@ -277,22 +278,40 @@ struct Ensemble final : fcc_ensemble {
         }
         if (!graph_ok) {
             std::lock_guard<std::mutex> lk(g_capture_lock);
-            if (gexec) {
-                HIPCHK(hipGraphExecDestroy(gexec));
-                gexec = nullptr;
-            }
-            hipGraph_t graph;
-            HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-            int rc = launch_gate();
-            if (!rc) rc = launch_forces();
-            if (!rc) rc = launch_update();
-            hipError_t e = hipStreamEndCapture(stream, &graph);
+            for (hipGraphExec_t *g : {&gexec, &gexec_n})
+                if (*g) {
+                    HIPCHK(hipGraphExecDestroy(*g));
+                    *g = nullptr;
+                }
+            int rc = capture_steps(1, &gexec);
+            if (!rc) rc = capture_steps(GRAPH_STEPS, &gexec_n);
             if (rc) return rc;
-            HIPCHK(e);
-            HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
-            HIPCHK(hipGraphDestroy(graph));
             graph_ok = true;
         }
+        return FCC_OK;
+    }
+
+    int capture_steps(int steps, hipGraphExec_t *out) {
+        hipGraph_t graph;
+        HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        int rc = FCC_OK;
+        for (int q = 0; q < steps && !rc; q++) {
+            rc = launch_gate();
+            if (!rc) rc = launch_forces();
+            if (!rc) rc = launch_update();
+        }
+        hipError_t e = hipStreamEndCapture(stream, &graph);
+        if (rc) return rc;
+        HIPCHK(e);
+        HIPCHK(hipGraphInstantiate(out, graph, nullptr, nullptr, 0));
+        HIPCHK(hipGraphDestroy(graph));
+        return FCC_OK;
+    }
+
+    // k graph steps: k / GRAPH_STEPS launches of the long graph, the rest one by one.
+    int replay(int64_t k) {
+        for (int64_t q = 0; q < k / GRAPH_STEPS; q++) HIPCHK(hipGraphLaunch(gexec_n, stream));
+        for (int64_t q = 0; q < k % GRAPH_STEPS; q++) HIPCHK(hipGraphLaunch(gexec, stream));
         return FCC_OK;
     }
 
@@ -328,7 +347,7 @@ struct Ensemble final : fcc_ensemble {
         for (size_t r = 0; r < m.size(); r++)
             if (m[r]->has_contact) mi = std::min(mi, interval[r]);
         if (mi >= 1e300) return 64;
-        return std::min<int64_t>(16, std::max<int64_t>(2, (int64_t)std::llround(mi / 16)));
+        return std::min<int64_t>(4 * GRAPH_STEPS, std::max<int64_t>(GRAPH_STEPS, (int64_t)std::llround(mi / 8)));
     }
 
     int ensure_async() {
@@ -346,11 +365,13 @@ struct Ensemble final : fcc_ensemble {
     int issue(int64_t k, int slot) {
         const size_t R = m.size();
         if (!want_timed) {
-            for (int64_t q = 0; q < k; q++) HIPCHK(hipGraphLaunch(gexec, stream));
+            int rc = replay(k);
+            if (rc) return rc;
         } else {
             // the chunk's last step eagerly between events: the kernels and
             // arguments of the captured step
-            for (int64_t q = 0; q + 1 < k; q++) HIPCHK(hipGraphLaunch(gexec, stream));
+            int rc0 = replay(k - 1);
+            if (rc0) return rc0;
             int rc = launch_gate();
             if (rc) return rc;
             if ((rc = launch_forces(tev[slot][0], tev[slot][1]))) return rc;
@@ -559,7 +580,8 @@ struct Ensemble final : fcc_ensemble {
                 hipError_t e = hipSuccess;
                 for (int q = 0; q < 2 && e == hipSuccess; q++) e = hipGraphLaunch(gexec, stream);
                 if (e == hipSuccess) e = hipEventRecord(e0, stream);
-                for (int64_t q = 0; q < steps && e == hipSuccess; q++) e = hipGraphLaunch(gexec, stream);
+                for (int64_t q = 0; q < steps / GRAPH_STEPS && e == hipSuccess; q++) e = hipGraphLaunch(gexec_n, stream);
+                for (int64_t q = 0; q < steps % GRAPH_STEPS && e == hipSuccess; q++) e = hipGraphLaunch(gexec, stream);
                 if (e == hipSuccess) e = hipEventRecord(e1, stream);
                 if (e == hipSuccess) e = hipEventSynchronize(e1);
                 float t = 0.f;
