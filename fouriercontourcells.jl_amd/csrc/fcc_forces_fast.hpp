@@ -72,13 +72,25 @@ __device__ __forceinline__ void node_window(float dx, float dy, float lim2, floa
     if (k0 < 0) k0 += nth;
 }
 
-// Node-indexed bits (bit k) of a window starting at node k0 as window
-// positions (bit t = k - k0 mod nθ).
-__device__ __forceinline__ uint64_t sure_shift(uint64_t bits, int k0, int nth) {
+// Window-position bits (bit t) of a window starting at node k0 as
+// node-indexed bits (bit k = k0 + t mod nθ).
+__device__ __forceinline__ uint64_t window_to_nodes(uint64_t bits, int k0, int nth) {
     if (k0 == 0) return bits;
     const uint64_t full = nth == 64 ? ~0ull : ((1ull << nth) - 1);
-    return ((bits >> k0) | (bits << (nth - k0))) & full;
+    return ((bits << k0) | (bits >> (nth - k0))) & full;
 }
+
+// Bit k of a 64-bit node mask, k uniform over the wave.
+__device__ __forceinline__ uint32_t node_bit(uint64_t m, int k) {
+    return k < 32 ? __builtin_amdgcn_ubfe((uint32_t)m, (uint32_t)k, 1u)
+                  : __builtin_amdgcn_ubfe((uint32_t)(m >> 32), (uint32_t)(k - 32), 1u);
+}
+
+// Smallest squared distance for the direction of a node-to-centre offset:
+// an offset of exactly zero takes z = 0, where the contour polynomial is a0 > 0,
+// so the node counts as inside, as with the reference's ψ = 0 (R(0) > 0).
+template <typename T>
+__device__ __forceinline__ T tiny_d2() { return sizeof(T) == 4 ? T(1e-30) : T(1e-300); }
 
 // R_j(ψ)/R0 = a0 + Σ_{m=1..K} αm cos (m+1)ψ + βm sin (m+1)ψ as Re Σ cn z^n,
 // z = (c1, s1), by complex Horner (src/geometry.jl:282-291 evaluates the
@@ -231,11 +243,11 @@ __device__ __forceinline__ void k_forces_fast_impl(const PP &p, const DP &d, int
                 }
                 const T d2 = dx * dx + dy * dy;
                 if (d2 <= lim2) mask |= 1ull << t;
-                if (d2 < rin2) sure |= 1ull << k;
+                if (d2 < rin2) sure |= 1ull << t;
                 if (++k == nth) k = 0;
             }
-            uint64_t hit = sure;
-            mask &= ~(sure_shift(sure, k0, nth));
+            mask &= ~sure;
+            uint64_t hit = window_to_nodes(sure, k0, nth);
             if (!mask) goto count_hits;
             {
             T lj[NC];
@@ -258,12 +270,10 @@ __device__ __forceinline__ void k_forces_fast_impl(const PP &p, const DP &d, int
                     dx = min_image(dx, Lxf);
                     dy = min_image(dy, Lyf);
                 }
-                const T d2 = dx * dx + dy * dy;
+                const T d2 = fmax(dx * dx + dy * dy, tiny_d2<T>());
                 const T rs = f_rsqrt(d2);
-                const bool pos = d2 > T(0);
-                const T dist = pos ? d2 * rs : T(0);
-                const T cp = pos ? dx * rs : T(1);
-                const T sp = pos ? dy * rs : T(0);
+                const T dist = d2 * rs;
+                const T cp = dx * rs, sp = dy * rs;
                 if (r0j * contour_horner<T, K>(lj, cp, sp) - dist >= T(0)) hit |= 1ull << kk;
             }
             }
@@ -305,7 +315,7 @@ __device__ __forceinline__ void k_forces_fast_impl(const PP &p, const DP &d, int
             for (int c = 0; c < (NCLS > 0 ? NCLS : 1); c++) {
                 uint32_t n = 0;
 #pragma unroll
-                for (int qq = 0; qq < NPLANE; qq++) n |= (uint32_t)((pl[c][qq] >> k) & 1u) << qq;
+                for (int qq = 0; qq < NPLANE; qq++) n |= node_bit(pl[c][qq], k) << qq;
                 g += p.Jcls[c] * (T)n;
             }
             const T pp = g * p.dphi;

@@ -234,11 +234,11 @@ __device__ __forceinline__ void k_forces_mfma_impl(const PP &p, const DP &d,
                 }
                 const float d2 = dx * dx + dy * dy;
                 if (d2 <= lim2) mask |= 1ull << t;
-                if (d2 < rin2) sure |= 1ull << k;
+                if (d2 < rin2) sure |= 1ull << t;
                 if (++k == NTH_M) k = 0;
             }
-            uint64_t hit = sure;
-            mask &= ~(sure_shift(sure, k0, NTH_M));
+            mask &= ~sure;
+            uint64_t hit = window_to_nodes(sure, k0, NTH_M);
             if (mask) {
                 float lj[NC];
                 {
@@ -260,12 +260,10 @@ __device__ __forceinline__ void k_forces_mfma_impl(const PP &p, const DP &d,
                         dx = min_image(dx, Lxf);
                         dy = min_image(dy, Lyf);
                     }
-                    const float d2 = dx * dx + dy * dy;
+                    const float d2 = fmaxf(dx * dx + dy * dy, tiny_d2<float>());
                     const float rs = f_rsqrt(d2);
-                    const bool pos = d2 > 0.f;
-                    const float dist = pos ? d2 * rs : 0.f;
-                    const float cp = pos ? dx * rs : 1.f;
-                    const float sp = pos ? dy * rs : 0.f;
+                    const float dist = d2 * rs;
+                    const float cp = dx * rs, sp = dy * rs;
                     if (r0j * contour_horner<float, K>(lj, cp, sp) - dist >= 0.f) hit |= 1ull << kk;
                 }
             }
@@ -289,6 +287,13 @@ __device__ __forceinline__ void k_forces_mfma_impl(const PP &p, const DP &d,
         for (int q = 0; q < NPLANE; q++) other[q] = shfl_xor64(pl[c][q], 32);
         planes_add(pl[c], other);
     }
+
+    // the lane's 20 nodes of each counter plane at bits 0..19
+    uint32_t plh[NCLS][NPLANE];
+#pragma unroll
+    for (int c = 0; c < NCLS; c++)
+#pragma unroll
+        for (int q = 0; q < NPLANE; q++) plh[c][q] = (uint32_t)(pl[c][q] >> (h * NH));
 
     // ---- pass 2: synthesis again; the contour gradients of the lane's nodes
     // replace R, R', R'' in their register slots
@@ -314,7 +319,7 @@ __device__ __forceinline__ void k_forces_mfma_impl(const PP &p, const DP &d,
             for (int c = 0; c < NCLS; c++) {
                 uint32_t n = 0;
 #pragma unroll
-                for (int qq = 0; qq < NPLANE; qq++) n |= (uint32_t)((pl[c][qq] >> k) & 1u) << qq;
+                for (int qq = 0; qq < NPLANE; qq++) n |= __builtin_amdgcn_ubfe(plh[c][qq], (uint32_t)j, 1u) << qq;
                 g += p.Jcls[c] * (float)n;
             }
             const float pp = g * p.dphi;
