@@ -462,6 +462,7 @@ struct Sim final : fcc_sim {
             if ((rc = alloc(&asyn, A.size())) || (rc = alloc(&bpull, B.size()))) return rc;
             HIPCHK(hipMemcpy(asyn, A.data(), A.size() * 2, hipMemcpyHostToDevice));
             HIPCHK(hipMemcpy(bpull, B.data(), B.size() * 2, hipMemcpyHostToDevice));
+            if (ldc != LDC_M) return fail(FCC_ERR_STATE, "MFMA forces kernel expects coefficient rows of %d", LDC_M);
             mfma_T = TT;
         }
 

@@ -33,6 +33,7 @@ constexpr float SYN_SCALE = 1024.f;  // lab pre-scale
 constexpr float PULL_SCALE = 256.f;  // contour-gradient pre-scale
 constexpr int NTH_M = 40;            // nodes handled by the MFMA kernel
 constexpr int NH = NTH_M / 2;        // nodes per lane half
+constexpr int LDC_M = 16;            // row stride of the coefficient rows (ncoef ≤ 15)
 
 template <bool BF>
 struct MfmaType;
@@ -146,7 +147,7 @@ __device__ __forceinline__ void k_forces_mfma_impl(const PP &p, const DP &d,
     const int64_t ii = live ? i : 0;
     const float r0 = live ? d.R0[ii] : 1.f;
     const float unsc = r0 * (1.f / SYN_SCALE);
-    const float *lab_row = d.lab + ii * p.ldc;
+    const float *lab_row = d.lab + ii * LDC_M;
 
     // ---- synthesis on MFMA
     f16v_t acc[T];
@@ -243,7 +244,7 @@ __device__ __forceinline__ void k_forces_mfma_impl(const PP &p, const DP &d,
                 float lj[NC];
                 {
                     float a[W];
-                    row_load<float, W>(a, d.lab + (int64_t)j * p.ldc);
+                    row_load<float, W>(a, d.lab + (int64_t)j * LDC_M);
 #pragma unroll
                     for (int c = 0; c < NC; c++) lj[c] = a[c];
                 }
@@ -366,10 +367,17 @@ __device__ __forceinline__ void k_forces_mfma_impl(const PP &p, const DP &d,
     for (int r = 0; r < 16; r++) gacc[r] += __shfl_down(gacc[r], 16);
     const int n = lane & 31;
     if (n < NC) {
+        const int64_t body0 = (int64_t)blk * 32 + 4 * h;
+        float *base = d.glab + body0 * LDC_M + n;
+        if ((int64_t)blk * 32 + 32 <= p.N) {
 #pragma unroll
-        for (int r = 0; r < 16; r++) {
-            const int64_t body = (int64_t)blk * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
-            if (body < p.N) d.glab[body * p.ldc + n] = gacc[r] * (1.f / PULL_SCALE);
+            for (int r = 0; r < 16; r++) base[(8 * (r >> 2) + (r & 3)) * LDC_M] = gacc[r] * (1.f / PULL_SCALE);
+        } else {
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const int rr = 8 * (r >> 2) + (r & 3);
+                if (body0 + rr < p.N) base[rr * LDC_M] = gacc[r] * (1.f / PULL_SCALE);
+            }
         }
     }
     if (live && h == 0) {
