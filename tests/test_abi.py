@@ -56,6 +56,33 @@ def test_host_validation_errors():
         F.Overlap([[1.0, 2.0]]).tables()
 
 
+def test_library_validates_before_touching_the_device():
+    # argument checks of fcc_create / fcc_ensemble_create return before any
+    # HIP call (FCC_ERR_ARGUMENT / FCC_ERR_UNSUPPORTED with a message)
+    path = _abi.library_path()
+    assert os.path.exists(path), "libfcc_hip.so not built"
+    eng = _abi.Engine(ctypes.CDLL(path), "fcc_", "hip")
+    h = ctypes.c_void_p()
+    assert eng.create(None, None, ctypes.byref(h)) == _abi.FCC_ERR_ARGUMENT
+    assert eng.ensemble_create(0, None, None, ctypes.byref(h)) == _abi.FCC_ERR_ARGUMENT
+    assert "replica" in eng.last_error()
+    b, box = F.hexagonal_monolayer(4, 4)
+    bad = F.api._config(b, box, (F.BodyType(),), (F.LinearRepulsion(1.0),), 0.005, 40, "fp32", 1, 0.3, 0)
+    st = b.struct()
+    assert eng.create(ctypes.byref(bad), ctypes.byref(st), ctypes.byref(h)) == _abi.FCC_ERR_UNSUPPORTED
+    assert "LinearRepulsion" in eng.last_error()
+    cfgs = (_abi.FccConfig * 2)()
+    structs = (_abi.FccBodies * 2)()
+    b8, box8 = F.hexagonal_monolayer(4, 4, maxmode=8)
+    cfgs[0] = F.api._config(b, box, (F.BodyType(),), (F.Tension(0.01),), 0.005, 40, "fp32", 1, 0.3, 0)
+    cfgs[1] = F.api._config(b8, box8, (F.BodyType(),), (F.Tension(0.01),), 0.005, 40, "fp32", 2, 0.3, 0)
+    structs[0], structs[1] = b.struct(), b8.struct()
+    assert eng.ensemble_create(2, cfgs, structs, ctypes.byref(h)) == _abi.FCC_ERR_UNSUPPORTED
+    assert "replica 1" in eng.last_error()
+    assert eng.ensemble_destroy(None) == _abi.FCC_OK
+    assert eng.destroy(None) == _abi.FCC_OK
+
+
 def test_hexagonal_monolayer_geometry():
     # src/initial.jl:16-24: box 2r nx × √3 r ny with N π R0² = packing Lx Ly
     b, box = F.hexagonal_monolayer(32, 32, packing=0.95)

@@ -143,6 +143,25 @@ def test_parent_drag_and_generic_kernel():
         assert_same(a, b)
 
 
+def test_list_overflow_and_kernel_switch_mid_run():
+    # skin 4.1 on a hexagonal lattice: 30 neighbours at the start (below the
+    # fast kernels' limit of 32 and the default capacity of 32); compressions
+    # push lists past both while the replicas run, which exercises the
+    # synchronous regrow and the switch to the generic forces kernel
+    say("overflow / kernel switch")
+    reps = [fast_replica(seed=s, nx=12, ny=12, v0=3.0, skin=4.1) for s in (1, 2)]
+    seq = [F.run(alone(rp, "fp32"), 150) for rp in reps]
+    ens = F.Ensemble(reps, precision="fp32")
+    ens.run(150)
+    caps = [m.neighbors.capacity for m in ens.members]
+    top = [int(m.neighbors.count.max()) for m in ens.members]
+    say(f"  capacities {caps}, largest lists {top}")
+    for a, b in zip(seq, ens.members):
+        assert_same(a, b)
+        assert a.neighbors.capacity == b.neighbors.capacity
+        assert np.array_equal(a.neighbors.idx, b.neighbors.idx)
+
+
 def test_replicas_must_share_modes_and_precision():
     a = fast_replica(1, maxmode=6)
     b = fast_replica(2, maxmode=8)
