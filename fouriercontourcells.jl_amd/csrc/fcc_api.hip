@@ -101,9 +101,7 @@ static int validate(const fcc_config *cfg, const fcc_bodies *b) {
         const int k = cfg->terms[q].kind;
         if (k == FCC_TERM_TENSION || k == FCC_TERM_PERIMETER_ELASTICITY || k == FCC_TERM_BENDING)
             nbody++;
-        else if (k == FCC_TERM_LINEAR_REPULSION)
-            return fail(FCC_ERR_UNSUPPORTED, "LinearRepulsion has no native kernel");
-        else if (k != FCC_TERM_OVERLAP)
+        else if (k != FCC_TERM_OVERLAP && k != FCC_TERM_LINEAR_REPULSION)
             return fail(FCC_ERR_ARGUMENT, "energy terms must be BodyTerms, ContactTerms or BondTerms");
     }
     if (nbody > MAXB) return fail(FCC_ERR_UNSUPPORTED, "at most %d native body terms", MAXB);
@@ -205,7 +203,7 @@ struct Sim final : fcc_sim {
     bool own_stream = true;
     int64_t N = 0;
     int K = 0, nc = 0, ldc = 0, nth = 0, KM = 0;
-    bool mixed = false, debug = false, has_contact = false, has_pdrag = false;
+    bool mixed = false, debug = false, has_contact = false, has_pdrag = false, has_linrep = false;
     double skin = 0.0, range = 0.0;
     int32_t ncx = 1, ncy = 1;
     double wx = 0, wy = 0;
@@ -287,10 +285,18 @@ struct Sim final : fcc_sim {
         int nb = 0;
         int overlaps[FCC_MAX_TERMS];
         int nov = 0;
+        bool linrep = false;
         for (int q = 0; q < c->n_terms; q++) {
             const fcc_term &tm = c->terms[q];
             if (tm.kind == FCC_TERM_OVERLAP) {
                 overlaps[nov++] = q;
+                continue;
+            }
+            if (tm.kind == FCC_TERM_LINEAR_REPULSION) {
+                // g(d) = k·d for d > 0 (src/energies/contact.jl:39-41), summed per type of body i
+                for (int t = 0; t < FCC_MAX_TYPES; t++)
+                    if (type_mask(tm) & (1u << t)) p.krep[t] += (T)tm.p[0];
+                linrep = true;
                 continue;
             }
             p.bkind[nb] = tm.kind;
@@ -301,7 +307,8 @@ struct Sim final : fcc_sim {
             nb++;
         }
         p.nbody = nb;
-        has_contact = nov > 0;
+        has_contact = nov > 0 || linrep;
+        has_linrep = linrep;
         p.ncontact = has_contact ? 1 : 0;
         // combined Overlap table: right fold over the terms (contact.jl:46-51)
         for (int rel = 0; rel < 2; rel++)
@@ -544,7 +551,7 @@ struct Sim final : fcc_sim {
     // The fast kernel needs K in {5, 7}, at most two coupling classes and
     // neighbour lists shorter than 2^NPLANE (its per-node hit counters).
     bool fast_ok() const {
-        return (K == 5 || K == 7) && p.ncls <= 2 && (!has_contact || maxcount < (1 << NPLANE));
+        return (K == 5 || K == 7) && p.ncls <= 2 && !has_linrep && (!has_contact || maxcount < (1 << NPLANE));
     }
 
     int launch_forces(int use_status, hipEvent_t eb = nullptr, hipEvent_t ee = nullptr) {
@@ -645,9 +652,9 @@ struct Sim final : fcc_sim {
             HIPCHK(hipMemcpyAsync(&st, d.st, sizeof st, hipMemcpyDeviceToHost, stream));
             HIPCHK(hipStreamSynchronize(stream));
             if (st.overflow <= d.cap) {
-                if (fast_ok() != (bool)((K == 5 || K == 7) && p.ncls <= 2 && (!has_contact || st.overflow < (1 << NPLANE))))
-                    gen++;  // the forces kernel baked into the graph changes
+                const bool fast_before = fast_ok();
                 maxcount = st.overflow;
+                if (fast_ok() != fast_before) gen++;  // the forces kernel baked into the graph changes
                 break;
             }
             const int32_t cap = 2 * st.overflow;

@@ -82,6 +82,7 @@ struct Params {
     int small_box;  // a node offset may need a second minimum image
     double invLx, invLy;  // 1/L: minimum image of neighbour offsets outside small boxes
     int cls_uniform;      // every pair of types (related or not) couples in class 0
+    T krep[FCC_MAX_TYPES];  // LinearRepulsion: Σ k of the terms applying to body type ti
 };
 
 // Device buffers of one simulation. Per-body coefficient rows are stored

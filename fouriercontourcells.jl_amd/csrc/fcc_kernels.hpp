@@ -142,6 +142,7 @@ __device__ __forceinline__ void k_forces_impl(const PP &p, const DP &d, int use_
             load_row<T, KM>(lj, d.lab + (int64_t)j * p.ldc, K);
             const T r0j = d.R0[j];
             const T Jv = p.Jc[pi == d.parent[j] ? 1 : 0][ti][d.type[j]];
+            const T kr = p.krep[ti];
             while (mask) {
                 const int t = __builtin_ctzll(mask);
                 mask &= mask - 1;
@@ -154,7 +155,7 @@ __device__ __forceinline__ void k_forces_impl(const PP &p, const DP &d, int use_
                 const T cp = dist > T(0) ? dx / dist : T(1);
                 const T sp = dist > T(0) ? dy / dist : T(0);
                 const T dd = r0j * contour_dir<T, KM>(lj, K, cp, sp) - dist;
-                sP[kk][lane] += dd >= T(0) ? Jv : T(0);
+                sP[kk][lane] += (dd >= T(0) ? Jv : T(0)) + (dd > T(0) ? kr * dd : T(0));
             }
         }
     }

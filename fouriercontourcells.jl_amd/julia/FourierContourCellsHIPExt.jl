@@ -10,7 +10,7 @@
 #   run!(sim, 10_000)
 #   p = shape_index(sim)
 #
-# Terms or sizes without a native kernel (user BodyTerms, LinearRepulsion, bond
+# Terms or sizes without a native kernel (user BodyTerms, bond
 # terms, nθ > 64, more than 4 body types) make `fcc_create` return
 # FCC_ERR_UNSUPPORTED; the extension then builds the reference Simulation on
 # the CPU backend instead (a user-defined term runs on the reference path).
@@ -19,7 +19,7 @@ module FourierContourCellsHIPExt
 
 using FourierContourCells
 using FourierContourCells: Bodies, PeriodicBox, BodyType, EnergyTerm, OnTypes, Tension,
-    PerimeterElasticity, Overlap, is_body_term, is_contact_term, is_bond_term, check_parents,
+    PerimeterElasticity, Overlap, LinearRepulsion, is_body_term, is_contact_term, is_bond_term, check_parents,
     nbodies, to_host
 import FourierContourCells: step!, run!, compute_gradients!, update_neighbors!, shape_index,
     perimeters, areas, positions, unwrapped_positions, body_energy
@@ -81,6 +81,8 @@ function fcc_term(t)
     elseif base isa Overlap
         nt = size(base.J, 1)
         return FccTerm(16, length(on), ont, (0.0, 0.0), table(Float64.(base.J), nt), table(Float64.(base.Jrelated), nt))
+    elseif base isa LinearRepulsion
+        return FccTerm(17, length(on), ont, (Float64(base.k), 0.0), ntuple(_ -> 0.0, 16), ntuple(_ -> 0.0, 16))
     end
     return nothing  # no native kernel
 end

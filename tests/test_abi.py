@@ -67,10 +67,12 @@ def test_library_validates_before_touching_the_device():
     assert eng.ensemble_create(0, None, None, ctypes.byref(h)) == _abi.FCC_ERR_ARGUMENT
     assert "replica" in eng.last_error()
     b, box = F.hexagonal_monolayer(4, 4)
-    bad = F.api._config(b, box, (F.BodyType(),), (F.LinearRepulsion(1.0),), 0.005, 40, "fp32", 1, 0.3, 0)
+    bad = F.api._config(b, box, (F.BodyType(),), (F.Tension(0.01),), 0.005, 80, "fp32", 1, 0.3, 0)
     st = b.struct()
     assert eng.create(ctypes.byref(bad), ctypes.byref(st), ctypes.byref(h)) == _abi.FCC_ERR_UNSUPPORTED
-    assert "LinearRepulsion" in eng.last_error()
+    assert "64" in eng.last_error()
+    bad.n_theta = 10  # must exceed 2·maxmode (src/basis.jl:38)
+    assert eng.create(ctypes.byref(bad), ctypes.byref(st), ctypes.byref(h)) == _abi.FCC_ERR_ARGUMENT
     cfgs = (_abi.FccConfig * 2)()
     structs = (_abi.FccBodies * 2)()
     b8, box8 = F.hexagonal_monolayer(4, 4, maxmode=8)

@@ -262,3 +262,36 @@ def test_generic_kernel_matches_oracle(hip, oracle_engine, precision):
     scale = np.max(np.abs(ga.gx))
     np.testing.assert_allclose(gg.gx, ga.gx, rtol=tol, atol=tol * scale)
     np.testing.assert_allclose(gg.gcoef, ga.gcoef, rtol=tol, atol=tol * np.max(np.abs(ga.gcoef)))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "mixed_fp16"])
+def test_linear_repulsion_matches_oracle(hip, oracle_engine, precision):
+    # LinearRepulsion g(d) = k·d for d > 0 (src/energies/contact.jl:30-41),
+    # restricted to cells, with Overlap on all bodies: the generic forces
+    # kernel accumulates the depth-dependent density per node
+    say("linear repulsion", precision)
+    cells, box = F.hexagonal_monolayer(6, 4, packing=1.05, seed=2, maxmode=6)
+    b = F.with_nuclei(cells, ratio=0.4)
+    rng = np.random.default_rng(9)
+    b.coef[1:, :] = 0.04 * rng.standard_normal((b.coef.shape[0] - 1, len(b)))
+    energies = (F.Tension(0.1), F.OnTypes(F.LinearRepulsion(3.0), 1),
+                F.Overlap([[1.0, 1.0], [1.0, 1.0]], related=[[0.0, -2.0], [-2.0, 0.0]]))
+    types = (F.BodyType(v0=0.2, Dr=0.1), F.BodyType(Dr=0.1, drag="parent"))
+    a = F.refresh(F.Simulation(b, box, types=types, energies=energies, debug=True, engine=oracle_engine))
+    g = F.refresh(F.Simulation(b, box, types=types, energies=energies, debug=True, precision=precision))
+    tol = 1e-10 if precision == "fp64" else 2e-4
+    ga, gg = a.grad, g.grad
+    depth = ga.P[(ga.P != 0) & (ga.P != 1.0) & (ga.P != -2.0)]
+    assert depth.size > 20  # nodes where the depth-dependent term contributes
+    if precision == "fp64":
+        np.testing.assert_allclose(gg.P, ga.P, rtol=tol, atol=tol)
+    scale = np.max(np.abs(ga.gx))
+    np.testing.assert_allclose(gg.gx, ga.gx, rtol=tol, atol=tol * scale)
+    np.testing.assert_allclose(gg.gy, ga.gy, rtol=tol, atol=tol * scale)
+    np.testing.assert_allclose(gg.gth, ga.gth, rtol=tol, atol=tol * np.max(np.abs(ga.gth)))
+    np.testing.assert_allclose(gg.gcoef, ga.gcoef, rtol=tol, atol=tol * np.max(np.abs(ga.gcoef)))
+    if precision == "fp64":
+        F.step(a, 20)
+        F.step(g, 20)
+        np.testing.assert_allclose(g.bodies.x, a.bodies.x, rtol=1e-8)
+        np.testing.assert_allclose(g.bodies.coef, a.bodies.coef, rtol=1e-8, atol=1e-12)
