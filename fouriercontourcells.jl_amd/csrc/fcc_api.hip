@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "fcc_forces_mfma.hpp"
+#include "fcc_observe.hpp"
 #include "../../include/fcc_bench.h"
 
 using namespace fcc;
@@ -101,7 +102,7 @@ static int validate(const fcc_config *cfg, const fcc_bodies *b) {
         const int k = cfg->terms[q].kind;
         if (k == FCC_TERM_TENSION || k == FCC_TERM_PERIMETER_ELASTICITY || k == FCC_TERM_BENDING)
             nbody++;
-        else if (k != FCC_TERM_OVERLAP && k != FCC_TERM_LINEAR_REPULSION)
+        else if (k != FCC_TERM_OVERLAP && k != FCC_TERM_LINEAR_REPULSION && k != FCC_TERM_HARMONIC_BOND)
             return fail(FCC_ERR_ARGUMENT, "energy terms must be BodyTerms, ContactTerms or BondTerms");
     }
     if (nbody > MAXB) return fail(FCC_ERR_UNSUPPORTED, "at most %d native body terms", MAXB);
@@ -177,6 +178,7 @@ struct fcc_sim {
     virtual int set_bodies(const fcc_bodies *b) = 0;
     virtual int get_bodies(fcc_bodies *b) = 0;
     virtual int set_dt(double dt) = 0;
+    virtual int set_bonds(int64_t n, const int32_t *pairs) = 0;
     virtual int step(int64_t n) = 0;
     virtual int run(int64_t n) = 0;
     virtual int update_neighbors(int force) = 0;
@@ -192,6 +194,11 @@ struct fcc_sim {
     virtual int time_step_kernels(int64_t, double *, double *) = 0;
     virtual int run_timed(int64_t, double *, double *) = 0;
     virtual int time_kernels(int64_t, double *, double *) = 0;
+    virtual int checkpoint(void *, int64_t, int64_t *) = 0;
+    virtual int restore(const void *, int64_t) = 0;
+    virtual int observer_start(const fcc_observer_config *) = 0;
+    virtual int observe() = 0;
+    virtual int observer_read(int32_t *, int64_t *, double *, double *, double *, uint64_t *) = 0;
 };
 
 template <typename T>
@@ -204,6 +211,8 @@ struct Sim final : fcc_sim {
     int64_t N = 0;
     int K = 0, nc = 0, ldc = 0, nth = 0, KM = 0;
     bool mixed = false, debug = false, has_contact = false, has_pdrag = false, has_linrep = false;
+    bool has_bonds = false;  // bond terms and at least one bond
+    int64_t nbonds = 0;
     double skin = 0.0, range = 0.0;
     int32_t ncx = 1, ncy = 1;
     double wx = 0, wy = 0;
@@ -290,6 +299,13 @@ struct Sim final : fcc_sim {
             const fcc_term &tm = c->terms[q];
             if (tm.kind == FCC_TERM_OVERLAP) {
                 overlaps[nov++] = q;
+                continue;
+            }
+            if (tm.kind == FCC_TERM_HARMONIC_BOND) {
+                p.bdmask[p.nbondt] = type_mask(tm);
+                p.bdk[p.nbondt] = (T)tm.p[0];
+                p.bdr0[p.nbondt] = (T)tm.p[1];
+                p.nbondt++;
                 continue;
             }
             if (tm.kind == FCC_TERM_LINEAR_REPULSION) {
@@ -554,7 +570,17 @@ struct Sim final : fcc_sim {
         return (K == 5 || K == 7) && p.ncls <= 2 && !has_linrep && (!has_contact || maxcount < (1 << NPLANE));
     }
 
+    // Forces of one step: the forces kernel, then the bond kernel when bond
+    // terms act on bonds (compute_gradients!, src/simulation.jl:150-158).
     int launch_forces(int use_status, hipEvent_t eb = nullptr, hipEvent_t ee = nullptr) {
+        int rc = launch_forces_kernel(use_status, eb, ee);
+        if (rc || !has_bonds) return rc;
+        hipLaunchKernelGGL((k_bonds<T>), blocks(N, 256), 256, 0, stream, p, d, use_status ? 1 : 0);
+        HIPCHK(hipGetLastError());
+        return FCC_OK;
+    }
+
+    int launch_forces_kernel(int use_status, hipEvent_t eb, hipEvent_t ee) {
         const unsigned g = blocks(N, 64);
         const int dbg = debug ? 1 : 0;
         if constexpr (sizeof(T) == 4) {
@@ -1155,6 +1181,264 @@ struct Sim final : fcc_sim {
         return FCC_OK;
     }
 
+    // BondList(bonds, N) (src/energies/bonds.jl:54-72): checks, then each
+    // bond (i, j) appended to the partner lists of i and of j in the order
+    // given; stored slot-major for coalesced reads.
+    int set_bonds(int64_t n, const int32_t *pairs) override {
+        if (n < 0 || (n > 0 && !pairs)) return fail(FCC_ERR_ARGUMENT, "bad bond arguments");
+        std::vector<int32_t> count(N, 0);
+        std::vector<int64_t> key(n);
+        for (int64_t q = 0; q < n; q++) {
+            const int32_t i = pairs[2 * q], j = pairs[2 * q + 1];
+            if (i < 1 || i > N || j < 1 || j > N)
+                return fail(FCC_ERR_ARGUMENT, "bond (%d, %d) refers to a body outside 1:%lld", i, j, (long long)N);
+            if (i == j) return fail(FCC_ERR_ARGUMENT, "bond (%d, %d) connects a body to itself", i, j);
+            key[q] = (int64_t)std::min(i, j) * (N + 1) + std::max(i, j);
+            count[i - 1]++;
+            count[j - 1]++;
+        }
+        std::sort(key.begin(), key.end());
+        for (int64_t q = 1; q < n; q++)
+            if (key[q] == key[q - 1])
+                return fail(FCC_ERR_ARGUMENT, "duplicate bond (%lld, %lld)", (long long)(key[q] / (N + 1)),
+                            (long long)(key[q] % (N + 1)));
+        int32_t cap = 1;
+        for (int32_t c : count) cap = std::max(cap, c);
+        std::vector<int32_t> idx((size_t)cap * N, 0);
+        std::fill(count.begin(), count.end(), 0);
+        for (int64_t q = 0; q < n; q++) {
+            const int32_t a = pairs[2 * q] - 1, b = pairs[2 * q + 1] - 1;
+            idx[(size_t)count[a]++ * N + a] = b;
+            idx[(size_t)count[b]++ * N + b] = a;
+        }
+        HIPCHK(hipStreamSynchronize(stream));  // no step in flight reads the old list
+        release(d.bidx);
+        release(d.bcount);
+        d.bidx = d.bcount = nullptr;
+        nbonds = n;
+        if (n > 0) {
+            int rc;
+            if ((rc = alloc(&d.bidx, idx.size())) || (rc = alloc(&d.bcount, (size_t)N)) || (rc = upload(d.bidx, idx)) ||
+                (rc = upload(d.bcount, count)))
+                return rc;
+        }
+        has_bonds = p.nbondt > 0 && n > 0;
+        gen++;  // the captured step gains or loses the bond kernel
+        return FCC_OK;
+    }
+
+    // ---------------------------------------------------- checkpoints
+    // Everything a continued run reads: the body state, the lab frame and
+    // bounds of the last update, the Verlet reference and the neighbour list
+    // (its order fixes the summation order of the contact sums), the device
+    // clock (Philox draw index) and rebuild flag, and the host counters.
+    struct CkptHeader {
+        char magic[8];
+        int32_t version, precision;
+        int64_t N;
+        int32_t ncoef, ldc, nth, cap;
+        int64_t clock, step, builds;
+        double t, dt, Lx, Ly, skin;
+        uint64_t seed;
+        int32_t rebuild, valid, maxcount, ntypes;
+    };
+    static constexpr char CKPT_MAGIC[8] = {'F', 'C', 'C', 'C', 'K', 'P', 'T', '1'};
+
+    size_t ckpt_bytes(int32_t cap) const {
+        const size_t n = (size_t)N;
+        return sizeof(CkptHeader) + n * (2 * sizeof(double) + 4 * sizeof(int32_t) + 4 * sizeof(T)) +
+               2 * n * ldc * sizeof(T) + n * (2 * sizeof(double) + sizeof(T)) + (size_t)cap * n * sizeof(int32_t) +
+               n * sizeof(int32_t);
+    }
+
+    // (device pointer, bytes) of every array in the checkpoint, in order
+    std::vector<std::pair<void *, size_t>> ckpt_arrays() {
+        const size_t n = (size_t)N;
+        return {{d.x, n * 8},           {d.y, n * 8},          {d.imgx, n * 4},           {d.imgy, n * 4},
+                {d.type, n * 4},        {d.parent, n * 4},     {d.th, n * sizeof(T)},     {d.R0, n * sizeof(T)},
+                {d.rmax, n * sizeof(T)}, {d.rmin, n * sizeof(T)}, {d.coef, n * ldc * sizeof(T)},
+                {d.lab, n * ldc * sizeof(T)}, {d.x0, n * 8},    {d.y0, n * 8},             {d.rmax0, n * sizeof(T)},
+                {d.nidx, (size_t)d.cap * n * 4}, {d.ncount, n * 4}};
+    }
+
+    int checkpoint(void *buf, int64_t bytes, int64_t *need) override {
+        const size_t total = ckpt_bytes(d.cap);
+        if (need) *need = (int64_t)total;
+        if (!buf) return FCC_OK;
+        if (bytes < (int64_t)total) return fail(FCC_ERR_DIMENSION, "checkpoint needs %lld bytes", (long long)total);
+        DevStatus st;
+        HIPCHK(hipMemcpyAsync(&st, d.st, sizeof st, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        CkptHeader h{};
+        std::memcpy(h.magic, CKPT_MAGIC, 8);
+        h.version = 1;
+        h.precision = cfg.precision;
+        h.N = N;
+        h.ncoef = nc;
+        h.ldc = ldc;
+        h.nth = nth;
+        h.cap = d.cap;
+        h.clock = st.clock;
+        h.step = nstep;
+        h.builds = builds;
+        h.t = t;
+        h.dt = cfg.dt;
+        h.Lx = cfg.Lx;
+        h.Ly = cfg.Ly;
+        h.skin = cfg.skin;
+        h.seed = cfg.seed;
+        h.rebuild = st.rebuild;
+        h.valid = valid ? 1 : 0;
+        h.maxcount = maxcount;
+        h.ntypes = cfg.n_types;
+        char *out = (char *)buf;
+        std::memcpy(out, &h, sizeof h);
+        size_t off = sizeof h;
+        for (auto &a : ckpt_arrays()) {
+            HIPCHK(hipMemcpyAsync(out + off, a.first, a.second, hipMemcpyDeviceToHost, stream));
+            off += a.second;
+        }
+        HIPCHK(hipStreamSynchronize(stream));
+        return FCC_OK;
+    }
+
+    int restore(const void *buf, int64_t bytes) override {
+        if (!buf || bytes < (int64_t)sizeof(CkptHeader)) return fail(FCC_ERR_DIMENSION, "checkpoint too short");
+        CkptHeader h;
+        std::memcpy(&h, buf, sizeof h);
+        if (std::memcmp(h.magic, CKPT_MAGIC, 8) != 0 || h.version != 1)
+            return fail(FCC_ERR_ARGUMENT, "not a checkpoint of this library");
+        if (h.precision != cfg.precision || h.N != N || h.ncoef != nc || h.ldc != ldc || h.nth != nth ||
+            h.ntypes != cfg.n_types)
+            return fail(FCC_ERR_DIMENSION, "checkpoint of another precision or size");
+        if (h.dt != cfg.dt || h.Lx != cfg.Lx || h.Ly != cfg.Ly || h.skin != cfg.skin || h.seed != cfg.seed)
+            return fail(FCC_ERR_ARGUMENT, "checkpoint of another configuration (dt, box, skin or seed)");
+        if (h.cap < 1 || bytes < (int64_t)ckpt_bytes(h.cap)) return fail(FCC_ERR_DIMENSION, "checkpoint truncated");
+        const char *in = (const char *)buf;
+        // body types and parents fix the children lists built at creation
+        {
+            const size_t n = (size_t)N, off = sizeof h + n * (8 + 8 + 4 + 4);
+            std::vector<int32_t> ty, pa;
+            int rc;
+            if ((rc = download(ty, d.type, n)) || (rc = download(pa, d.parent, n))) return rc;
+            if (std::memcmp(ty.data(), in + off, n * 4) != 0 || std::memcmp(pa.data(), in + off + n * 4, n * 4) != 0)
+                return fail(FCC_ERR_ARGUMENT, "checkpoint with other body types or parents");
+        }
+        HIPCHK(hipStreamSynchronize(stream));
+        if (h.cap != d.cap) {
+            int32_t *ni = nullptr;
+            int rc = alloc(&ni, (size_t)h.cap * N);
+            if (rc) return rc;
+            release(d.nidx);
+            d.nidx = ni;
+            d.cap = h.cap;
+        }
+        size_t off = sizeof h;
+        for (auto &a : ckpt_arrays()) {
+            HIPCHK(hipMemcpyAsync(a.first, in + off, a.second, hipMemcpyHostToDevice, stream));
+            off += a.second;
+        }
+        DevStatus st;
+        HIPCHK(hipMemcpyAsync(&st, d.st, sizeof st, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        st.rebuild = h.rebuild;
+        st.update = 0;
+        st.overflow = 0;
+        st.stop = 0;
+        st.clock = h.clock;
+        st.target = INT64_MAX;
+        HIPCHK(hipMemcpyAsync(d.st, &st, sizeof st, hipMemcpyHostToDevice, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        nstep = h.step;
+        builds = h.builds;
+        t = h.t;
+        valid = h.valid != 0;
+        maxcount = h.maxcount;
+        gen++;  // kernel choice and list buffers may differ from the captured step
+        return FCC_OK;
+    }
+
+    // ---------------------------------------------------- observables
+    // Device buffers of the current observation (fcc_observer_*).
+    struct Observer {
+        bool on = false;
+        fcc_observer_config cfg{};
+        int64_t nblk = 0, n = 0;  // workgroups of k_observe; samples taken
+        double *x0 = nullptr, *y0 = nullptr, *part = nullptr, *msd = nullptr, *pm = nullptr, *p2 = nullptr;
+        unsigned long long *hist = nullptr;
+        int64_t *clk = nullptr;
+    } obs;
+
+    void observer_free() {
+        release(obs.x0);
+        release(obs.y0);
+        release(obs.part);
+        release(obs.msd);
+        release(obs.pm);
+        release(obs.p2);
+        release(obs.hist);
+        release(obs.clk);
+        obs = Observer{};
+    }
+
+    int observer_start(const fcc_observer_config *c) override {
+        HIPCHK(hipStreamSynchronize(stream));  // no sample of the old observation is in flight
+        observer_free();
+        const int nt = cfg.n_types;
+        const size_t ms = (size_t)c->max_samples, nh = (size_t)nt * (c->nbins + 2);
+        obs.cfg = *c;
+        obs.nblk = blocks(N, OBS_BLOCK);
+        int rc;
+        if ((rc = alloc(&obs.x0, N)) || (rc = alloc(&obs.y0, N)) || (rc = alloc(&obs.part, obs.nblk * nt * OBS_NV)) ||
+            (rc = alloc(&obs.msd, ms * nt)) || (rc = alloc(&obs.pm, ms * nt)) || (rc = alloc(&obs.p2, ms * nt)) ||
+            (rc = alloc(&obs.hist, nh)) || (rc = alloc(&obs.clk, ms))) {
+            observer_free();
+            return rc;
+        }
+        HIPCHK(hipMemsetAsync(obs.hist, 0, sizeof(unsigned long long) * nh, stream));
+        hipLaunchKernelGGL(k_unwrapped, blocks(N, 256), 256, 0, stream, N, d.x, d.y, d.imgx, d.imgy, p.Lx, p.Ly,
+                           obs.x0, obs.y0);
+        HIPCHK(hipGetLastError());
+        obs.on = true;
+        return FCC_OK;
+    }
+
+    int observe() override {
+        if (!obs.on) return fail(FCC_ERR_STATE, "no observation started (fcc_observer_start)");
+        if (obs.n >= obs.cfg.max_samples)
+            return fail(FCC_ERR_STATE, "the observation holds max_samples = %d samples", obs.cfg.max_samples);
+        const int nt = cfg.n_types;
+        const ObsArgs o{obs.x0, obs.y0, obs.cfg.pmin, (double)obs.cfg.nbins / (obs.cfg.pmax - obs.cfg.pmin),
+                        obs.cfg.nbins, nt, obs.part, obs.hist};
+        const size_t sh = sizeof(uint32_t) * nt * (obs.cfg.nbins + 2);
+        const unsigned g = (unsigned)obs.nblk;
+        switch (KM) {
+        case 5: hipLaunchKernelGGL((k_observe<T, 5>), g, OBS_BLOCK, sh, stream, p, d, o); break;
+        case 7: hipLaunchKernelGGL((k_observe<T, 7>), g, OBS_BLOCK, sh, stream, p, d, o); break;
+        default: hipLaunchKernelGGL((k_observe<T, 15>), g, OBS_BLOCK, sh, stream, p, d, o);
+        }
+        hipLaunchKernelGGL(k_observe_finish, 1, OBS_BLOCK, 0, stream, obs.part, obs.nblk, nt, obs.n, d.st, obs.msd,
+                           obs.pm, obs.p2, obs.clk);
+        HIPCHK(hipGetLastError());
+        obs.n++;
+        return FCC_OK;
+    }
+
+    int observer_read(int32_t *ns, int64_t *clock, double *msd, double *pm, double *p2, uint64_t *hist) override {
+        if (!obs.on) return fail(FCC_ERR_STATE, "no observation started (fcc_observer_start)");
+        const size_t nt = (size_t)cfg.n_types, n = (size_t)obs.n;
+        if (ns) *ns = (int32_t)obs.n;
+        if (clock && n) HIPCHK(hipMemcpyAsync(clock, obs.clk, sizeof(int64_t) * n, hipMemcpyDeviceToHost, stream));
+        if (msd && n) HIPCHK(hipMemcpyAsync(msd, obs.msd, sizeof(double) * n * nt, hipMemcpyDeviceToHost, stream));
+        if (pm && n) HIPCHK(hipMemcpyAsync(pm, obs.pm, sizeof(double) * n * nt, hipMemcpyDeviceToHost, stream));
+        if (p2 && n) HIPCHK(hipMemcpyAsync(p2, obs.p2, sizeof(double) * n * nt, hipMemcpyDeviceToHost, stream));
+        if (hist)
+            HIPCHK(hipMemcpyAsync(hist, obs.hist, sizeof(uint64_t) * nt * (obs.cfg.nbins + 2), hipMemcpyDeviceToHost,
+                                  stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        return FCC_OK;
+    }
+
     int get_counters(int64_t *st, double *tt, int64_t *clock, int64_t *b) override {
         DevStatus s;
         HIPCHK(hipMemcpyAsync(&s, d.st, sizeof s, hipMemcpyDeviceToHost, stream));
@@ -1211,6 +1495,7 @@ int fcc_destroy(fcc_sim *s) {
 int fcc_set_bodies(fcc_sim *s, const fcc_bodies *b) { FCC_HANDLE(s); return s->set_bodies(b); }
 int fcc_get_bodies(fcc_sim *s, fcc_bodies *b) { FCC_HANDLE(s); return s->get_bodies(b); }
 int fcc_set_dt(fcc_sim *s, double dt) { FCC_HANDLE(s); return s->set_dt(dt); }
+int fcc_set_bonds(fcc_sim *s, int64_t n, const int32_t *pairs) { FCC_HANDLE(s); return s->set_bonds(n, pairs); }
 int fcc_step(fcc_sim *s, int64_t n) { FCC_HANDLE(s); return s->step(n); }
 int fcc_run(fcc_sim *s, int64_t n) { FCC_HANDLE(s); return s->run(n); }
 int fcc_update_neighbors(fcc_sim *s, int32_t force) { FCC_HANDLE(s); return s->update_neighbors(force); }
@@ -1259,6 +1544,46 @@ int fcc_run_timed(fcc_sim *s, int64_t n, double *fms, double *ums) {
     FCC_HANDLE(s);
     if (n < 0) return fail(FCC_ERR_ARGUMENT, "steps must be non-negative");
     return s->run_timed(n, fms, ums);
+}
+
+static int check_observer(const fcc_observer_config *c) {
+    if (!c) return fail(FCC_ERR_ARGUMENT, "null observer configuration");
+    if (c->nbins < 1 || c->nbins > FCC_OBS_MAX_BINS) return fail(FCC_ERR_ARGUMENT, "nbins must be in 1:%d", FCC_OBS_MAX_BINS);
+    if (c->max_samples < 1) return fail(FCC_ERR_ARGUMENT, "max_samples must be positive");
+    if (!(c->pmin < c->pmax) || !std::isfinite(c->pmin) || !std::isfinite(c->pmax))
+        return fail(FCC_ERR_ARGUMENT, "the histogram range needs finite pmin < pmax");
+    return FCC_OK;
+}
+
+int fcc_observer_start(fcc_sim *s, const fcc_observer_config *c) {
+    FCC_HANDLE(s);
+    int rc = check_observer(c);
+    return rc ? rc : s->observer_start(c);
+}
+int fcc_observe(fcc_sim *s) { FCC_HANDLE(s); return s->observe(); }
+int fcc_checkpoint_save(fcc_sim *s, void *buf, int64_t bytes, int64_t *needed) {
+    FCC_HANDLE(s);
+    if (bytes < 0) return fail(FCC_ERR_ARGUMENT, "negative size");
+    return s->checkpoint(buf, bytes, needed);
+}
+int fcc_checkpoint_load(fcc_sim *s, const void *buf, int64_t bytes) {
+    FCC_HANDLE(s);
+    return s->restore(buf, bytes);
+}
+int fcc_run_observed(fcc_sim *s, int64_t n, int64_t every) {
+    FCC_HANDLE(s);
+    if (n < 0 || every < 1) return fail(FCC_ERR_ARGUMENT, "need n >= 0 and every >= 1");
+    for (int64_t done = 0; done < n;) {
+        const int64_t k = std::min(every, n - done);
+        int rc;
+        if ((rc = s->run(k)) || (rc = s->observe())) return rc;
+        done += k;
+    }
+    return FCC_OK;
+}
+int fcc_observer_read(fcc_sim *s, int32_t *ns, int64_t *clock, double *msd, double *pm, double *p2, uint64_t *hist) {
+    FCC_HANDLE(s);
+    return s->observer_read(ns, clock, msd, pm, p2, hist);
 }
 
 int fcc_philox_noise(uint64_t seed, uint64_t draw, int64_t n, int32_t precision, void *out) {
@@ -1371,6 +1696,32 @@ int fcc_ensemble_set_launch(fcc_ensemble *e, const fcc_launch *lc) { FCC_ENS(e);
 int fcc_ensemble_autotune(fcc_ensemble *e, int64_t steps, fcc_launch *best, double *ms) {
     FCC_ENS(e);
     return e->autotune(steps, best, ms);
+}
+
+int fcc_ensemble_observer_start(fcc_ensemble *e, const fcc_observer_config *c) {
+    FCC_ENS(e);
+    int rc = check_observer(c);
+    for (int32_t r = 0; r < e->size() && !rc; r++) rc = e->member(r)->observer_start(c);
+    return rc;
+}
+
+int fcc_ensemble_observe(fcc_ensemble *e) {
+    FCC_ENS(e);
+    int rc = FCC_OK;
+    for (int32_t r = 0; r < e->size() && !rc; r++) rc = e->member(r)->observe();
+    return rc;
+}
+
+int fcc_ensemble_run_observed(fcc_ensemble *e, int64_t n, int64_t every) {
+    FCC_ENS(e);
+    if (n < 0 || every < 1) return fail(FCC_ERR_ARGUMENT, "need n >= 0 and every >= 1");
+    for (int64_t done = 0; done < n;) {
+        const int64_t k = std::min(every, n - done);
+        int rc;
+        if ((rc = e->run(k)) || (rc = fcc_ensemble_observe(e))) return rc;
+        done += k;
+    }
+    return FCC_OK;
 }
 
 }  // extern "C"

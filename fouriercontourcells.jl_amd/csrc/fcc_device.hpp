@@ -83,6 +83,10 @@ struct Params {
     double invLx, invLy;  // 1/L: minimum image of neighbour offsets outside small boxes
     int cls_uniform;      // every pair of types (related or not) couples in class 0
     T krep[FCC_MAX_TYPES];  // LinearRepulsion: Σ k of the terms applying to body type ti
+    // HarmonicBond terms (k, r0) with OnTypes masks, in the order given
+    int nbondt;
+    uint32_t bdmask[FCC_MAX_TERMS];
+    T bdk[FCC_MAX_TERMS], bdr0[FCC_MAX_TERMS];
 };
 
 // Device buffers of one simulation. Per-body coefficient rows are stored
@@ -105,6 +109,7 @@ struct Ptrs {
     int32_t *cidx, *ccount;
     int32_t ccap;
     T *disp;
+    int32_t *bidx, *bcount;  // bond partners, slot-major (bidx[q*N + i]); null without bonds
     const T *S;   // [3][nth][ldc]: C, D, DD rows
     const T *cs;  // [nth][2]: cos φk, sin φk
     const T *Sc;  // [3][nth][ldc]: C, D, DD rows in coefficient order [a0, cos.., sin..]

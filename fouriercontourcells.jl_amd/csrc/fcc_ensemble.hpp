@@ -73,6 +73,7 @@ struct Ensemble final : fcc_ensemble {
     fcc_launch lc{256, 1, 0, 0};
     fcc_launch tiles_lc{0, 0, 0, 0};
     int var = -1, tiles_var = -1, ncls = 1;  // forces kernel: 2 MFMA, 1 fast, 0 generic
+    bool bonds = false;                      // a replica has bond terms on bonds: bond kernel after forces
     hipGraphExec_t gexec = nullptr, gexec_n = nullptr;  // one step; GRAPH_STEPS steps
     bool graph_ok = false;
     std::vector<double> interval;  // mean steps between rebuilds, per replica
@@ -148,7 +149,17 @@ struct Ensemble final : fcc_ensemble {
         return FCC_OK;
     }
 
+    // Forces of one ensemble step, then the bond kernel over the update tiles
+    // when a replica has bonds (replicas without bonds return at once).
     int launch_forces(hipEvent_t eb = nullptr, hipEvent_t ee = nullptr) {
+        int rc = launch_forces_kernel(eb, ee);
+        if (rc || !bonds) return rc;
+        hipLaunchKernelGGL((k_bonds_ens<T>), gu, lc.update_block, 0, stream, dP, dD, tu, 2);
+        HIPCHK(hipGetLastError());
+        return FCC_OK;
+    }
+
+    int launch_forces_kernel(hipEvent_t eb, hipEvent_t ee) {
         const Sim<T> &a = *m[0];
         const int dbg = a.debug ? 1 : 0;
         if constexpr (sizeof(T) == 4) {
@@ -225,16 +236,18 @@ struct Ensemble final : fcc_ensemble {
     // Tables, tiles and the captured step, brought up to date with the
     // replicas (kernel variant, list capacities, dt) and the launch shape.
     int prepare() {
-        bool fast = true;
+        bool fast = true, bd = false;
         int nc = 1;
         for (Sim<T> *s : m) {
             fast = fast && s->fast_ok();
             nc = std::max(nc, s->p.ncls);
+            bd = bd || s->has_bonds;
         }
         const int v = fast ? ((sizeof(T) == 4 && m[0]->mfma_T) ? 2 : 1) : 0;
-        if (v != var || nc != ncls) graph_ok = false;
+        if (v != var || nc != ncls || bd != bonds) graph_ok = false;
         var = v;
         ncls = nc;
+        bonds = bd;
         const size_t R = m.size();
         for (size_t r = 0; r < R; r++) {
             Sim<T> &s = *m[r];

@@ -759,14 +759,65 @@ __global__ __launch_bounds__(256) void k_philox(int64_t n, const uint32_t *ctr, 
     out[4 * i + 3] = c3;
 }
 
-// Contours R, R', R'' at the nodes (nθ x N, column-major) and the perimeter
-// and area sums of src/observables.jl:6-14.
+// bond_kernel! (src/energies/bonds.jl:98-110) for HarmonicBond terms
+// (:25-29): gx, gy += the sum over the partners j of dE/dDelta at Delta = the
+// minimum image of r_i - r_j, the terms summed as the right fold of
+// _bond_gradient (:80-85). Runs after the forces kernel of a step; with
+// use_status it skips a frozen step like the other kernels.
+template <typename T, int UB, typename PP, typename DP>
+__device__ __forceinline__ void k_bonds_impl(const PP &p, const DP &d, int use_status, int64_t blk) {
+#pragma clang fp contract(off)
+    const int64_t i = blk * (UB ? UB : (int)blockDim.x) + threadIdx.x;
+    if (i >= p.N || d.bcount == nullptr) return;
+    if (use_status && d.st->update != 0) return;
+    const int n = d.bcount[i];
+    if (n == 0) return;
+    const int ti = d.type[i];
+    const double xi = d.x[i], yi = d.y[i];
+    T g0 = T(0), g1 = T(0);
+    for (int q = 0; q < n; q++) {
+        const int64_t j = d.bidx[(int64_t)q * p.N + i];
+        const T dx = (T)min_image(xi - d.x[j], p.Lx);
+        const T dy = (T)min_image(yi - d.y[j], p.Ly);
+        const int tj = d.type[j];
+        T a0 = T(0), a1 = T(0);
+        for (int t = p.nbondt - 1; t >= 0; t--) {
+            T f = T(0);
+            if (((p.bdmask[t] >> ti) & 1u) && ((p.bdmask[t] >> tj) & 1u)) {
+                const T r = sqrt(dx * dx + dy * dy);
+                f = r > T(0) ? p.bdk[t] * (T(1) - p.bdr0[t] / r) : (p.bdr0[t] == T(0) ? p.bdk[t] : T(0));
+            }
+            a0 = f * dx + a0;
+            a1 = f * dy + a1;
+        }
+        g0 += a0;
+        g1 += a1;
+    }
+    d.gx[i] += g0;
+    d.gy[i] += g1;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_bonds(const Params<T> p, const Ptrs<T> d, int use_status) {
+    k_bonds_impl<T, 256>(p, d, use_status, blockIdx.x);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_bonds_ens(const Params<T> *ps, const Ptrs<T> *ds, const Tile *tiles,
+                                                   int use_status) {
+    const Tile tl = load_uniform(tiles, blockIdx.x);
+    if (tl.rep < 0) return;
+    const FCC_CONST Params<T> &p = as_const(ps)[tl.rep];
+    const FCC_CONST Ptrs<T> &d = as_const(ds)[tl.rep];
+    k_bonds_impl<T, 0>(p, d, use_status, tl.blk);
+}
+
+// Perimeter and area sums of body i (src/observables.jl:6-14); given R, also
+// the contours R, R', R'' at its nodes (nθ x N, column-major).
 template <typename T, int KM>
-__global__ __launch_bounds__(256) void k_geometry(const Params<T> p, const Ptrs<T> d, T *R, T *dR, T *ddR,
-                                                  T *per, T *area) {
+__device__ __forceinline__ void contour_sums(const Params<T> &p, const Ptrs<T> &d, int64_t i, T *R, T *dR, T *ddR,
+                                             T &per, T &area) {
     constexpr int NCM = 2 * KM + 1;
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= p.N) return;
     const FCC_CONST T *S = as_const(d.S);
     T l[NCM];
     load_row<T, KM>(l, d.lab + i * p.ldc, p.K);
@@ -783,9 +834,20 @@ __global__ __launch_bounds__(256) void k_geometry(const Params<T> p, const Ptrs<
         ps += sqrt(r * r + dr * dr);
         as += r * r;
     }
+    per = ps * p.dphi;
+    area = as * (p.dphi / T(2));
+}
+
+template <typename T, int KM>
+__global__ __launch_bounds__(256) void k_geometry(const Params<T> p, const Ptrs<T> d, T *R, T *dR, T *ddR,
+                                                  T *per, T *area) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= p.N) return;
+    T ps, as;
+    contour_sums<T, KM>(p, d, i, R, dR, ddR, ps, as);
     if (per) {
-        per[i] = ps * p.dphi;
-        area[i] = as * (p.dphi / T(2));
+        per[i] = ps;
+        area[i] = as;
     }
 }
 

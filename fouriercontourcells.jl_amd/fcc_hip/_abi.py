@@ -29,6 +29,7 @@ FCC_TERM_PERIMETER_ELASTICITY = 2
 FCC_TERM_BENDING = 3
 FCC_TERM_OVERLAP = 16
 FCC_TERM_LINEAR_REPULSION = 17
+FCC_TERM_HARMONIC_BOND = 32
 
 FCC_FLAG_DEBUG_BUFFERS = 1
 
@@ -84,6 +85,18 @@ class FccConfig(ctypes.Structure):
     ]
 
 
+FCC_OBS_MAX_BINS = 2046
+
+
+class FccObserverConfig(ctypes.Structure):
+    _fields_ = [
+        ("nbins", ctypes.c_int32),
+        ("max_samples", ctypes.c_int32),
+        ("pmin", ctypes.c_double),
+        ("pmax", ctypes.c_double),
+    ]
+
+
 class FccLaunch(ctypes.Structure):
     _fields_ = [
         ("update_block", ctypes.c_int32),
@@ -122,6 +135,7 @@ _SIGNATURES = {
     "set_bodies": [_vp, ctypes.POINTER(FccBodies)],
     "get_bodies": [_vp, ctypes.POINTER(FccBodies)],
     "set_dt": [_vp, ctypes.c_double],
+    "set_bonds": [_vp, _i64, _ip],
     "step": [_vp, _i64],
     "update_neighbors": [_vp, _i32],
     "compute_gradients": [_vp, _i32],
@@ -153,6 +167,15 @@ _DEVICE_ONLY = {
     "ensemble_get_launch": [_vp, ctypes.POINTER(FccLaunch)],
     "ensemble_set_launch": [_vp, ctypes.POINTER(FccLaunch)],
     "ensemble_autotune": [_vp, _i64, ctypes.POINTER(FccLaunch), _dp],
+    "checkpoint_save": [_vp, _vp, _i64, ctypes.POINTER(_i64)],
+    "checkpoint_load": [_vp, _vp, _i64],
+    "observer_start": [_vp, ctypes.POINTER(FccObserverConfig)],
+    "observe": [_vp],
+    "run_observed": [_vp, _i64, _i64],
+    "observer_read": [_vp, _ip, ctypes.POINTER(_i64), _dp, _dp, _dp, ctypes.POINTER(ctypes.c_uint64)],
+    "ensemble_observer_start": [_vp, ctypes.POINTER(FccObserverConfig)],
+    "ensemble_observe": [_vp],
+    "ensemble_run_observed": [_vp, _i64, _i64],
 }
 
 # Every symbol include/fcc.h declares.

@@ -133,12 +133,30 @@ class Overlap(EnergyTerm):
 
 @dataclass(frozen=True)
 class LinearRepulsion(EnergyTerm):
-    """g(d) = k·d for d > 0 (src/energies/contact.jl:37-41); no native kernel."""
+    """g(d) = k·d for d > 0 (src/energies/contact.jl:37-41)."""
     k: float
     kind = _abi.FCC_TERM_LINEAR_REPULSION
 
     def params(self):
         return (self.k, 0.0)
+
+
+@dataclass(frozen=True)
+class HarmonicBond(EnergyTerm):
+    """HarmonicBond(k; r0 = 0): (k/2)(|r_i - r_j| - r0)^2 between bonded
+    centres (src/energies/bonds.jl:12-29); acts on the `bonds` pairs."""
+    k: float
+    r0: float = 0.0
+    kind = _abi.FCC_TERM_HARMONIC_BOND
+
+    def params(self):
+        return (self.k, self.r0)
+
+
+def parent_bonds(bodies):
+    """parent_bonds (src/energies/bonds.jl:37): one bond (i, parent) per body
+    that is not its own parent, 1-based."""
+    return [(i + 1, int(p)) for i, p in enumerate(bodies.parent) if p != i + 1]
 
 
 @dataclass(frozen=True)
@@ -343,13 +361,38 @@ class Simulation:
     """
 
     def __init__(self, bodies, box, types, energies=(), dt=0.005, ntheta=40, precision="fp64",
-                 seed=1, skin=0.3, debug=False, engine=None):
+                 seed=1, skin=0.3, debug=False, engine=None, bonds=()):
         self._setup(engine if engine is not None else _abi.hip_engine(), bodies, box, types, energies,
                     dt, ntheta, precision, seed, skin, debug)
         h = ctypes.c_void_p()
         _check(self.engine, self.engine.create(ctypes.byref(self._cfg),
                                                ctypes.byref(self._bodies.struct()), ctypes.byref(h)))
         self._h = h
+        if len(bonds):
+            self.set_bonds(bonds)
+
+    def checkpoint(self):
+        """The run state as bytes (fcc_checkpoint_save): body state, Verlet
+        reference, neighbour list, device clock and counters."""
+        need = ctypes.c_int64()
+        _check(self.engine, self.engine.checkpoint_save(self._h, None, 0, ctypes.byref(need)))
+        buf = ctypes.create_string_buffer(need.value)
+        _check(self.engine, self.engine.checkpoint_save(self._h, buf, need.value, ctypes.byref(need)))
+        return buf.raw
+
+    def restore(self, data):
+        """Continue from `checkpoint()` bytes of a simulation with the same
+        configuration (fcc_checkpoint_load)."""
+        buf = ctypes.create_string_buffer(bytes(data), len(data))
+        _check(self.engine, self.engine.checkpoint_load(self._h, buf, len(data)))
+        return self
+
+    def set_bonds(self, bonds):
+        """The `bonds` pairs (i, j), 1-based (src/simulation.jl:14-15)."""
+        pairs = np.ascontiguousarray(np.asarray(list(bonds), dtype=np.int32).reshape(-1, 2))
+        _check(self.engine, self.engine.set_bonds(self._h, len(pairs),
+                                                  pairs.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+        self.bonds = [tuple(int(v) for v in b) for b in pairs]
 
     def _setup(self, engine, bodies, box, types, energies, dt, ntheta, precision, seed, skin, debug):
         self.engine = engine
@@ -474,9 +517,9 @@ class Replica:
     """The arguments of Simulation(...) that an ensemble replica keeps as
     its own (bodies, box, types, energies, dt, seed, skin)."""
 
-    def __init__(self, bodies, box, types, energies=(), dt=0.005, seed=1, skin=0.3):
+    def __init__(self, bodies, box, types, energies=(), dt=0.005, seed=1, skin=0.3, bonds=()):
         self.bodies, self.box, self.types, self.energies = bodies, box, tuple(types), tuple(energies)
-        self.dt, self.seed, self.skin = dt, seed, skin
+        self.dt, self.seed, self.skin, self.bonds = dt, seed, skin, list(bonds)
 
 
 class Ensemble:
@@ -515,6 +558,8 @@ class Ensemble:
             _check(self.engine, self.engine.ensemble_member(self._h, r, ctypes.byref(mh)))
             sim._h = mh
             sim._owner = self
+            if len(replicas[r].bonds):
+                sim.set_bonds(replicas[r].bonds)
         if tune:
             self.autotune(tune_steps)
 
@@ -558,6 +603,28 @@ class Ensemble:
         lc = _abi.FccLaunch(int(v.get("update_block", 256)), int(v.get("xcd_order", 1)),
                             int(v.get("chunk", 0)), 0)
         _check(self.engine, self.engine.ensemble_set_launch(self._h, ctypes.byref(lc)))
+
+    def observer_start(self, nbins=256, prange=(3.5, 5.5), max_samples=1024):
+        """`observer_start` on every replica."""
+        c = _observer_config(nbins, prange, max_samples)
+        _check(self.engine, self.engine.ensemble_observer_start(self._h, ctypes.byref(c)))
+        for sim in self.members:
+            sim._obs = c
+        return self
+
+    def observe(self):
+        """One device sample of every replica."""
+        _check(self.engine, self.engine.ensemble_observe(self._h))
+        return self
+
+    def run_observed(self, nsteps, every):
+        """`run` in runs of `every` steps, every replica sampled after each."""
+        _check(self.engine, self.engine.ensemble_run_observed(self._h, int(nsteps), int(every)))
+        return self
+
+    def observations(self):
+        """`observations` of every replica."""
+        return [observations(sim) for sim in self.members]
 
     def autotune(self, steps=20):
         """Time every candidate launch shape over `steps` steps from the same
@@ -660,6 +727,63 @@ def unwrapped_positions(sim):
 def body_energy(sim):
     """body_energy (src/observables.jl:47)."""
     return float(np.sum(sim.grad.Ebody))
+
+
+# ------------------------------------------------- device observations
+
+
+def _observer_config(nbins, prange, max_samples):
+    pmin, pmax = (float(v) for v in prange)
+    return _abi.FccObserverConfig(int(nbins), int(max_samples), pmin, pmax)
+
+
+def observer_start(sim, nbins=256, prange=(3.5, 5.5), max_samples=1024):
+    """Start a device-side observation of `sim` (fcc_observer_start): the
+    current unwrapped positions (src/observables.jl:24-29) become the origin
+    of the mean squared displacement; each later `observe` adds one sample of
+    the per-type mean squared displacement, mean shape index and mean squared
+    shape index (src/observables.jl:18) and bins every body's shape index
+    into a per-type histogram over `prange` with `nbins` bins (plus one bin
+    below and one at or above the range)."""
+    c = _observer_config(nbins, prange, max_samples)
+    _check(sim.engine, sim.engine.observer_start(sim._h, ctypes.byref(c)))
+    sim._obs = c
+    return sim
+
+
+def observe(sim):
+    """One device sample of the current state (no host transfer)."""
+    _check(sim.engine, sim.engine.observe(sim._h))
+    return sim
+
+
+def run_observed(sim, nsteps, every):
+    """run!(sim, nsteps; every, callback) (src/simulation.jl:220-235) with a
+    device sample as the callback."""
+    _check(sim.engine, sim.engine.run_observed(sim._h, int(nsteps), int(every)))
+    return sim
+
+
+def observations(sim):
+    """The samples so far: `clock` [samples] (device step clock), `msd`,
+    `p_mean`, `p2_mean` [samples x types], `hist` [types x (nbins + 2)] and
+    the bin `edges` [nbins + 1] of bins 1..nbins."""
+    c = getattr(sim, "_obs", None)
+    if c is None:
+        raise RuntimeError("no observation started (observer_start)")
+    nt, ms, nb = len(sim.types), c.max_samples, c.nbins
+    ns = ctypes.c_int32()
+    clock = np.zeros(ms, np.int64)
+    msd, pm, p2 = (np.zeros((ms, nt)) for _ in range(3))
+    hist = np.zeros((nt, nb + 2), np.uint64)
+    dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    _check(sim.engine, sim.engine.observer_read(
+        sim._h, ctypes.byref(ns), clock.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), dp(msd), dp(pm), dp(p2),
+        hist.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))))
+    n = ns.value
+    edges = c.pmin + (c.pmax - c.pmin) * np.arange(nb + 1) / nb
+    return SimpleNamespace(clock=clock[:n], msd=msd[:n], p_mean=pm[:n], p2_mean=p2[:n], hist=hist,
+                           edges=edges, nbins=nb, prange=(c.pmin, c.pmax))
 
 
 def philox_noise(n, seed, draw, precision="fp64", engine=None):

@@ -10,21 +10,23 @@
 #   run!(sim, 10_000)
 #   p = shape_index(sim)
 #
-# Terms or sizes without a native kernel (user BodyTerms, bond
-# terms, nθ > 64, more than 4 body types) make `fcc_create` return
+# Terms or sizes without a native kernel (user BodyTerms and BondTerms,
+# nθ > 64, more than 4 body types) make `fcc_create` return
 # FCC_ERR_UNSUPPORTED; the extension then builds the reference Simulation on
 # the CPU backend instead (a user-defined term runs on the reference path).
-# `HIPEnsemble` runs independent replicas by one launch per kernel and step.
+# `HIPEnsemble` runs independent replicas by one launch per kernel and step;
+# `observer_start!` / `run_observed!` / `observations` accumulate long-run
+# observables on the device.
 module FourierContourCellsHIPExt
 
 using FourierContourCells
 using FourierContourCells: Bodies, PeriodicBox, BodyType, EnergyTerm, OnTypes, Tension,
-    PerimeterElasticity, Overlap, LinearRepulsion, is_body_term, is_contact_term, is_bond_term, check_parents,
+    PerimeterElasticity, Overlap, LinearRepulsion, HarmonicBond, is_body_term, is_contact_term, is_bond_term, check_parents,
     nbodies, to_host
 import FourierContourCells: step!, run!, compute_gradients!, update_neighbors!, shape_index,
     perimeters, areas, positions, unwrapped_positions, body_energy
 using FCCHIP: HIPBackend, libfcc_hip, Bending
-import FCCHIP: HIPEnsemble, autotune!
+import FCCHIP: HIPEnsemble, autotune!, observer_start!, observe!, run_observed!, observations
 
 const FCC_OK = Cint(0)
 const FCC_ERR_UNSUPPORTED = Cint(-3)
@@ -83,6 +85,9 @@ function fcc_term(t)
         return FccTerm(16, length(on), ont, (0.0, 0.0), table(Float64.(base.J), nt), table(Float64.(base.Jrelated), nt))
     elseif base isa LinearRepulsion
         return FccTerm(17, length(on), ont, (Float64(base.k), 0.0), ntuple(_ -> 0.0, 16), ntuple(_ -> 0.0, 16))
+    elseif base isa HarmonicBond
+        return FccTerm(32, length(on), ont, (Float64(base.k), Float64(base.r0)), ntuple(_ -> 0.0, 16),
+            ntuple(_ -> 0.0, 16))
     end
     return nothing  # no native kernel
 end
@@ -97,6 +102,7 @@ mutable struct HIPSimulation{T}
     box::PeriodicBox{Float64}
     nθ::Int
     dt::Float64
+    ntypes::Int
 end
 
 """
@@ -119,7 +125,7 @@ function FourierContourCells.Simulation(bodies::Bodies, box::PeriodicBox, ::HIPB
         (getfield(host, f) for f in fieldnames(Bodies)))...)
     check_parents(host, types)
     terms = map(fcc_term, energies)
-    native = isempty(bonds) && all(!isnothing, terms) && length(types) <= 4 && length(terms) <= 8
+    native = all(!isnothing, terms) && length(types) <= 4 && length(terms) <= 8
     if !native
         @warn "no native kernel for this configuration: using the reference path" maxlog = 1
         return FourierContourCells.Simulation(bodies, box; types, energies, bonds, dt, nθ,
@@ -138,11 +144,12 @@ function FourierContourCells.Simulation(bodies::Bodies, box::PeriodicBox, ::HIPB
     end
     if rc == FCC_ERR_UNSUPPORTED
         @warn "libfcc_hip: $(lasterror()); using the reference path" maxlog = 1
-        return FourierContourCells.Simulation(bodies, box; types, energies, dt, nθ, seed, skin)
+        return FourierContourCells.Simulation(bodies, box; types, energies, bonds, dt, nθ, seed, skin)
     end
     check(rc)
-    sim = HIPSimulation{T}(h[], host, PeriodicBox{Float64}(box.Lx, box.Ly), nθ, dt)
+    sim = HIPSimulation{T}(h[], host, PeriodicBox{Float64}(box.Lx, box.Ly), nθ, dt, length(types))
     finalizer(s -> ccall((:fcc_destroy, libfcc_hip), Cint, (Ptr{Cvoid},), s.handle), sim)
+    set_bonds!(sim, bonds)
     return sim
 end
 
@@ -218,6 +225,52 @@ function unwrapped_positions(sim::HIPSimulation)
 end
 body_energy(sim::HIPSimulation) = sum(compute_gradients!(sim).Ebody)
 
+# `bonds` of Simulation (src/simulation.jl:14-15): pairs (i, j), 1-based,
+# checked and built into the bond list like BondList (src/energies/bonds.jl:54-72)
+function set_bonds!(sim::HIPSimulation, bonds)
+    pairs = Int32[v for b in bonds for v in (b[1], b[2])]
+    check(ccall((:fcc_set_bonds, libfcc_hip), Cint, (Ptr{Cvoid}, Int64, Ptr{Int32}), sim.handle,
+        length(pairs) ÷ 2, pairs))
+    return sim
+end
+
+# ---- long-run observables on the device: run!(sim, n; every, callback) with a
+# callback evaluating shape_index / unwrapped_positions (src/observables.jl:18-29)
+
+struct FccObserverConfig
+    nbins::Int32; max_samples::Int32; pmin::Cdouble; pmax::Cdouble
+end
+
+const OBSERVERS = IdDict{Any, FccObserverConfig}()
+
+function observer_start!(sim::HIPSimulation; nbins::Integer = 256, prange = (3.5, 5.5), max_samples::Integer = 1024)
+    c = FccObserverConfig(nbins, max_samples, prange[1], prange[2])
+    check(ccall((:fcc_observer_start, libfcc_hip), Cint, (Ptr{Cvoid}, Ref{FccObserverConfig}), sim.handle, c))
+    OBSERVERS[sim] = c
+    return sim
+end
+
+observe!(sim::HIPSimulation) = (check(ccall((:fcc_observe, libfcc_hip), Cint, (Ptr{Cvoid},), sim.handle)); sim)
+
+run_observed!(sim::HIPSimulation, nsteps::Integer; every::Integer) =
+    (check(ccall((:fcc_run_observed, libfcc_hip), Cint, (Ptr{Cvoid}, Int64, Int64), sim.handle, nsteps, every)); sim)
+
+"Samples so far: clock, msd / p_mean / p2_mean (types x samples), hist ((nbins + 2) x types), edges."
+function observations(sim::HIPSimulation)
+    c = OBSERVERS[sim]
+    nt, ms = sim.ntypes, Int(c.max_samples)
+    ns = Ref{Int32}(0)
+    clock = zeros(Int64, ms)
+    msd, pm, p2 = zeros(nt, ms), zeros(nt, ms), zeros(nt, ms)
+    hist = zeros(UInt64, c.nbins + 2, nt)
+    check(ccall((:fcc_observer_read, libfcc_hip), Cint,
+        (Ptr{Cvoid}, Ref{Int32}, Ptr{Int64}, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{UInt64}),
+        sim.handle, ns, clock, msd, pm, p2, hist))
+    n = Int(ns[])
+    edges = c.pmin .+ (c.pmax - c.pmin) .* (0:c.nbins) ./ c.nbins
+    return (; clock = clock[1:n], msd = msd[:, 1:n], p_mean = pm[:, 1:n], p2_mean = p2[:, 1:n], hist, edges)
+end
+
 # ---- ensembles: independent simulations advanced by one launch per kernel
 # and step (README.md:249-261 runs them as concurrent tasks instead)
 
@@ -234,7 +287,7 @@ end
     HIPEnsemble(specs; T = Float32, nθ = 40)
 
 `specs` is a vector of named tuples `(bodies, box, types, energies, dt = 0.005,
-seed = 1, skin = 0.3)`, one per replica. Replicas share `T`, `nθ` and the mode
+seed = 1, skin = 0.3, bonds = ())`, one per replica. Replicas share `T`, `nθ` and the mode
 count. `members` are `HIPSimulation`s (borrowed handles).
 """
 function HIPEnsemble(specs::AbstractVector; T = Float32, nθ::Integer = 40)
@@ -268,7 +321,8 @@ function HIPEnsemble(specs::AbstractVector; T = Float32, nθ::Integer = 40)
         check(ccall((:fcc_ensemble_member, libfcc_hip), Cint, (Ptr{Cvoid}, Int32, Ref{Ptr{Cvoid}}),
             h[], r - 1, m))
         push!(members, HIPSimulation{T}(m[], host, PeriodicBox{Float64}(sp.box.Lx, sp.box.Ly), nθ,
-            get(sp, :dt, 0.005)))
+            get(sp, :dt, 0.005), length(sp.types)))
+        set_bonds!(members[end], get(sp, :bonds, ()))
     end
     ens = HIPEnsembleHandle(h[], members)
     finalizer(e -> ccall((:fcc_ensemble_destroy, libfcc_hip), Cint, (Ptr{Cvoid},), e.handle), ens)
@@ -278,6 +332,21 @@ end
 # run!(sim, n) of every replica
 run!(e::HIPEnsembleHandle, nsteps::Integer) =
     (check(ccall((:fcc_ensemble_run, libfcc_hip), Cint, (Ptr{Cvoid}, Int64), e.handle, nsteps)); e)
+
+function observer_start!(e::HIPEnsembleHandle; nbins::Integer = 256, prange = (3.5, 5.5),
+        max_samples::Integer = 1024)
+    c = FccObserverConfig(nbins, max_samples, prange[1], prange[2])
+    check(ccall((:fcc_ensemble_observer_start, libfcc_hip), Cint, (Ptr{Cvoid}, Ref{FccObserverConfig}),
+        e.handle, c))
+    foreach(m -> OBSERVERS[m] = c, e.members)
+    return e
+end
+
+observe!(e::HIPEnsembleHandle) = (check(ccall((:fcc_ensemble_observe, libfcc_hip), Cint, (Ptr{Cvoid},), e.handle)); e)
+
+run_observed!(e::HIPEnsembleHandle, nsteps::Integer; every::Integer) =
+    (check(ccall((:fcc_ensemble_run_observed, libfcc_hip), Cint, (Ptr{Cvoid}, Int64, Int64), e.handle, nsteps,
+        every)); e)
 
 function autotune!(e::HIPEnsembleHandle; steps::Integer = 20)
     best = Ref(FccLaunch(0, 0, 0, 0))

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define FCC_ABI_VERSION 2
+#define FCC_ABI_VERSION 3
 
 #define FCC_MAX_TYPES 4        /* body types per simulation */
 #define FCC_MAX_TERMS 8        /* energy terms per simulation */
@@ -60,7 +60,9 @@ enum fcc_term_kind {
     FCC_TERM_PERIMETER_ELASTICITY = 2,  /* p[0] = lambda, p[1] = p0 */
     FCC_TERM_BENDING = 3,               /* p[0] = kappa */
     FCC_TERM_OVERLAP = 16,              /* J, Jrel tables */
-    FCC_TERM_LINEAR_REPULSION = 17      /* p[0] = k (oracle only) */
+    FCC_TERM_LINEAR_REPULSION = 17,     /* p[0] = k */
+    FCC_TERM_HARMONIC_BOND = 32         /* p[0] = k, p[1] = r0 (src/energies/bonds.jl:12-29);
+                                           acts on the pairs of fcc_set_bonds */
 };
 
 /* Simulation flags. */
@@ -133,6 +135,13 @@ int fcc_set_bodies(fcc_sim *sim, const fcc_bodies *bodies);
 int fcc_get_bodies(fcc_sim *sim, fcc_bodies *bodies);
 int fcc_set_dt(fcc_sim *sim, double dt);
 
+/* The `bonds` argument of Simulation (src/simulation.jl:14-15,96): n pairs
+ * (i, j) of 1-based body ids, pairs[2q], pairs[2q + 1], on which the bond
+ * terms act; built into the per-body partner list of BondList
+ * (src/energies/bonds.jl:47-76) with its checks (FCC_ERR_ARGUMENT for a body
+ * outside 1:N, a self bond or a duplicate). n = 0 removes all bonds. */
+int fcc_set_bonds(fcc_sim *sim, int64_t n, const int32_t *pairs);
+
 /* step!(sim, n) (src/simulation.jl:165-173): eager steps, one host
  * synchronisation per step for the skin check. */
 int fcc_step(fcc_sim *sim, int64_t n);
@@ -197,6 +206,71 @@ int fcc_philox_noise(uint64_t seed, uint64_t draw, int64_t n, int32_t precision,
 int fcc_philox4x32(const uint32_t *ctr, const uint32_t *key, int64_t n,
                    uint32_t *out);
 
+/* ---- checkpoint and resume
+ *
+ * The reference has no checkpoint; a resumed Simulation built from saved
+ * bodies restarts its device clock (the Philox draw index,
+ * src/simulation.jl:38-64,99) and its Verlet reference
+ * (src/neighbors.jl:110-119), so the continued run differs from the
+ * uninterrupted one. A checkpoint here holds the body state, the lab frame,
+ * the Verlet reference, the neighbour list (whose order fixes the contact
+ * summation order), the device clock, the rebuild flag and the step, time
+ * and build counters: loading it into a simulation created with the same
+ * configuration, body types and parents continues bitwise as the saved one
+ * would have. The blob is this library's format (native byte order). */
+
+/* Write the checkpoint into buf (synchronises). *needed receives its size;
+ * buf = NULL only queries it. FCC_ERR_DIMENSION if bytes is too small. */
+int fcc_checkpoint_save(fcc_sim *sim, void *buf, int64_t bytes, int64_t *needed);
+
+/* Restore a checkpoint. FCC_ERR_DIMENSION for another precision, size or
+ * truncated data; FCC_ERR_ARGUMENT for another dt, box, skin, seed, other
+ * body types or parents, or data that is not a checkpoint. */
+int fcc_checkpoint_load(fcc_sim *sim, const void *buf, int64_t bytes);
+
+/* ---- long-run observables on the device
+ *
+ * The reference evaluates shape_index and unwrapped_positions on the host
+ * from full copies of the state (src/observables.jl:6-29), typically in a
+ * run!(sim, n; every, callback) loop (src/simulation.jl:220-235). Here a
+ * sample is enqueued on the simulation's stream without a host transfer and
+ * accumulates, per body type t (1-based t = index + 1):
+ *   - the mean squared displacement of the unwrapped centres from their
+ *     positions at fcc_observer_start, mean over the bodies of type t;
+ *   - the mean shape index p = perimeter / sqrt(area) and the mean of p^2;
+ *   - a histogram of p pooled over all samples: bin 0 holds p < pmin, bin
+ *     b in 1..nbins holds 1 + floor((p - pmin) * nbins / (pmax - pmin)) == b,
+ *     bin nbins + 1 holds p >= pmax (and NaN).
+ * p is the value fcc_shape_index returns, computed in the simulation's
+ * precision; sums are FP64 and reduced in a fixed order. */
+#define FCC_OBS_MAX_BINS 2046  /* FCC_MAX_TYPES x (nbins + 2) counters in 32 KiB of LDS */
+
+typedef struct fcc_observer_config {
+    int32_t nbins;         /* shape-index bins, 1..FCC_OBS_MAX_BINS */
+    int32_t max_samples;   /* length of the sample series, >= 1 */
+    double pmin, pmax;     /* histogram range, pmin < pmax */
+} fcc_observer_config;
+
+/* Start (or restart) an observation: the current unwrapped positions become
+ * the origin of the displacements; the series and the histogram are cleared. */
+int fcc_observer_start(fcc_sim *sim, const fcc_observer_config *cfg);
+
+/* Take one sample of the current state (asynchronous). FCC_ERR_STATE without
+ * an observation or when max_samples samples have been taken. */
+int fcc_observe(fcc_sim *sim);
+
+/* run!(sim, n; every, callback) with a device sample as the callback: n
+ * steps in runs of `every` steps, one sample after each. */
+int fcc_run_observed(fcc_sim *sim, int64_t n, int64_t every);
+
+/* Copy the observation out (synchronises): n_samples; per sample the device
+ * clock (steps since creation), and msd, p_mean, p2_mean as [n_types x
+ * n_samples] column-major (type fastest); hist as [(nbins + 2) x n_types]
+ * column-major (bin fastest). Arrays are sized for max_samples; any pointer
+ * may be NULL. A type without bodies reads NaN. */
+int fcc_observer_read(fcc_sim *sim, int32_t *n_samples, int64_t *clock, double *msd,
+                      double *p_mean, double *p2_mean, uint64_t *hist);
+
 /* ---- ensembles of independent simulations
  *
  * Independent simulations advanced together, in place of one task and stream
@@ -239,6 +313,14 @@ int fcc_ensemble_set_launch(fcc_ensemble *ens, const fcc_launch *launch);
  * it with its device time per step in milliseconds. */
 int fcc_ensemble_autotune(fcc_ensemble *ens, int64_t steps, fcc_launch *best,
                           double *ms_per_step);
+
+/* Observations of every replica (fcc_observer_start / fcc_observe on each
+ * member; fcc_observer_read per member handle). */
+int fcc_ensemble_observer_start(fcc_ensemble *ens, const fcc_observer_config *cfg);
+int fcc_ensemble_observe(fcc_ensemble *ens);
+
+/* fcc_ensemble_run in runs of `every` steps, every replica sampled after each. */
+int fcc_ensemble_run_observed(fcc_ensemble *ens, int64_t n, int64_t every);
 
 #ifdef __cplusplus
 }

@@ -68,8 +68,12 @@ struct fcco_sim {
     int32_t ccap;
     int32_t *cidx, *ccount;
     /* terms and types */
-    int nbody, ncontact;
-    fcc_term body[FCC_MAX_TERMS], contact[FCC_MAX_TERMS];
+    int nbody, ncontact, nbondt;
+    fcc_term body[FCC_MAX_TERMS], contact[FCC_MAX_TERMS], bondt[FCC_MAX_TERMS];
+    /* bond list (BondList, src/energies/bonds.jl:47-76): partners of body i in
+     * bidx[q + bcap * i], q < bcount[i] */
+    int32_t bcap;
+    int32_t *bidx, *bcount;
     double range;
     fcc_body_type types[FCC_MAX_TYPES];
     uint32_t key[2];
@@ -807,9 +811,119 @@ static void euler_maruyama(fcco_sim *s, int frame, int geom, int skin, int clock
     if (clock) s->clock += 1;
 }
 
+/* ------------------------------------------------------------------ bonds */
+
+/* bond_gradient(::HarmonicBond, Δ, ti, tj), src/energies/bonds.jl:25-29:
+ * ∂E/∂Δ = k (1 - r0/r) Δ; at r = 0, k Δ (r0 = 0) or zero (r0 > 0). */
+static void harmonic_bond_gradient(const fcc_term *t, double dx, double dy, double g[2]) {
+    const double r = sqrt(dx * dx + dy * dy);
+    const double k = t->p[0], r0 = t->p[1];
+    const double f = r > 0.0 ? k * (1.0 - r0 / r) : (r0 == 0.0 ? k : 0.0);
+    g[0] = f * dx;
+    g[1] = f * dy;
+}
+
+/* bond_kernel!, src/energies/bonds.jl:98-110, with the right fold over the
+ * terms of _bond_gradient (:80-85): gx, gy += sum over partners j of
+ * ∂E/∂Δ at Δ = minimum image of r_i - r_j. */
+static void bond_gradients(fcco_sim *s) {
+    if (s->nbondt == 0 || !s->bcount) return;
+    const int64_t N = s->N;
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (int64_t i = 0; i < N; i++) {
+        double gi0 = 0.0, gi1 = 0.0;
+        for (int32_t q = 0; q < s->bcount[i]; q++) {
+            const int32_t j = s->bidx[q + (int64_t)s->bcap * i];
+            const double dx = fcco_minimum_image(s->x[i] - s->x[j], s->Lx);
+            const double dy = fcco_minimum_image(s->y[i] - s->y[j], s->Ly);
+            double a0 = 0.0, a1 = 0.0;
+            for (int t = s->nbondt - 1; t >= 0; t--) {
+                double g[2] = {0.0, 0.0};
+                if (applies(&s->bondt[t], s->type[i]) && applies(&s->bondt[t], s->type[j]))
+                    harmonic_bond_gradient(&s->bondt[t], dx, dy, g);
+                a0 = g[0] + a0;
+                a1 = g[1] + a1;
+            }
+            gi0 += a0;
+            gi1 += a1;
+        }
+        s->gx[i] += gi0;
+        s->gy[i] += gi1;
+    }
+}
+
+static int cmp_i64(const void *a, const void *b) {
+    const int64_t x = *(const int64_t *)a, y = *(const int64_t *)b;
+    return (x > y) - (x < y);
+}
+
+/* BondList(bonds, N), src/energies/bonds.jl:54-72: checks, then every bond
+ * (i, j) appended to the columns of i and of j in the order given. */
+int fcco_set_bonds(fcco_sim *s, int64_t n, const int32_t *pairs) {
+    const int64_t N = s->N;
+    if (n < 0 || (n > 0 && !pairs)) return fail(FCC_ERR_ARGUMENT, "bad bond arguments");
+    int32_t *count = calloc((size_t)N, sizeof(int32_t));
+    if (!count) return fail(FCC_ERR_ARGUMENT, "out of memory");
+    for (int64_t q = 0; q < n; q++) {
+        const int32_t i = pairs[2 * q], j = pairs[2 * q + 1];
+        if (i < 1 || i > N || j < 1 || j > N) {
+            free(count);
+            return fail(FCC_ERR_ARGUMENT, "bond (%d, %d) refers to a body outside 1:%lld", i, j, (long long)N);
+        }
+        if (i == j) {
+            free(count);
+            return fail(FCC_ERR_ARGUMENT, "bond (%d, %d) connects a body to itself", i, j);
+        }
+        count[i - 1]++;
+        count[j - 1]++;
+    }
+    /* duplicates: equal (min, max) keys, found by sorting */
+    int64_t *key = malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+    if (!key) {
+        free(count);
+        return fail(FCC_ERR_ARGUMENT, "out of memory");
+    }
+    for (int64_t q = 0; q < n; q++) {
+        const int64_t a = pairs[2 * q], b = pairs[2 * q + 1];
+        key[q] = a < b ? a * (N + 1) + b : b * (N + 1) + a;
+    }
+    qsort(key, (size_t)n, sizeof(int64_t), cmp_i64);
+    for (int64_t q = 1; q < n; q++)
+        if (key[q] == key[q - 1]) {
+            const long long a = key[q] / (N + 1), b = key[q] % (N + 1);
+            free(key);
+            free(count);
+            return fail(FCC_ERR_ARGUMENT, "duplicate bond (%lld, %lld)", a, b);
+        }
+    free(key);
+    int32_t cap = 1;
+    for (int64_t i = 0; i < N; i++)
+        if (count[i] > cap) cap = count[i];
+    int32_t *idx = calloc((size_t)cap * (size_t)N, sizeof(int32_t));
+    if (!idx) {
+        free(count);
+        return fail(FCC_ERR_ARGUMENT, "out of memory");
+    }
+    memset(count, 0, sizeof(int32_t) * (size_t)N);
+    for (int64_t q = 0; q < n; q++) {
+        const int32_t ab[2][2] = {{pairs[2 * q] - 1, pairs[2 * q + 1] - 1}, {pairs[2 * q + 1] - 1, pairs[2 * q] - 1}};
+        for (int e = 0; e < 2; e++) {
+            const int32_t a = ab[e][0], b = ab[e][1];
+            idx[count[a] + (int64_t)cap * a] = b;
+            count[a]++;
+        }
+    }
+    free(s->bidx);
+    free(s->bcount);
+    s->bidx = idx;
+    s->bcount = count;
+    s->bcap = cap;
+    return FCC_OK;
+}
+
 /* --------------------------------------------------------------- stepping */
 
-/* compute_gradients!, src/simulation.jl:150-158 (no bond terms). */
+/* compute_gradients!, src/simulation.jl:150-158. */
 int fcco_compute_gradients(fcco_sim *s, int32_t frame) {
     if (s->nbody > 0) {
         body_terms(s);
@@ -828,6 +942,7 @@ int fcco_compute_gradients(fcco_sim *s, int32_t frame) {
         if (!s->valid) return fail(FCC_ERR_STATE, "neighbour list not built");
         contact_gradients(s);
     }
+    bond_gradients(s);
     pullback(s, frame);
     return FCC_OK;
 }
@@ -888,7 +1003,7 @@ int fcco_destroy(fcco_sim *s) {
                     s->type, s->parent, s->lab, s->labR0, s->R, s->dR, s->ddR, s->rmax, s->GR,
                     s->GD, s->GDD, s->P, s->gx, s->gy, s->gth, s->glab, s->gcoef, s->Ebody,
                     s->cellid, s->perm, s->start, s->stop, s->scratch, s->nidx, s->ncount, s->x0,
-                    s->y0, s->rmax0, s->cidx, s->ccount, s->disp};
+                    s->y0, s->rmax0, s->cidx, s->ccount, s->disp, s->bidx, s->bcount};
     for (size_t q = 0; q < sizeof ptrs / sizeof ptrs[0]; q++) free(ptrs[q]);
     free(s);
     return FCC_OK;
@@ -938,7 +1053,7 @@ static int validate(const fcc_config *cfg, const fcc_bodies *b) {
     for (int q = 0; q < cfg->n_terms; q++) {
         int k = cfg->terms[q].kind;
         if (k != FCC_TERM_TENSION && k != FCC_TERM_PERIMETER_ELASTICITY && k != FCC_TERM_BENDING &&
-            k != FCC_TERM_OVERLAP && k != FCC_TERM_LINEAR_REPULSION)
+            k != FCC_TERM_OVERLAP && k != FCC_TERM_LINEAR_REPULSION && k != FCC_TERM_HARMONIC_BOND)
             return fail(FCC_ERR_ARGUMENT, "energy terms must be BodyTerms, ContactTerms or BondTerms");
     }
     return FCC_OK;
@@ -966,6 +1081,8 @@ int fcco_create(const fcc_config *cfg, const fcc_bodies *b, fcco_sim **out) {
         const fcc_term *t = &cfg->terms[q];
         if (t->kind == FCC_TERM_OVERLAP || t->kind == FCC_TERM_LINEAR_REPULSION)
             s->contact[s->ncontact++] = *t;
+        else if (t->kind == FCC_TERM_HARMONIC_BOND)
+            s->bondt[s->nbondt++] = *t;
         else
             s->body[s->nbody++] = *t;
     }

@@ -29,6 +29,7 @@ int fcco_destroy(fcco_sim *s);
 int fcco_set_bodies(fcco_sim *s, const fcc_bodies *b);
 int fcco_get_bodies(fcco_sim *s, fcc_bodies *b);
 int fcco_set_dt(fcco_sim *s, double dt);
+int fcco_set_bonds(fcco_sim *s, int64_t n, const int32_t *pairs);
 int fcco_step(fcco_sim *s, int64_t n);
 int fcco_update_neighbors(fcco_sim *s, int32_t force);
 int fcco_compute_gradients(fcco_sim *s, int32_t frame);
