@@ -164,6 +164,39 @@ def kernel_durations(sim, reps=50):
     return f.value, u.value
 
 
+def binning_durations(sim, reps=20):
+    """Cell-list and neighbour-list build of the current state, `reps` times
+    (fcc_time_rebuild): average milliseconds of the stages cell index, scan,
+    scatter, cell sort, neighbours, from HIP events between them."""
+    out = (ctypes.c_double * 5)()
+    rc = sim.engine.time_rebuild(sim._h, int(reps), out)
+    if rc != 0:
+        raise RuntimeError(sim.engine.last_error())
+    return list(out)
+
+
+def binning_roofline(sim, stage_ms, builds, steps):
+    """Algorithmic HBM bytes per stage of one list build (N bodies, C cells,
+    S = total neighbour entries) and the achieved GB/s: cell index 4C + 28N
+    (count clear; x, y read, cell id write, count atomic), scan 20C (two
+    passes), scatter 16N, cell sort 8C + 8N, neighbours 28N + 4S + 8C (own x,
+    y, rmax, perm; list and counts written; cell starts and counts)."""
+    N = sim.N
+    cells = sim.cells
+    C = cells.ncx * cells.ncy
+    S = int(sim.neighbors.count.sum())
+    names = ["cell_index", "scan", "scatter", "cell_sort", "neighbors"]
+    nbytes = [4 * C + 28 * N, 20 * C, 16 * N, 8 * C + 8 * N, 28 * N + 4 * S + 8 * C]
+    stages = {}
+    for n, ms, b in zip(names, stage_ms, nbytes):
+        stages[n] = {"ms": ms, "bytes": b, "achieved": b / (ms * 1e-3) / 1e9, "unit": "GB/s",
+                     "frac": b / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS}
+    total = sum(stage_ms)
+    return {"bound": "hbm", "peak": PEAK_HBM_GBS, "cells": C, "list_entries": S, "stages": stages,
+            "ms_per_build": total, "builds_per_step": builds / steps,
+            "ms_per_step_amortised": total * builds / steps}
+
+
 def calibrate_copy(nbytes=1 << 30, reps=5):
     """A plain device copy of known size (torch), to scale the FETCH_SIZE /
     WRITE_SIZE counters of a rocprofv3 --pmc run of this same mode."""
@@ -373,6 +406,9 @@ def main():
     tested1, inr1 = contact_pairs(sim)
     tested, inr = (tested0 + tested1) / 2, (inr0 + inr1) / 2
     c = sim.counters()
+    binning = binning_roofline(sim, binning_durations(sim), c.builds, args.warmup + args.steps)
+    print("[bench] list build " + ", ".join(f"{k} {v['ms'] * 1e3:.1f} us" for k, v in binning["stages"].items())
+          + f" = {binning['ms_per_build'] * 1e3:.1f} us per build", file=sys.stderr, flush=True)
     value = sim.N * args.steps / sec
     print(f"[bench] {args.precision}: {value / 1e6:.1f} M body-steps/s ({sec / args.steps * 1e3:.4f} ms/step), "
           f"builds {c.builds}", file=sys.stderr, flush=True)
@@ -428,6 +464,7 @@ def main():
                             "achieved": upd_bytes / (ums * 1e-3) / 1e9, "peak": PEAK_HBM_GBS,
                             "unit": "GB/s", "frac": upd_bytes / (ums * 1e-3) / 1e9 / PEAK_HBM_GBS,
                             "launch_ms": ums, "launch_ms_in_run": run_ums},
+        "roofline_binning": binning,
         "cpu_baseline": cpu,
         "ensemble": ensemble,
     }

@@ -194,6 +194,10 @@ struct fcc_sim {
     virtual int time_step_kernels(int64_t, double *, double *) = 0;
     virtual int run_timed(int64_t, double *, double *) = 0;
     virtual int time_kernels(int64_t, double *, double *) = 0;
+    virtual int time_rebuild(int64_t, double *) = 0;
+    virtual int contours(double *, double *) = 0;
+    virtual int frame_request() = 0;
+    virtual int frame_read(double *, double *, int64_t *) = 0;
     virtual int checkpoint(void *, int64_t, int64_t *) = 0;
     virtual int restore(const void *, int64_t) = 0;
     virtual int observer_start(const fcc_observer_config *) = 0;
@@ -218,6 +222,8 @@ struct Sim final : fcc_sim {
     double wx = 0, wy = 0;
     int64_t nce = 1;
     int32_t *cellid = nullptr, *perm = nullptr, *cstart = nullptr, *ccnt = nullptr, *cfill = nullptr;
+    int32_t *ctile = nullptr;  // per-tile totals of the cell-count scan
+    unsigned ntile = 1;
     uint16_t *asyn = nullptr, *bpull = nullptr;  // MFMA operand tables (mixed modes)
     int mfma_T = 0;
     bool valid = false;
@@ -235,6 +241,11 @@ struct Sim final : fcc_sim {
 
     ~Sim() override {
         if (stream) (void)hipStreamSynchronize(stream);
+        if (fr.copy) (void)hipStreamSynchronize(fr.copy);
+        if (fr.ready) (void)hipEventDestroy(fr.ready);
+        if (fr.done) (void)hipEventDestroy(fr.done);
+        if (fr.hX) (void)hipHostFree(fr.hX);
+        if (fr.copy) (void)hipStreamDestroy(fr.copy);
         if (gexec) (void)hipGraphExecDestroy(gexec);
         if (gexec_n) (void)hipGraphExecDestroy(gexec_n);
         for (hipEvent_t e : tev)
@@ -504,8 +515,9 @@ struct Sim final : fcc_sim {
         nce = (int64_t)ncx * ncy;
         // node offsets |dxij + R c| stay below L/2 when L > 4 cutoff
         p.small_box = (c->Lx > 4.0 * cutoff && c->Ly > 4.0 * cutoff) ? 0 : 1;
+        ntile = blocks(nce, SCAN_TILE);
         if ((rc = alloc(&cellid, N)) || (rc = alloc(&perm, N)) || (rc = alloc(&cstart, nce)) ||
-            (rc = alloc(&ccnt, nce)) || (rc = alloc(&cfill, nce)))
+            (rc = alloc(&ccnt, nce)) || (rc = alloc(&cfill, nce)) || (rc = alloc(&ctile, ntile)))
             return rc;
         HIPCHK(hipMemsetAsync(cellid, 0, sizeof(int32_t) * N, stream));
         HIPCHK(hipMemsetAsync(perm, 0, sizeof(int32_t) * N, stream));
@@ -661,18 +673,27 @@ struct Sim final : fcc_sim {
     }
 
     // update_celllist! + update_neighbors!(nl) (src/neighbors.jl:81-92,223-242).
-    int rebuild() {
+    // `ev` (bench timing): six events recorded around the stages cell index,
+    // scan, scatter, cell sort and neighbours of the first pass.
+    int rebuild(hipEvent_t *ev = nullptr) {
         const unsigned gN = blocks(N, 256), gC = blocks(nce, 256);
+        if (ev) HIPCHK(hipEventRecord(ev[0], stream));
         hipLaunchKernelGGL(k_fill_i32, gC, 256, 0, stream, nce, ccnt, 0);
         hipLaunchKernelGGL(k_cell_index, gN, 256, 0, stream, N, d.x, d.y, ncx, ncy, wx, wy, cellid, ccnt);
-        hipLaunchKernelGGL(k_scan, 1, 1024, 0, stream, nce, ccnt, cstart, cfill);
+        if (ev) HIPCHK(hipEventRecord(ev[1], stream));
+        hipLaunchKernelGGL(k_scan_tiles, ntile, SCAN_TILE, 0, stream, nce, ccnt, cstart, ctile);
+        hipLaunchKernelGGL(k_scan_offsets, ntile, SCAN_TILE, 0, stream, nce, ctile, cstart, cfill);
+        if (ev) HIPCHK(hipEventRecord(ev[2], stream));
         hipLaunchKernelGGL(k_scatter, gN, 256, 0, stream, N, cellid, cfill, perm);
+        if (ev) HIPCHK(hipEventRecord(ev[3], stream));
         hipLaunchKernelGGL(k_cell_sort, gC, 256, 0, stream, nce, cstart, ccnt, perm);
         HIPCHK(hipGetLastError());
-        for (;;) {
+        for (bool first = true;; first = false) {
             hipLaunchKernelGGL(k_clear_status, 1, 1, 0, stream, d.st);
+            if (ev && first) HIPCHK(hipEventRecord(ev[4], stream));
             hipLaunchKernelGGL((k_neighbors<T>), gN, 256, 0, stream, N, d.x, d.y, d.rmax, perm, cstart, ccnt,
                                p.Lx, p.Ly, ncx, ncy, wx, wy, range + skin, d.cap, d.nidx, d.ncount, d.st);
+            if (ev && first) HIPCHK(hipEventRecord(ev[5], stream));
             HIPCHK(hipGetLastError());
             DevStatus st;
             HIPCHK(hipMemcpyAsync(&st, d.st, sizeof st, hipMemcpyDeviceToHost, stream));
@@ -709,7 +730,8 @@ struct Sim final : fcc_sim {
         const unsigned gN = blocks(N, 256), gC = blocks(nce, 256);
         hipLaunchKernelGGL(k_fill_i32, gC, 256, 0, sb, nce, ccnt, 0);
         hipLaunchKernelGGL(k_cell_index, gN, 256, 0, sb, N, d.x, d.y, ncx, ncy, wx, wy, cellid, ccnt);
-        hipLaunchKernelGGL(k_scan, 1, 1024, 0, sb, nce, ccnt, cstart, cfill);
+        hipLaunchKernelGGL(k_scan_tiles, ntile, SCAN_TILE, 0, sb, nce, ccnt, cstart, ctile);
+        hipLaunchKernelGGL(k_scan_offsets, ntile, SCAN_TILE, 0, sb, nce, ctile, cstart, cfill);
         hipLaunchKernelGGL(k_scatter, gN, 256, 0, sb, N, cellid, cfill, perm);
         hipLaunchKernelGGL(k_cell_sort, gC, 256, 0, sb, nce, cstart, ccnt, perm);
         hipLaunchKernelGGL(k_begin_build, 1, 1, 0, sb, d.st);
@@ -789,6 +811,30 @@ struct Sim final : fcc_sim {
 
     double acc_f = 0.0, acc_u = 0.0;
     int64_t acc_n = 0;
+
+    // `reps` forced list rebuilds of the current state, timed per stage.
+    int time_rebuild(int64_t reps, double *stage_ms) override {
+        hipEvent_t ev[6] = {};
+        for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+        double acc[5] = {0, 0, 0, 0, 0};
+        int rc = FCC_OK;
+        for (int64_t q = 0; q < reps && !rc; q++) {
+            if ((rc = rebuild(ev))) break;
+            for (int k = 0; k < 5; k++) {
+                float ms = 0.f;
+                hipError_t e = hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
+                if (e != hipSuccess) {
+                    rc = fail(FCC_ERR_DEVICE, "hipEventElapsedTime: %s", hipGetErrorString(e));
+                    break;
+                }
+                acc[k] += ms;
+            }
+        }
+        for (auto &e : ev) (void)hipEventDestroy(e);
+        if (rc) return rc;
+        for (int k = 0; k < 5; k++) stage_ms[k] = acc[k] / (double)reps;
+        return FCC_OK;
+    }
 
     int run_timed(int64_t n, double *fms, double *ums) override {
         want_timed = true;
@@ -1227,6 +1273,81 @@ struct Sim final : fcc_sim {
         return FCC_OK;
     }
 
+    // ---------------------------------------------------- contour frames
+    int launch_contours(double *X, double *Y) {
+        const unsigned g = blocks(N, 256);
+        switch (KM) {
+        case 5: hipLaunchKernelGGL((k_contours<T, 5>), g, 256, 0, stream, p, d, X, Y); break;
+        case 7: hipLaunchKernelGGL((k_contours<T, 7>), g, 256, 0, stream, p, d, X, Y); break;
+        default: hipLaunchKernelGGL((k_contours<T, 15>), g, 256, 0, stream, p, d, X, Y);
+        }
+        HIPCHK(hipGetLastError());
+        return FCC_OK;
+    }
+
+    int contours(double *X, double *Y) override {
+        if (!X || !Y) return fail(FCC_ERR_ARGUMENT, "null output arrays");
+        const size_t m = (size_t)N * nth;
+        double *dX, *dY;
+        int rc;
+        if ((rc = alloc(&dX, m)) || (rc = alloc(&dY, m))) return rc;
+        if (!(rc = launch_contours(dX, dY))) {
+            hipError_t e = hipMemcpyAsync(X, dX, sizeof(double) * m, hipMemcpyDeviceToHost, stream);
+            if (e == hipSuccess) e = hipMemcpyAsync(Y, dY, sizeof(double) * m, hipMemcpyDeviceToHost, stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(stream);
+            if (e != hipSuccess) rc = fail(FCC_ERR_DEVICE, "contour copy: %s", hipGetErrorString(e));
+        }
+        release(dX);
+        release(dY);
+        return rc;
+    }
+
+    // Frames for live views: the contour kernel runs in stream order into a
+    // device staging buffer; a copy stream moves it to pinned host memory
+    // while the simulation stream goes on with the next steps.
+    struct Frame {
+        double *dX = nullptr, *dY = nullptr;  // device staging (dX holds X, Y and the clock)
+        double *hX = nullptr;                 // pinned host copy, same layout
+        hipStream_t copy = nullptr;
+        hipEvent_t ready = nullptr, done = nullptr;
+        bool pending = false, used = false;
+    } fr;
+
+    int frame_request() override {
+        const size_t m = (size_t)N * nth;
+        if (!fr.copy) {
+            HIPCHK(hipStreamCreateWithFlags(&fr.copy, hipStreamNonBlocking));
+            HIPCHK(hipEventCreateWithFlags(&fr.ready, hipEventDisableTiming));
+            HIPCHK(hipEventCreateWithFlags(&fr.done, hipEventDisableTiming));
+            int rc;
+            if ((rc = alloc(&fr.dX, 2 * m + 1))) return rc;
+            fr.dY = fr.dX + m;
+            HIPCHK(hipHostMalloc((void **)&fr.hX, sizeof(double) * (2 * m + 1), hipHostMallocDefault));
+        }
+        // the staging buffer is rewritten only after the last frame's copy
+        if (fr.used) HIPCHK(hipStreamWaitEvent(stream, fr.done, 0));
+        int rc = launch_contours(fr.dX, fr.dY);
+        if (rc) return rc;
+        HIPCHK(hipMemcpyAsync(fr.dX + 2 * m, &d.st->clock, sizeof(int64_t), hipMemcpyDeviceToDevice, stream));
+        HIPCHK(hipEventRecord(fr.ready, stream));
+        HIPCHK(hipStreamWaitEvent(fr.copy, fr.ready, 0));
+        HIPCHK(hipMemcpyAsync(fr.hX, fr.dX, sizeof(double) * (2 * m + 1), hipMemcpyDeviceToHost, fr.copy));
+        HIPCHK(hipEventRecord(fr.done, fr.copy));
+        fr.pending = fr.used = true;
+        return FCC_OK;
+    }
+
+    int frame_read(double *X, double *Y, int64_t *clock) override {
+        if (!fr.pending) return fail(FCC_ERR_STATE, "no frame requested (fcc_frame_request)");
+        HIPCHK(hipEventSynchronize(fr.done));
+        const size_t m = (size_t)N * nth;
+        if (X) std::memcpy(X, fr.hX, sizeof(double) * m);
+        if (Y) std::memcpy(Y, fr.hX + m, sizeof(double) * m);
+        if (clock) std::memcpy(clock, fr.hX + 2 * m, sizeof(int64_t));
+        fr.pending = false;
+        return FCC_OK;
+    }
+
     // ---------------------------------------------------- checkpoints
     // Everything a continued run reads: the body state, the lab frame and
     // bounds of the last update, the Verlet reference and the neighbour list
@@ -1308,11 +1429,11 @@ struct Sim final : fcc_sim {
         std::memcpy(&h, buf, sizeof h);
         if (std::memcmp(h.magic, CKPT_MAGIC, 8) != 0 || h.version != 1)
             return fail(FCC_ERR_ARGUMENT, "not a checkpoint of this library");
-        if (h.precision != cfg.precision || h.N != N || h.ncoef != nc || h.ldc != ldc || h.nth != nth ||
-            h.ntypes != cfg.n_types)
+        if (h.precision != cfg.precision || h.N != N || h.ncoef != nc || h.ldc != ldc || h.nth != nth)
             return fail(FCC_ERR_DIMENSION, "checkpoint of another precision or size");
-        if (h.dt != cfg.dt || h.Lx != cfg.Lx || h.Ly != cfg.Ly || h.skin != cfg.skin || h.seed != cfg.seed)
-            return fail(FCC_ERR_ARGUMENT, "checkpoint of another configuration (dt, box, skin or seed)");
+        if (h.dt != cfg.dt || h.Lx != cfg.Lx || h.Ly != cfg.Ly || h.skin != cfg.skin || h.seed != cfg.seed ||
+            h.ntypes != cfg.n_types)
+            return fail(FCC_ERR_ARGUMENT, "checkpoint of another configuration (dt, box, skin, seed or types)");
         if (h.cap < 1 || bytes < (int64_t)ckpt_bytes(h.cap)) return fail(FCC_ERR_DIMENSION, "checkpoint truncated");
         const char *in = (const char *)buf;
         // body types and parents fix the children lists built at creation
@@ -1540,6 +1661,12 @@ int fcc_time_kernels(fcc_sim *s, int64_t reps, double *fms, double *ums) {
     return s->time_kernels(reps, fms, ums);
 }
 
+int fcc_time_rebuild(fcc_sim *s, int64_t reps, double *stage_ms) {
+    FCC_HANDLE(s);
+    if (reps < 1 || !stage_ms) return fail(FCC_ERR_ARGUMENT, "need reps >= 1 and an output array");
+    return s->time_rebuild(reps, stage_ms);
+}
+
 int fcc_run_timed(fcc_sim *s, int64_t n, double *fms, double *ums) {
     FCC_HANDLE(s);
     if (n < 0) return fail(FCC_ERR_ARGUMENT, "steps must be non-negative");
@@ -1561,6 +1688,12 @@ int fcc_observer_start(fcc_sim *s, const fcc_observer_config *c) {
     return rc ? rc : s->observer_start(c);
 }
 int fcc_observe(fcc_sim *s) { FCC_HANDLE(s); return s->observe(); }
+int fcc_contours(fcc_sim *s, double *X, double *Y) { FCC_HANDLE(s); return s->contours(X, Y); }
+int fcc_frame_request(fcc_sim *s) { FCC_HANDLE(s); return s->frame_request(); }
+int fcc_frame_read(fcc_sim *s, double *X, double *Y, int64_t *clock) {
+    FCC_HANDLE(s);
+    return s->frame_read(X, Y, clock);
+}
 int fcc_checkpoint_save(fcc_sim *s, void *buf, int64_t bytes, int64_t *needed) {
     FCC_HANDLE(s);
     if (bytes < 0) return fail(FCC_ERR_ARGUMENT, "negative size");

@@ -498,8 +498,8 @@ __device__ __forceinline__ void k_update_impl(const PP &p, const DP &d, int use_
     d.th[i] = thn;
     d.x[i] = xw;
     d.y[i] = yw;
-    d.imgx[i] += sx;
-    d.imgy[i] += sy;
+    if (sx != 0) d.imgx[i] += sx;  // image counters change only on a wrap (rare): no traffic otherwise
+    if (sy != 0) d.imgy[i] += sy;
     d.rmax[i] = rm;
     d.rmin[i] = rmn;
     if constexpr (EXACT) {
@@ -569,28 +569,59 @@ __global__ __launch_bounds__(256) void k_cell_index(int64_t N, const double *x, 
     atomicAdd(&count[c], 1);
 }
 
-// Exclusive scan of the cell counts (one workgroup of 1024): start[c].
-__global__ __launch_bounds__(1024) void k_scan(int64_t n, const int32_t *count, int32_t *start,
-                                               int32_t *fill) {
-    __shared__ int32_t part[1024];
-    const int t = threadIdx.x;
-    const int64_t per = (n + 1023) / 1024;
-    const int64_t a = t * per, b = a + per < n ? a + per : n;
-    int32_t s = 0;
-    for (int64_t c = a; c < b; c++) s += count[c];
-    part[t] = s;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        int32_t v = t >= off ? part[t - off] : 0;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
+// Exclusive scan of the cell counts, start[c] = sum of count[c'] for c' < c,
+// in two launches over tiles of SCAN_TILE cells: k_scan_tiles scans each
+// tile (coalesced loads, wave prefix sums, one LDS pass over the 16 wave
+// totals) and stores the tile total; k_scan_offsets adds the totals of the
+// tiles before it (a parallel reduction per tile) and writes start and fill.
+constexpr int SCAN_TILE = 1024;
+
+__device__ __forceinline__ int32_t wave_prefix_incl(int32_t v, int lane) {
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int32_t u = __shfl_up(v, off);
+        v += lane >= off ? u : 0;
     }
-    int32_t acc = part[t] - s;
-    for (int64_t c = a; c < b; c++) {
-        start[c] = acc;
-        fill[c] = acc;
-        acc += count[c];
+    return v;
+}
+
+__global__ __launch_bounds__(SCAN_TILE) void k_scan_tiles(int64_t n, const int32_t *count, int32_t *start,
+                                                          int32_t *tile_sum) {
+    __shared__ int32_t wsum[SCAN_TILE / 64];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int64_t c = (int64_t)blockIdx.x * SCAN_TILE + t;
+    const int32_t v = c < n ? count[c] : 0;
+    const int32_t incl = wave_prefix_incl(v, lane);
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    if (t < 64) {
+        const int32_t x = t < SCAN_TILE / 64 ? wsum[t] : 0;
+        const int32_t xi = wave_prefix_incl(x, t);
+        if (t < SCAN_TILE / 64) wsum[t] = xi - x;  // exclusive over the waves
+    }
+    __syncthreads();
+    if (c < n) start[c] = wsum[w] + incl - v;
+    if (t == SCAN_TILE - 1) tile_sum[blockIdx.x] = wsum[w] + incl;
+}
+
+__global__ __launch_bounds__(SCAN_TILE) void k_scan_offsets(int64_t n, const int32_t *tile_sum, int32_t *start,
+                                                            int32_t *fill) {
+    __shared__ int32_t wsum[SCAN_TILE / 64];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    int32_t a = 0;
+    for (int64_t b = t; b < (int64_t)blockIdx.x; b += SCAN_TILE) a += tile_sum[b];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) a += __shfl_xor(a, off);
+    if (lane == 0) wsum[w] = a;
+    __syncthreads();
+    int32_t off = 0;
+#pragma unroll
+    for (int q = 0; q < SCAN_TILE / 64; q++) off += wsum[q];
+    const int64_t c = (int64_t)blockIdx.x * SCAN_TILE + t;
+    if (c < n) {
+        const int32_t s = start[c] + off;
+        start[c] = s;
+        fill[c] = s;
     }
 }
 
@@ -640,6 +671,7 @@ __global__ __launch_bounds__(256) void k_neighbors(int64_t N, const double *x, c
     cx = cx > 0 ? cx : 0;  // bodies outside [0, L): clamp (no-op inside the box)
     cy = cy > 0 ? cy : 0;
     const int32_t rx = ncx >= 3 ? 1 : 0, ry = ncy >= 3 ? 1 : 0;
+    const double qLx = 0.25 * Lx, qLy = 0.25 * Ly;
     int32_t n = 0;
     for (int32_t oy = -ry; oy <= ry; oy++) {
         int32_t yy = (cy + oy) % ncy;
@@ -652,10 +684,13 @@ __global__ __launch_bounds__(256) void k_neighbors(int64_t N, const double *x, c
             for (int32_t q = s0; q < s1; q++) {
                 const int32_t j = perm[q];
                 if (j == i) continue;
+                // minimum_image (src/box.jl:14); below a quarter box the
+                // rounded quotient is +-0 and the result is dx itself, so
+                // the division is skipped there (same value, bit for bit)
                 double dx = xi - x[j];
-                dx = dx - Lx * rint(dx / Lx);
+                if (!(fabs(dx) < qLx)) dx = dx - Lx * rint(dx / Lx);
                 double dy = yi - y[j];
-                dy = dy - Ly * rint(dy / Ly);
+                if (!(fabs(dy) < qLy)) dy = dy - Ly * rint(dy / Ly);
                 const double lim = ri + (double)rmax[j];
                 if (dx * dx + dy * dy <= lim * lim) {
                     n++;

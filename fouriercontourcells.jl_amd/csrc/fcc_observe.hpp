@@ -138,4 +138,24 @@ __global__ __launch_bounds__(OBS_BLOCK) void k_observe_finish(const double *part
     }
 }
 
+// contours(sim) (src/observables.jl:36-44): contour nodes around the wrapped
+// centres, X = x + R cos(phi_k), Y = y + R sin(phi_k) (n_theta x N, column-major).
+template <typename T, int KM>
+__global__ __launch_bounds__(256) void k_contours(const Params<T> p, const Ptrs<T> d, double *X, double *Y) {
+    constexpr int NCM = 2 * KM + 1;
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= p.N) return;
+    const FCC_CONST T *S = as_const(d.S);
+    T l[NCM];
+    load_row<T, KM>(l, d.lab + i * p.ldc, p.K);
+    const T r0 = d.R0[i];
+    const double xi = d.x[i], yi = d.y[i];
+    for (int k = 0; k < p.nth; k++) {
+        T r, dr, ddr;
+        synth<T, KM>(S, p.nth, k, l, p.K, false, r0, r, dr, ddr);
+        X[i * p.nth + k] = xi + (double)r * (double)d.cs[2 * k];
+        Y[i * p.nth + k] = yi + (double)r * (double)d.cs[2 * k + 1];
+    }
+}
+
 }  // namespace fcc

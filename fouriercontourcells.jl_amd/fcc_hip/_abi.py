@@ -143,6 +143,7 @@ _SIGNATURES = {
     "get_contour_gradients": [_vp, _dp, _dp, _dp, _dp],
     "get_geometry": [_vp, _dp, _dp, _dp, _dp, _dp],
     "shape_index": [_vp, _dp, _dp, _dp],
+    "contours": [_vp, _dp, _dp],
     "get_celllist": [_vp, _ip, _ip, _ip, _ip, _ip, _ip],
     "get_neighbors": [_vp, _ip, _ip, _ip],
     "set_neighbor_capacity": [_vp, _i32],
@@ -159,6 +160,7 @@ _DEVICE_ONLY = {
     "time_step_kernels": [_vp, _i64, _dp, _dp],
     "run_timed": [_vp, _i64, _dp, _dp],
     "time_kernels": [_vp, _i64, _dp, _dp],
+    "time_rebuild": [_vp, _i64, _dp],
     "ensemble_create": [_i32, ctypes.POINTER(FccConfig), ctypes.POINTER(FccBodies), ctypes.POINTER(_vp)],
     "ensemble_destroy": [_vp],
     "ensemble_member": [_vp, _i32, ctypes.POINTER(_vp)],
@@ -167,6 +169,8 @@ _DEVICE_ONLY = {
     "ensemble_get_launch": [_vp, ctypes.POINTER(FccLaunch)],
     "ensemble_set_launch": [_vp, ctypes.POINTER(FccLaunch)],
     "ensemble_autotune": [_vp, _i64, ctypes.POINTER(FccLaunch), _dp],
+    "frame_request": [_vp],
+    "frame_read": [_vp, _dp, _dp, ctypes.POINTER(_i64)],
     "checkpoint_save": [_vp, _vp, _i64, ctypes.POINTER(_i64)],
     "checkpoint_load": [_vp, _vp, _i64],
     "observer_start": [_vp, ctypes.POINTER(FccObserverConfig)],

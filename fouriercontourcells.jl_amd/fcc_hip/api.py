@@ -724,6 +724,31 @@ def unwrapped_positions(sim):
     return np.vstack([b.x + b.imgx * sim.box.Lx, b.y + b.imgy * sim.box.Ly])
 
 
+def contours(sim):
+    """contours(sim) (src/observables.jl:36-44): host nθ x N matrices (X, Y)
+    of contour node coordinates around the wrapped centres."""
+    X, Y = (np.zeros((sim.ntheta, sim.N), order="F") for _ in range(2))
+    dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    _check(sim.engine, sim.engine.contours(sim._h, dp(X), dp(Y)))
+    return X, Y
+
+
+def frame_request(sim):
+    """Enqueue the contours of the current state and their copy to the host
+    (fcc_frame_request); steps issued afterwards overlap the copy."""
+    _check(sim.engine, sim.engine.frame_request(sim._h))
+    return sim
+
+
+def frame_read(sim):
+    """The frame of the last `frame_request`: (X, Y, device clock)."""
+    X, Y = (np.zeros((sim.ntheta, sim.N), order="F") for _ in range(2))
+    clock = ctypes.c_int64()
+    dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    _check(sim.engine, sim.engine.frame_read(sim._h, dp(X), dp(Y), ctypes.byref(clock)))
+    return X, Y, clock.value
+
+
 def body_energy(sim):
     """body_energy (src/observables.jl:47)."""
     return float(np.sum(sim.grad.Ebody))

@@ -24,9 +24,10 @@ using FourierContourCells: Bodies, PeriodicBox, BodyType, EnergyTerm, OnTypes, T
     PerimeterElasticity, Overlap, LinearRepulsion, HarmonicBond, is_body_term, is_contact_term, is_bond_term, check_parents,
     nbodies, to_host
 import FourierContourCells: step!, run!, compute_gradients!, update_neighbors!, shape_index,
-    perimeters, areas, positions, unwrapped_positions, body_energy
+    perimeters, areas, positions, unwrapped_positions, body_energy, contours
 using FCCHIP: HIPBackend, libfcc_hip, Bending
-import FCCHIP: HIPEnsemble, autotune!, observer_start!, observe!, run_observed!, observations
+import FCCHIP: HIPEnsemble, autotune!, observer_start!, observe!, run_observed!, observations,
+    frame_request!, frame_read, checkpoint, restore!
 
 const FCC_OK = Cint(0)
 const FCC_ERR_UNSUPPORTED = Cint(-3)
@@ -224,6 +225,38 @@ function unwrapped_positions(sim::HIPSimulation)
     return permutedims(hcat(b.x .+ b.imgx .* sim.box.Lx, b.y .+ b.imgy .* sim.box.Ly))
 end
 body_energy(sim::HIPSimulation) = sum(compute_gradients!(sim).Ebody)
+
+# src/observables.jl:36-44
+function contours(sim::HIPSimulation)
+    X, Y = zeros(sim.nθ, nbodies(sim)), zeros(sim.nθ, nbodies(sim))
+    check(ccall((:fcc_contours, libfcc_hip), Cint, (Ptr{Cvoid}, Ptr{Cdouble}, Ptr{Cdouble}), sim.handle, X, Y))
+    return X, Y
+end
+
+# frames for live views: the copy of frame k overlaps the steps issued after it
+frame_request!(sim::HIPSimulation) =
+    (check(ccall((:fcc_frame_request, libfcc_hip), Cint, (Ptr{Cvoid},), sim.handle)); sim)
+function frame_read(sim::HIPSimulation)
+    X, Y = zeros(sim.nθ, nbodies(sim)), zeros(sim.nθ, nbodies(sim))
+    clock = Ref{Int64}(0)
+    check(ccall((:fcc_frame_read, libfcc_hip), Cint, (Ptr{Cvoid}, Ptr{Cdouble}, Ptr{Cdouble}, Ref{Int64}),
+        sim.handle, X, Y, clock))
+    return X, Y, clock[]
+end
+
+# checkpoint / resume (bitwise continuation, with the device clock)
+function checkpoint(sim::HIPSimulation)
+    n = Ref{Int64}(0)
+    check(ccall((:fcc_checkpoint_save, libfcc_hip), Cint, (Ptr{Cvoid}, Ptr{UInt8}, Int64, Ref{Int64}),
+        sim.handle, C_NULL, 0, n))
+    blob = Vector{UInt8}(undef, n[])
+    check(ccall((:fcc_checkpoint_save, libfcc_hip), Cint, (Ptr{Cvoid}, Ptr{UInt8}, Int64, Ref{Int64}),
+        sim.handle, blob, length(blob), n))
+    return blob
+end
+restore!(sim::HIPSimulation, blob::Vector{UInt8}) =
+    (check(ccall((:fcc_checkpoint_load, libfcc_hip), Cint, (Ptr{Cvoid}, Ptr{UInt8}, Int64), sim.handle, blob,
+        length(blob))); sim)
 
 # `bonds` of Simulation (src/simulation.jl:14-15): pairs (i, j), 1-based,
 # checked and built into the bond list like BondList (src/energies/bonds.jl:54-72)

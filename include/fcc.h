@@ -206,6 +206,21 @@ int fcc_philox_noise(uint64_t seed, uint64_t draw, int64_t n, int32_t precision,
 int fcc_philox4x32(const uint32_t *ctr, const uint32_t *key, int64_t n,
                    uint32_t *out);
 
+/* contours(sim) (src/observables.jl:36-44): contour node coordinates around
+ * the wrapped centres, X, Y [n_theta x N] (synchronises). */
+int fcc_contours(fcc_sim *sim, double *X, double *Y);
+
+/* Frames for live views (the Makie producer of
+ * ext/FourierContourCellsMakieExt.jl:209-276 pulls geometry after every
+ * run!): fcc_frame_request enqueues the contours of the current state on the
+ * simulation's stream and their copy to pinned host memory on a copy stream,
+ * and returns; steps issued afterwards run while the copy is in flight.
+ * fcc_frame_read waits for that frame and copies it out (X, Y as
+ * fcc_contours, clock = the device clock of the frame). One frame is held at
+ * a time; FCC_ERR_STATE if none was requested. */
+int fcc_frame_request(fcc_sim *sim);
+int fcc_frame_read(fcc_sim *sim, double *X, double *Y, int64_t *clock);
+
 /* ---- checkpoint and resume
  *
  * The reference has no checkpoint; a resumed Simulation built from saved

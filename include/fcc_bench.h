@@ -35,6 +35,13 @@ int fcc_run_timed(fcc_sim *sim, int64_t n, double *forces_ms, double *update_ms)
  * simulation is left unchanged. */
 int fcc_time_kernels(fcc_sim *sim, int64_t reps, double *forces_ms, double *update_ms);
 
+/* `reps` forced rebuilds of the cell and neighbour lists of the current state
+ * (as fcc_update_neighbors(sim, 1)), with HIP events between the stages;
+ * stage_ms[5] receives the average duration in milliseconds of: cell index
+ * (count clear + cell ids and counts), scan (two launches), scatter, cell
+ * sort, neighbour list. Counts as `reps` list builds. */
+int fcc_time_rebuild(fcc_sim *sim, int64_t reps, double *stage_ms);
+
 /* fcc_ensemble_run with the same eager last step per chunk between HIP
  * events; average forces and update kernel durations of an ensemble step
  * (all replicas), in milliseconds. Results equal fcc_ensemble_run's. */

@@ -1259,6 +1259,19 @@ int fcco_shape_index(fcco_sim *s, double *per, double *area, double *p) {
     return FCC_OK;
 }
 
+/* contours(sim), src/observables.jl:36-44: X = x' .+ R .* cos.(φ),
+ * Y = y' .+ R .* sin.(φ) with φ = node_angle (src/basis.jl:73). */
+int fcco_contours(fcco_sim *s, double *X, double *Y) {
+    const int nth = s->nth;
+    for (int64_t i = 0; i < s->N; i++)
+        for (int k = 0; k < nth; k++) {
+            const double R = s->R[k + nth * i];
+            X[k + nth * i] = s->x[i] + R * s->cs[2 * k];
+            Y[k + nth * i] = s->y[i] + R * s->cs[2 * k + 1];
+        }
+    return FCC_OK;
+}
+
 int fcco_get_celllist(fcco_sim *s, int32_t *ncx, int32_t *ncy, int32_t *cellid, int32_t *perm,
                       int32_t *start, int32_t *stop) {
     if (ncx) *ncx = s->ncx;

@@ -41,6 +41,7 @@ int fcco_get_geometry(fcco_sim *s, double *lab, double *rmax, double *R,
                       double *dR, double *ddR);
 int fcco_shape_index(fcco_sim *s, double *perimeters, double *areas,
                      double *shape_index);
+int fcco_contours(fcco_sim *s, double *X, double *Y);
 int fcco_get_celllist(fcco_sim *s, int32_t *ncx, int32_t *ncy, int32_t *cellid,
                       int32_t *perm, int32_t *start, int32_t *stop);
 int fcco_get_neighbors(fcco_sim *s, int32_t *capacity, int32_t *count,

@@ -9,7 +9,10 @@ oracle by test_gpu_parity.py / test_gpu_precision.py and, for a circle, to
 Bar: histogram counts identical (the per-body p are the values
 fcc_shape_index returns, bins computed with the same FP64 expression);
 per-type means within 1e-12 relative (FP64 sums in another order); the
-device clock of each sample equal to the simulation's clock.
+device clock of each sample equal to the simulation's clock. Contours
+(src/observables.jl:36-44, fcc_contours) against the oracle within 1e-12
+(FP64) / 1e-5 (FP32 and mixed); asynchronous frames equal to the contours
+of the state they were requested at.
 """
 import sys
 
@@ -156,3 +159,43 @@ def test_ensemble_observations_equal_replicas_alone():
         assert len(oe.clock) == 4
         for f in ("clock", "msd", "p_mean", "p2_mean", "hist"):
             assert np.array_equal(getattr(oa, f), getattr(oe, f)), f
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "mixed_fp16"])
+def test_contours_match_oracle(hip, oracle_engine, precision):
+    # contours (src/observables.jl:36-44) of deformed bodies, device vs oracle
+    say("contours", precision)
+    b, box = F.hexagonal_monolayer(6, 6, packing=0.95, maxmode=6, seed=4)
+    rng = np.random.default_rng(4)
+    b.coef[1:, :] = 0.04 * rng.standard_normal((b.coef.shape[0] - 1, len(b)))
+    kw = dict(types=(F.BodyType(),), energies=(F.Tension(0.01),))
+    Xa, Ya = F.contours(F.Simulation(b, box, engine=oracle_engine, **kw))
+    Xg, Yg = F.contours(F.Simulation(b, box, precision=precision, **kw))
+    atol = 1e-12 if precision == "fp64" else 1e-5
+    np.testing.assert_allclose(Xg, Xa, rtol=0, atol=atol)
+    np.testing.assert_allclose(Yg, Ya, rtol=0, atol=atol)
+
+
+def test_frames_overlap_the_next_steps():
+    say("frames")
+    s = monolayer_sim("fp32")
+    with pytest.raises(RuntimeError):
+        F.frame_read(s)  # nothing requested
+    F.run(s, 30)
+    X0, Y0 = F.contours(s)
+    c0 = s.counters().clock
+    F.frame_request(s)
+    F.run(s, 50)  # issued while the frame's copy may still be in flight
+    X, Y, clock = F.frame_read(s)
+    assert clock == c0 and np.array_equal(X, X0) and np.array_equal(Y, Y0)
+    for _ in range(3):  # requests reuse the staging buffer once the last copy is done
+        F.frame_request(s)
+        F.run(s, 10)
+    X, Y, clock = F.frame_read(s)
+    assert clock == c0 + 50 + 20
+    with pytest.raises(RuntimeError):
+        F.frame_read(s)  # already read
+    F.frame_request(s)
+    X1, Y1 = F.contours(s)
+    X, Y, clock = F.frame_read(s)
+    assert clock == c0 + 80 and np.array_equal(X, X1) and np.array_equal(Y, Y1)

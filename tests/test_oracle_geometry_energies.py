@@ -138,3 +138,23 @@ def test_on_types_restricts_terms(oracle_engine):
         F.refresh(sim)
         assert (F.body_energy(sim) > 0) == active
         assert bool(np.any(sim.grad.gcoef != 0)) == active
+
+
+def test_contours_of_round_and_deformed_bodies(oracle_engine):
+    # contours (src/observables.jl:36-44): a round body of radius R0 has its
+    # nodes at (x + R0 cos phi_k, y + R0 sin phi_k); a deformed one at R(phi_k)
+    # of the direct series around its centre
+    coef = np.zeros((11, 2))
+    coef[0, 0] = 1.0
+    coef[1:6, 1] = [0.1, -0.05, 0.03, 0.0, 0.02]
+    coef[6:11, 1] = [0.02, 0.04, 0.0, -0.01, 0.01]
+    coef[0, 1] = math.sqrt(1 - np.sum(coef[1:, 1] ** 2) / 2)
+    b = F.Bodies([3.0, 11.0], [4.0, 12.5], [0.0, 0.7], coef, [1.5, 1.3])
+    sim = F.Simulation(b, F.PeriodicBox(20.0, 20.0), types=(F.BodyType(),), ntheta=40, engine=oracle_engine)
+    X, Y = F.contours(sim)
+    phi = 2 * math.pi * np.arange(40) / 40
+    np.testing.assert_allclose(X[:, 0], 3.0 + 1.5 * np.cos(phi), rtol=0, atol=1e-14)
+    np.testing.assert_allclose(Y[:, 0], 4.0 + 1.5 * np.sin(phi), rtol=0, atol=1e-14)
+    R = np.array([rdirect_body(coef[:, 1], 0.7, 1.3, p) for p in phi])
+    np.testing.assert_allclose(X[:, 1], 11.0 + R * np.cos(phi), rtol=0, atol=1e-13)
+    np.testing.assert_allclose(Y[:, 1], 12.5 + R * np.sin(phi), rtol=0, atol=1e-13)
