@@ -201,6 +201,7 @@ __device__ __forceinline__ void philox_pair(uint32_t k0, uint32_t k1, uint32_t i
 // minimum_image (src/box.jl:14), round ties-to-even.
 template <typename T>
 __device__ __forceinline__ T min_image(T d, T L) {
+#pragma clang fp contract(off)
     return d - L * rint(d / L);
 }
 
@@ -213,6 +214,7 @@ __device__ __forceinline__ double min_image_nb(double d, double L, double invL, 
 
 // wrap (src/box.jl:21-30).
 __device__ __forceinline__ double wrap(double x, double L, int32_t &shift) {
+#pragma clang fp contract(off)
     double s = floor(x / L);
     double xp = x - s * L;
     if (xp >= L) {

@@ -242,6 +242,7 @@ __global__ __launch_bounds__(64) void k_forces_ens(const Params<T> *ps, const Pt
 // sin θ) in FP32 (geometry.jl:271-273 recurrence).
 template <typename T, int KM>
 __device__ __forceinline__ void harmonics(T th, int K, T (&cn)[KM + 1], T (&sn)[KM + 1]) {
+#pragma clang fp contract(off)
     if constexpr (sizeof(T) == 8) {
 #pragma unroll
         for (int m = 1; m <= KM; m++)
@@ -265,6 +266,7 @@ __device__ __forceinline__ void harmonics(T th, int K, T (&cn)[KM + 1], T (&sn)[
 // enforce_shape! (src/dynamics.jl:174-194) on a register row, in U precision.
 template <typename U, typename T, int KM>
 __device__ __forceinline__ void enforce_shape(T (&c)[2 * KM + 1], int K) {
+#pragma clang fp contract(off)
     U sm = U(0);
 #pragma unroll
     for (int m = 1; m <= KM; m++)
@@ -294,6 +296,7 @@ __device__ __forceinline__ void enforce_shape(T (&c)[2 * KM + 1], int K) {
 template <typename T, int KM>
 __device__ __forceinline__ T lab_frame(const T (&c)[2 * KM + 1], T th, T r0, int K, T (&l)[2 * KM + 1],
                                        T &rmin) {
+#pragma clang fp contract(off)
     T cn[KM + 1], sn[KM + 1];
     harmonics<T, KM>(th, K, cn, sn);
     l[0] = c[0];
@@ -336,6 +339,7 @@ __global__ __launch_bounds__(256) void k_init(const Params<T> p, const Ptrs<T> d
 template <typename T, int KM>
 __device__ __forceinline__ void body_frame(const T (&gl)[2 * KM + 1], const T (&c)[2 * KM + 1], T th, T r0,
                                            int K, T (&gc)[2 * KM + 1]) {
+#pragma clang fp contract(off)
     T cn[KM + 1], sn[KM + 1];
     harmonics<T, KM>(th, K, cn, sn);
     const T ga0 = r0 * gl[0];
@@ -433,6 +437,130 @@ __global__ __launch_bounds__(256) void k_displacement_ens(const Params<T> *ps, c
     k_displacement_impl<T, 0>(p, d, tl.blk);
 }
 
+// Pieces of the Euler–Maruyama step. They and the helpers they call
+// (harmonics, body_frame, enforce_shape, lab_frame, wrap, min_image) are
+// compiled without FMA contraction, as the reference's arithmetic is (Julia
+// does not contract): the same operations give the same bits wherever they
+// are inlined.
+
+// Displacement of a body without parent drag (src/dynamics.jl:84-99):
+// self-propulsion along θ, mobility times the centre force, translational
+// noise from Philox block 0 of draw `draw` (src/dynamics.jl:258-277).
+template <typename T, typename PP>
+__device__ __forceinline__ void body_displacement(const PP &p, int64_t i, int ti, T th, T gx, T gy,
+                                                  uint64_t draw, T &d0, T &d1) {
+#pragma clang fp contract(off)
+    T s, co;
+    fsincos(th, &s, &co);
+    T x1 = T(0), x2 = T(0);
+    if (p.Dt[ti] != T(0)) philox_pair<T>(p.key0, p.key1, (uint32_t)(i + 1), draw, 0, x1, x2);
+    const T st = sqrt(T(2) * p.Dt[ti] * p.dt);
+    d0 = p.dt * (p.v0[ti] * co - p.mur[ti] * gx) + st * x1;
+    d1 = p.dt * (p.v0[ti] * s - p.mur[ti] * gy) + st * x2;
+}
+
+// Rotational noise increment (Philox block 1 of draw `draw`).
+template <typename T, typename PP>
+__device__ __forceinline__ T rotation_noise(const PP &p, int64_t i, int ti, uint64_t draw) {
+#pragma clang fp contract(off)
+    T x3 = T(0), x4;
+    if (p.Dr[ti] != T(0)) philox_pair<T>(p.key0, p.key1, (uint32_t)(i + 1), draw, 1, x3, x4);
+    return sqrt(T(2) * p.Dr[ti] * p.dt) * x3;
+}
+
+// Body-frame gradient, orientation and shape update with the shape
+// constraint (src/dynamics.jl:310-333, src/gradients.jl:347-362; the
+// constraint in FP64 when F64S). Returns the new θ; c holds the new
+// coefficients and gc the body-frame gradient.
+template <typename T, int KM, bool F64S, typename PP>
+__device__ __forceinline__ T shape_step(const PP &p, int ti, int K, const T (&gl)[2 * KM + 1], T (&c)[2 * KM + 1],
+                                        T th, T r0, T gth, T d2, T (&gc)[2 * KM + 1]) {
+#pragma clang fp contract(off)
+    body_frame<T, KM>(gl, c, th, r0, K, gc);
+    const T thn = th - p.dt * p.muth[ti] * gth + d2;
+    const T h = p.dt * p.muab[ti];
+#pragma unroll
+    for (int m = 1; m <= KM; m++) {
+        c[m] -= h * gc[m];
+        c[KM + m] -= h * gc[KM + m];
+    }
+    if constexpr (F64S)
+        enforce_shape<double, T, KM>(c, K);
+    else
+        enforce_shape<T, T, KM>(c, K);
+    return thn;
+}
+
+// Position wrap in FP64 (src/box.jl:21-30), the image counters, the lab
+// frame of the new state (src/geometry.jl:208-227) and the Verlet skin check
+// (src/neighbors.jl:409-416) of body i, given its new coefficients c and
+// orientation thn and its displacement (d0, d1).
+template <typename T, int KM, bool EXACT, typename PP, typename DP>
+__device__ __forceinline__ void place_body(const PP &p, const DP &d, int64_t i, int K, const T (&c)[2 * KM + 1],
+                                           T thn, T r0, T d0, T d1) {
+#pragma clang fp contract(off)
+    constexpr int NCM = 2 * KM + 1;
+    constexpr int W = row_width<NCM, T>();
+    T l[NCM];
+    int32_t sx, sy;
+    const double xw = wrap(d.x[i] + (double)d0, p.Lx, sx);
+    const double yw = wrap(d.y[i] + (double)d1, p.Ly, sy);
+    T rmn;
+    const T rm = lab_frame<T, KM>(c, thn, r0, K, l, rmn);
+    d.x[i] = xw;
+    d.y[i] = yw;
+    if (sx != 0) d.imgx[i] += sx;  // image counters change only on a wrap (rare): no traffic otherwise
+    if (sy != 0) d.imgy[i] += sy;
+    d.rmax[i] = rm;
+    d.rmin[i] = rmn;
+    if constexpr (EXACT) {
+        T b[W];
+#pragma unroll
+        for (int q = 0; q < W; q++) b[q] = q < NCM ? l[q] : T(0);
+        row_store<T, W>(b, d.lab + i * p.ldc);
+    } else {
+        store_row<T, KM>(l, d.lab + i * p.ldc, K);
+    }
+    if (p.skin_check) {
+        const double dx = min_image(xw - d.x0[i], p.Lx);
+        const double dy = min_image(yw - d.y0[i], p.Ly);
+        const double dr = (double)rm - (double)d.rmax0[i];
+        const double delta = sqrt(dx * dx + dy * dy) + (dr > 0.0 ? dr : 0.0);
+        // one store per wave whose bodies tripped (not one per body)
+        const uint64_t trips = __ballot(2.0 * delta > (double)p.skin);
+        if (trips != 0 && (int)(threadIdx.x & 63) == __ffsll((long long)trips) - 1)
+            __hip_atomic_store(&d.st->rebuild, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+template <typename T, int KM, bool EXACT>
+__device__ __forceinline__ void store_coef(T *row, const T (&c)[2 * KM + 1], int K) {
+    constexpr int NCM = 2 * KM + 1;
+    constexpr int W = row_width<NCM, T>();
+    if constexpr (EXACT) {
+        T a[W];
+#pragma unroll
+        for (int q = 0; q < W; q++) a[q] = q < NCM ? c[q] : T(0);
+        row_store<T, W>(a, row);
+    } else {
+        store_row<T, KM>(c, row, K);
+    }
+}
+
+template <typename T, int KM, bool EXACT>
+__device__ __forceinline__ void load_coef(T (&c)[2 * KM + 1], const T *row, int K) {
+    constexpr int NCM = 2 * KM + 1;
+    constexpr int W = row_width<NCM, T>();
+    if constexpr (EXACT) {
+        T a[W];
+        row_load<T, W>(a, row);
+#pragma unroll
+        for (int q = 0; q < NCM; q++) c[q] = a[q];
+    } else {
+        load_row<T, KM>(c, row, K);
+    }
+}
+
 // Fused Euler–Maruyama update (src/dynamics.jl:310-333) with the
 // displacement and Philox noise of src/dynamics.jl:84-99,258-277 (bodies
 // without parent drag), the body-frame gradient, the shape constraint (FP64
@@ -441,7 +569,6 @@ __global__ __launch_bounds__(256) void k_displacement_ens(const Params<T> *ps, c
 template <typename T, int KM, bool EXACT, bool F64S, int UB, typename PP, typename DP>
 __device__ __forceinline__ void k_update_impl(const PP &p, const DP &d, int use_disp, int debug, int64_t blk) {
     constexpr int NCM = 2 * KM + 1;
-    constexpr int W = row_width<NCM, T>();
     const int64_t i = (int64_t)blk * (UB ? UB : (int)blockDim.x) + threadIdx.x;
     if (i >= p.N || d.st->update != 0) return;
     const uint64_t draw = (uint64_t)(d.st->clock - 1);
@@ -454,81 +581,18 @@ __device__ __forceinline__ void k_update_impl(const PP &p, const DP &d, int use_
         d1 = d.disp[3 * i + 1];
         d2 = d.disp[3 * i + 2];
     } else {
-        T s, co;
-        fsincos(th, &s, &co);
-        T x1 = T(0), x2 = T(0), x3 = T(0), x4;
-        if (p.Dt[ti] != T(0)) philox_pair<T>(p.key0, p.key1, (uint32_t)(i + 1), draw, 0, x1, x2);
-        if (p.Dr[ti] != T(0)) philox_pair<T>(p.key0, p.key1, (uint32_t)(i + 1), draw, 1, x3, x4);
-        const T st = sqrt(T(2) * p.Dt[ti] * p.dt);
-        d0 = p.dt * (p.v0[ti] * co - p.mur[ti] * d.gx[i]) + st * x1;
-        d1 = p.dt * (p.v0[ti] * s - p.mur[ti] * d.gy[i]) + st * x2;
-        d2 = sqrt(T(2) * p.Dr[ti] * p.dt) * x3;
+        body_displacement<T>(p, i, ti, th, d.gx[i], d.gy[i], draw, d0, d1);
+        d2 = rotation_noise<T>(p, i, ti, draw);
     }
-    T gl[NCM], c[NCM], gc[NCM], l[NCM];
-    if constexpr (EXACT) {
-        T a[W], b[W];
-        row_load<T, W>(a, d.glab + i * p.ldc);
-        row_load<T, W>(b, d.coef + i * p.ldc);
-#pragma unroll
-        for (int q = 0; q < NCM; q++) {
-            gl[q] = a[q];
-            c[q] = b[q];
-        }
-    } else {
-        load_row<T, KM>(gl, d.glab + i * p.ldc, K);
-        load_row<T, KM>(c, d.coef + i * p.ldc, K);
-    }
-    body_frame<T, KM>(gl, c, th, r0, K, gc);
-    const T thn = th - p.dt * p.muth[ti] * d.gth[i] + d2;
-    const T h = p.dt * p.muab[ti];
-#pragma unroll
-    for (int m = 1; m <= KM; m++) {
-        c[m] -= h * gc[m];
-        c[KM + m] -= h * gc[KM + m];
-    }
-    if constexpr (F64S)
-        enforce_shape<double, T, KM>(c, K);
-    else
-        enforce_shape<T, T, KM>(c, K);
-    int32_t sx, sy;
-    const double xw = wrap(d.x[i] + (double)d0, p.Lx, sx);
-    const double yw = wrap(d.y[i] + (double)d1, p.Ly, sy);
-    T rmn;
-    const T rm = lab_frame<T, KM>(c, thn, r0, K, l, rmn);
+    T gl[NCM], c[NCM], gc[NCM];
+    load_coef<T, KM, EXACT>(gl, d.glab + i * p.ldc, K);
+    load_coef<T, KM, EXACT>(c, d.coef + i * p.ldc, K);
+    const T thn = shape_step<T, KM, F64S>(p, ti, K, gl, c, th, r0, d.gth[i], d2, gc);
     d.th[i] = thn;
-    d.x[i] = xw;
-    d.y[i] = yw;
-    if (sx != 0) d.imgx[i] += sx;  // image counters change only on a wrap (rare): no traffic otherwise
-    if (sy != 0) d.imgy[i] += sy;
-    d.rmax[i] = rm;
-    d.rmin[i] = rmn;
-    if constexpr (EXACT) {
-        T a[W], b[W], g[W];
-#pragma unroll
-        for (int q = 0; q < W; q++) {
-            a[q] = q < NCM ? c[q] : T(0);
-            b[q] = q < NCM ? l[q] : T(0);
-            g[q] = q < NCM ? gc[q] : T(0);
-        }
-        row_store<T, W>(a, d.coef + i * p.ldc);
-        row_store<T, W>(b, d.lab + i * p.ldc);
-        if (debug) row_store<T, W>(g, d.gcoef + i * p.ldc);
-    } else {
-        store_row<T, KM>(c, d.coef + i * p.ldc, K);
-        store_row<T, KM>(l, d.lab + i * p.ldc, K);
-        if (debug) store_row<T, KM>(gc, d.gcoef + i * p.ldc, K);
-    }
+    store_coef<T, KM, EXACT>(d.coef + i * p.ldc, c, K);
+    if (debug) store_coef<T, KM, EXACT>(d.gcoef + i * p.ldc, gc, K);
     if (i == 0) d.st->stop = d.st->clock >= d.st->target ? 1 : 0;
-    if (p.skin_check) {
-        const double dx = min_image(xw - d.x0[i], p.Lx);
-        const double dy = min_image(yw - d.y0[i], p.Ly);
-        const double dr = (double)rm - (double)d.rmax0[i];
-        const double delta = sqrt(dx * dx + dy * dy) + (dr > 0.0 ? dr : 0.0);
-        // one store per wave whose bodies tripped (not one per body)
-        const uint64_t trips = __ballot(2.0 * delta > (double)p.skin);
-        if (trips != 0 && (int)(threadIdx.x & 63) == __ffsll((long long)trips) - 1)
-            __hip_atomic_store(&d.st->rebuild, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    place_body<T, KM, EXACT>(p, d, i, K, c, thn, r0, d0, d1);
 }
 
 template <typename T, int KM, bool EXACT, bool F64S>
@@ -651,19 +715,16 @@ __global__ __launch_bounds__(256) void k_cell_sort(int64_t nce, const int32_t *s
     }
 }
 
-// neighbor_kernel! (src/neighbors.jl:148-174): 3×3 cells, oy outer, ox
-// inner, cell members in sorted order; entries beyond `cap` are counted only.
-// The largest count goes to st->overflow.
+// neighbor_kernel! (src/neighbors.jl:148-174) for body i: 3×3 cells, oy
+// outer, ox inner, cell members in sorted order; entries beyond `cap` are
+// counted only. Returns the count.
 template <typename T>
-__global__ __launch_bounds__(256) void k_neighbors(int64_t N, const double *x, const double *y, const T *rmax,
-                                                   const int32_t *perm, const int32_t *start,
-                                                   const int32_t *count, double Lx, double Ly, int32_t ncx,
-                                                   int32_t ncy, double wx, double wy, double range,
-                                                   int32_t cap, int32_t *nidx, int32_t *ncount,
-                                                   DevStatus *st) {
+__device__ __forceinline__ int32_t neighbors_of(int64_t i, int64_t N, const double *x, const double *y,
+                                                const T *rmax, const int32_t *perm, const int32_t *start,
+                                                const int32_t *count, double Lx, double Ly, int32_t ncx,
+                                                int32_t ncy, double wx, double wy, double range, int32_t cap,
+                                                int32_t *nidx) {
 #pragma clang fp contract(off)
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= N) return;
     const double xi = x[i], yi = y[i], ri = (double)rmax[i] + range;
     int32_t cx = (int32_t)(xi / wx), cy = (int32_t)(yi / wy);
     cx = cx < ncx - 1 ? cx : ncx - 1;
@@ -699,6 +760,21 @@ __global__ __launch_bounds__(256) void k_neighbors(int64_t N, const double *x, c
             }
         }
     }
+    return n;
+}
+
+// The largest count goes to st->overflow.
+template <typename T>
+__global__ __launch_bounds__(256) void k_neighbors(int64_t N, const double *x, const double *y, const T *rmax,
+                                                   const int32_t *perm, const int32_t *start,
+                                                   const int32_t *count, double Lx, double Ly, int32_t ncx,
+                                                   int32_t ncy, double wx, double wy, double range,
+                                                   int32_t cap, int32_t *nidx, int32_t *ncount,
+                                                   DevStatus *st) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= N) return;
+    const int32_t n = neighbors_of<T>(i, N, x, y, rmax, perm, start, count, Lx, Ly, ncx, ncy, wx, wy, range,
+                                      cap, nidx);
     ncount[i] = n;
     atomicMax(&st->overflow, n);
 }

@@ -132,3 +132,70 @@ def test_mixed_graph_equals_eager(precision):
     assert a.counters().builds > 5
     assert np.array_equal(ba.x, bc.x) and np.array_equal(ba.coef, bc.coef)
     assert a.counters().clock == c.counters().clock == 120
+
+
+@pytest.mark.parametrize("precision", ["mixed_fp16", "mixed_bf16"])
+def test_force_error_at_full_size(oracle_engine, precision):
+    # The headline workload itself: hexagonal_monolayer(512, 512; packing =
+    # 1.0), N = 262144, modes 2..8, Tension(0.01) + Overlap(1.0), v0 0.1,
+    # Dr 0.01, evolved 300 mixed steps on the device (deformed, compressed
+    # contacts); the FP64 oracle and the mixed path then evaluate the
+    # gradients of that same state.
+    import oracle
+    say("full size", precision)
+    b, box = F.hexagonal_monolayer(512, 512, packing=1.0, maxmode=8, seed=1234)
+    kw = dict(types=(F.BodyType(v0=0.1, Dr=0.01, muab=0.1),), energies=(F.Tension(0.01), F.Overlap(1.0)),
+              dt=0.005, seed=1234, skin=0.3)
+    run = F.Simulation(b, box, precision=precision, **kw)
+    F.run(run, 300)
+    state = run.bodies
+    run.close()
+    oracle.set_threads(16)
+    ref_sim = F.refresh(F.Simulation(state, box, debug=True, engine=oracle_engine, **kw))
+    dev_sim = F.refresh(F.Simulation(state, box, debug=True, precision=precision, **kw))
+    ref, dev = ref_sim.grad, dev_sim.grad
+    flips = dev.P != ref.P
+    nflip, nodes_in_contact = int(np.count_nonzero(flips)), int(np.count_nonzero(ref.P))
+    ok = ~np.any(flips, axis=0)
+    fs, ts = force_scales(ref_sim, ref)
+    ef = np.maximum(rel_err(dev.gx, ref.gx, fs), rel_err(dev.gy, ref.gy, fs))[ok]
+    et = rel_err(dev.gth, ref.gth, ts)[ok]
+    # a flipped node changes its body's contour gradient by the coupling J:
+    # shape-mode gradients are compared on the bodies without a flip
+    ec = np.max(np.abs(dev.gcoef[1:, ok] - ref.gcoef[1:, ok])) / np.max(np.abs(ref.gcoef[1:, ok]))
+    say(f"  flips {nflip} of {nodes_in_contact} contact nodes; force {ef.max():.2e}, torque {et.max():.2e}, "
+        f"gcoef {ec:.2e}; bodies compared {ok.sum()} of {ok.size}")
+    assert nodes_in_contact > 1_000_000
+    assert nflip <= nodes_in_contact // 5000  # BF16 hi+lo: R to ~16 bits, FP16 hi+lo: ~22 bits
+    assert ok.sum() >= 0.995 * ok.size
+    assert ef.max() <= 1e-4
+    assert et.max() <= 1e-4
+    assert ec <= 1e-4
+
+
+def test_polydisperse_fp32_vs_fp64(oracle_engine):
+    # The plain-FP32 workload's state family at a smaller size: lognormal R0
+    # (mean 1, sigma 0.1), modes 2..6 uniform and rescaled so that
+    # sum sqrt(an^2 + bn^2) = 0.3, packing 0.95, Philox seed 7.
+    say("polydisperse fp32")
+    b, box = F.hexagonal_monolayer(64, 64, packing=0.95, maxmode=6, seed=7)
+    N = len(b)
+    u = F.philox_uniforms(7, 3 * N + b.coef.shape[0] * N)
+    z = np.sqrt(-2 * np.log(u[:N])) * np.cos(2 * np.pi * u[N:2 * N])
+    b.R0 = np.exp(0.1 * z - 0.005)
+    c = 2 * u[3 * N:3 * N + (b.coef.shape[0] - 1) * N].reshape(b.coef.shape[0] - 1, N) - 1
+    K = (b.coef.shape[0] - 1) // 2
+    amp = np.sum(np.sqrt(c[:K] ** 2 + c[K:] ** 2), axis=0)
+    b.coef[1:, :] = c * (0.3 / amp)
+    ref_sim, ref = gradients(b, box, "fp64", engine=oracle_engine)
+    dev_sim, dev = gradients(b, box, "fp32")
+    flips = dev.P != ref.P
+    ok = ~np.any(flips, axis=0)
+    fs, ts = force_scales(ref_sim, ref)
+    ef = np.maximum(rel_err(dev.gx, ref.gx, fs), rel_err(dev.gy, ref.gy, fs))[ok]
+    et = rel_err(dev.gth, ref.gth, ts)[ok]
+    ec = np.max(np.abs(dev.gcoef[1:] - ref.gcoef[1:])) / np.max(np.abs(ref.gcoef[1:]))
+    say(f"  flips {int(np.count_nonzero(flips))} of {int(np.count_nonzero(ref.P))}; force {ef.max():.2e}, "
+        f"torque {et.max():.2e}, gcoef {ec:.2e}")
+    assert ok.sum() >= 0.99 * ok.size
+    assert ef.max() <= 1e-4 and et.max() <= 1e-4 and ec <= 1e-4
