@@ -14,7 +14,8 @@
 // synthesis is run a second time (a few MFMAs), and the contour gradients
 // replace R, R', R'' in the same registers, where the pullback takes its A
 // operand (the row slots of a K-chunk are exactly the lane's 8 rows of that
-// chunk). The small LDS footprint keeps five waves per SIMD.
+// chunk). The small LDS footprint (5.4 KB per wave) leaves the occupancy to
+// the VGPR count.
 //
 // hi+lo: x = hi + lo with hi = rn(x), lo = rn(x - hi) in the 16-bit type;
 // A·B ≈ Ah·Bh + Ah·Bl + Al·Bh. lab is pre-scaled by 2^10 and G by 2^8 (powers
