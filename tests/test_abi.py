@@ -83,6 +83,16 @@ def test_library_validates_before_touching_the_device():
     assert "replica 1" in eng.last_error()
     assert eng.ensemble_destroy(None) == _abi.FCC_OK
     assert eng.destroy(None) == _abi.FCC_OK
+    # every entry point taking a handle rejects a null one before any HIP call
+    obs = _abi.FccObserverConfig(16, 4, 3.5, 4.0)
+    n = ctypes.c_int64()
+    for rc in (eng.set_bonds(None, 0, None), eng.observer_start(None, ctypes.byref(obs)), eng.observe(None),
+               eng.run_observed(None, 10, 5), eng.observer_read(None, None, None, None, None, None, None),
+               eng.contours(None, None, None), eng.frame_request(None), eng.frame_read(None, None, None, None),
+               eng.checkpoint_save(None, None, 0, ctypes.byref(n)), eng.checkpoint_load(None, None, 0),
+               eng.time_rebuild(None, 1, None), eng.ensemble_observer_start(None, ctypes.byref(obs)),
+               eng.ensemble_observe(None), eng.ensemble_run_observed(None, 10, 5)):
+        assert rc == _abi.FCC_ERR_ARGUMENT
 
 
 def test_hexagonal_monolayer_geometry():
