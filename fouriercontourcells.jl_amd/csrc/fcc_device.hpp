@@ -40,6 +40,16 @@ __device__ __forceinline__ bool step_frozen(DevStatus *st, int64_t blk, int lane
     return fr != 0;
 }
 
+// Tile of workgroup b in a launch of n workgroups, grouped by XCD. The
+// dispatcher places workgroup b on XCD b mod 8; XCD x takes the contiguous
+// tiles [x·q + min(x, r), … + q + (x < r)) with q = n / 8, r = n mod 8 (a
+// bijection on [0, n)), so the neighbours a tile gathers (the lattice rows on
+// either side) were mostly fetched into the same XCD's L2 by nearby tiles.
+__device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t n) {
+    const int64_t x = b & 7, t = b >> 3, q = n >> 3, r = n & 7;
+    return x * q + (x < r ? x : r) + t;
+}
+
 // Status check of the forces kernel: use_status 1 = the kernel opens the step
 // (step_frozen); 2 = a gate kernel already took the step's freeze decision
 // into `update` (ensemble steps, whose lists are rebuilt concurrently).

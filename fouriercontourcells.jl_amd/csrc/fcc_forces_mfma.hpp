@@ -92,10 +92,11 @@ __device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
 }
 
 // Synthesis on MFMA: lane (b, h) supplies lab coefficients 8h..8h+7 of body
-// b (pre-scaled by SYN_SCALE); acc[t] receives tile t of [R; R'; R''] (slots
-// σ = 16t + r), scaled back to R0·(S·lab).
+// b (pre-scaled by SYN_SCALE); acc[t] receives tile t of S·(SYN_SCALE·lab)
+// (slots σ = 16t + r). The contour is R = u·acc with u = R0 / SYN_SCALE; the
+// consumers fold u into per-body factors instead of rescaling every slot.
 template <int K, int T, bool BF>
-__device__ __forceinline__ void synth_tiles(const float *lab_row, bool live, int h, int lane, float unsc,
+__device__ __forceinline__ void synth_tiles(const float *lab_row, bool live, int h, int lane,
                                             const uint16_t *asyn, f16v_t (&acc)[T]) {
     typedef MfmaType<BF> MT;
     typedef typename MT::v_t v_t;
@@ -117,8 +118,6 @@ __device__ __forceinline__ void synth_tiles(const float *lab_row, bool live, int
         c = MT::mfma(ah, bh, c);
         c = MT::mfma(ah, bl, c);
         c = MT::mfma(al, bh, c);
-#pragma unroll
-        for (int r = 0; r < 16; r++) c[r] *= unsc;
         acc[t] = c;
     }
 }
@@ -147,30 +146,33 @@ __device__ __forceinline__ void k_forces_mfma_impl(const PP &p, const DP &d,
     const bool live = i < p.N;
     const int64_t ii = live ? i : 0;
     const float r0 = live ? d.R0[ii] : 1.f;
-    const float unsc = r0 * (1.f / SYN_SCALE);
+    const float u = r0 * (1.f / SYN_SCALE);  // R = u·slot
     const float *lab_row = d.lab + ii * LDC_M;
 
     // ---- synthesis on MFMA
     f16v_t acc[T];
-    synth_tiles<K, T, BF>(lab_row, live, h, lane, unsc, asyn, acc);
-    // slot σ = 16t + r → acc[t][r]; R at σ = j, R' at 20 + j, R'' at 40 + j
+    synth_tiles<K, T, BF>(lab_row, live, h, lane, asyn, acc);
+    // slot σ = 16t + r → acc[t][r]; R/u at σ = j, R'/u at 20 + j, R''/u at 40 + j
 #define SLOT(sig) acc[(sig) >> 4][(sig)&15]
 
     const int ti = live ? d.type[ii] : 0;
     const uint32_t tbit = 1u << ti;
     const bool contact = p.ncontact != 0;
 
-    // ---- pass 1: perimeter and bending integrals over both halves
+    // ---- pass 1: perimeter and bending integrals over both halves, from the
+    // unscaled slots: √(R² + R'²) is homogeneous of degree 1 in (R, R', R''),
+    // the bending integrand of degree -1
     float Qp = 0.f, Qb = 0.f;
 #pragma unroll
     for (int j = 0; j < NH; j++) {
-        const float R = SLOT(j), dR = SLOT(NH + j);
-        Qp += f_sqrt(R * R + dR * dR);
-        if constexpr (T == 4) Qb += term_q<float>(FCC_TERM_BENDING, R, dR, SLOT(2 * NH + j));
-        sR[h * NH + j][b] = R;
+        const float a = SLOT(j), ad = SLOT(NH + j);
+        Qp += f_sqrt(a * a + ad * ad);
+        if constexpr (T == 4) Qb += term_q<float>(FCC_TERM_BENDING, a, ad, SLOT(2 * NH + j));
+        sR[h * NH + j][b] = a * u;
     }
-    Qp += __shfl_xor(Qp, 32);
-    if constexpr (T == 4) Qb += __shfl_xor(Qb, 32);
+    Qp = (Qp + __shfl_xor(Qp, 32)) * u;
+    const float inv_u = SYN_SCALE / r0;
+    if constexpr (T == 4) Qb = (Qb + __shfl_xor(Qb, 32)) * inv_u;
     float Wp = 0.f, Wb = 0.f, Et[MAXB];
 #pragma unroll
     for (int q = 0; q < MAXB; q++) {
@@ -206,25 +208,37 @@ __device__ __forceinline__ void k_forces_mfma_impl(const PP &p, const DP &d,
         const float inv_dphi = (float)(1.0 / (double)p.dphi);
         int cnt = d.ncount[ii];
         if (cnt > d.cap) cnt = d.cap;
+        // The neighbour's position, bounds and lab row are loaded together
+        // (one round trip per neighbour) and the next neighbour's index is
+        // fetched while this one is processed.
+        const int32_t *nrow = d.nidx + ii;
+        int jn = h < cnt ? nrow[(int64_t)h * p.N] : 0;
         for (int q = h; q < cnt; q += 2) {
-            const int j = d.nidx[(int64_t)q * p.N + ii];
+            const int j = jn;
+            const double xj = d.x[j], yj = d.y[j];
+            const float rmaxj = d.rmax[j], rminj = d.rmin[j], r0j = d.R0[j];
+            float labj[W];
+            row_load<float, W>(labj, d.lab + (int64_t)j * LDC_M);
+            if (q + 2 < cnt) jn = nrow[(int64_t)(q + 2) * p.N];
             int cls = 0;
             if (!p.cls_uniform) {
                 cls = p.ccls[pi == d.parent[j] ? 1 : 0][ti][d.type[j]];
                 if (cls < 0) continue;
             }
-            const float dxij = (float)min_image_nb(xi - d.x[j], p.Lx, p.invLx, wrap2);
-            const float dyij = (float)min_image_nb(yi - d.y[j], p.Ly, p.invLy, wrap2);
-            const float lim = d.rmax[j] + p.range;
+            const float dxij = (float)min_image_nb(xi - xj, p.Lx, p.invLx, wrap2);
+            const float dyij = (float)min_image_nb(yi - yj, p.Ly, p.invLy, wrap2);
+            const float lim = rmaxj + p.range;
             const float lim2 = lim * lim;
             const float D2 = dxij * dxij + dyij * dyij;
             const float reach = (rmi + lim) * 1.0001f + 1e-6f;
             if (D2 > reach * reach) continue;
             int k0, wlen;
             node_window(dxij, dyij, lim2, D2, inv_dphi, NTH_M, k0, wlen);
-            float rin = d.rmin[j] * (1.f - 2e-5f) - 1e-6f;
+            float rin = rminj * (1.f - 2e-5f) - 1e-6f;
             const float rin2 = rin > 0.f ? rin * rin : -1.f;
-            uint64_t mask = 0, sure = 0;
+            // node-indexed masks (bit k = node k): within lim of r_j and not
+            // surely inside (mask); surely inside, then hits (hit)
+            uint64_t mask = 0, hit = 0;
             int k = k0;
             for (int t = 0; t < wlen; t++) {
                 const float Rk = sR[k][b];
@@ -235,26 +249,19 @@ __device__ __forceinline__ void k_forces_mfma_impl(const PP &p, const DP &d,
                     dy = min_image(dy, Lyf);
                 }
                 const float d2 = dx * dx + dy * dy;
-                if (d2 <= lim2) mask |= 1ull << t;
-                if (d2 < rin2) sure |= 1ull << t;
+                const uint64_t bit = 1ull << k;
+                if (d2 <= lim2) mask |= bit;
+                if (d2 < rin2) hit |= bit;
                 if (++k == NTH_M) k = 0;
             }
-            mask &= ~sure;
-            uint64_t hit = window_to_nodes(sure, k0, NTH_M);
+            mask &= ~hit;
             if (mask) {
                 float lj[NC];
-                {
-                    float a[W];
-                    row_load<float, W>(a, d.lab + (int64_t)j * LDC_M);
 #pragma unroll
-                    for (int c = 0; c < NC; c++) lj[c] = a[c];
-                }
-                const float r0j = d.R0[j];
+                for (int c = 0; c < NC; c++) lj[c] = labj[c];
                 while (mask) {
-                    const int t = __builtin_ctzll(mask);
+                    const int kk = __builtin_ctzll(mask);
                     mask &= mask - 1;
-                    int kk = k0 + t;
-                    if (kk >= NTH_M) kk -= NTH_M;
                     const float Rk = sR[kk][b];
                     float dx = dxij + Rk * scs[0][kk];
                     float dy = dyij + Rk * scs[1][kk];
@@ -299,21 +306,27 @@ __device__ __forceinline__ void k_forces_mfma_impl(const PP &p, const DP &d,
 
     // ---- pass 2: synthesis again; the contour gradients of the lane's nodes
     // replace R, R', R'' in their register slots
-    synth_tiles<K, T, BF>(lab_row, live, h, lane, unsc, asyn, acc);
+    synth_tiles<K, T, BF>(lab_row, live, h, lane, asyn, acc);
+    // Scaled gradients GR·sc, GD·sc, GDD·sc (sc = PULL_SCALE, a power of two)
+    // from the unscaled slots a = R/u: R·rsqrt(R² + R'²) is scale free, the
+    // bending gradients are of degree -2, the contact terms take u into the
+    // per-body factor pf; fx, fy, tt carry sc (and tt one more u) until the end.
     float fx = 0.f, fy = 0.f, tt = 0.f;
     const float sc = live ? PULL_SCALE : 0.f;
+    const float Wps = Wp * sc, Wbs = Wb * sc * inv_u * inv_u;
+    const float pf = p.dphi * u * sc;
 #pragma unroll
     for (int j = 0; j < NH; j++) {
         const int k = h * NH + j;
-        const float R = SLOT(j), dR = SLOT(NH + j);
-        const float rs = f_rsqrt(R * R + dR * dR);
-        float GR = Wp * (R * rs), GD = Wp * (dR * rs), GDD = 0.f;
+        const float a = SLOT(j), ad = SLOT(NH + j);
+        const float rs = f_rsqrt(a * a + ad * ad);
+        float GR = Wps * (a * rs), GD = Wps * (ad * rs), GDD = 0.f;
         if constexpr (T == 4) {
             float j0, j1, j2;
-            term_J<float>(FCC_TERM_BENDING, R, dR, SLOT(2 * NH + j), j0, j1, j2);
-            GR += Wb * j0;
-            GD += Wb * j1;
-            GDD = Wb * j2;
+            term_J<float>(FCC_TERM_BENDING, a, ad, SLOT(2 * NH + j), j0, j1, j2);
+            GR += Wbs * j0;
+            GD += Wbs * j1;
+            GDD = Wbs * j2;
         }
         if (contact) {
             float g = 0.f;
@@ -324,26 +337,26 @@ __device__ __forceinline__ void k_forces_mfma_impl(const PP &p, const DP &d,
                 for (int qq = 0; qq < NPLANE; qq++) n |= __builtin_amdgcn_ubfe(plh[c][qq], (uint32_t)j, 1u) << qq;
                 g += p.Jcls[c] * (float)n;
             }
-            const float pp = g * p.dphi;
+            const float pp = g * pf;
             const float c = scs[0][k], s = scs[1][k];
-            GR += pp * R;
-            fx += pp * (R * c + dR * s);
-            fy += pp * (R * s - dR * c);
-            tt += (-pp) * R * dR;
+            GR += pp * a;
+            fx += pp * (a * c + ad * s);
+            fy += pp * (a * s - ad * c);
+            tt += (-pp) * a * ad;
             if (debug && live) d.P[ii * NTH_M + k] = g;
         }
         if (debug && live) {
-            d.GR[ii * NTH_M + k] = GR;
-            d.GD[ii * NTH_M + k] = GD;
-            d.GDD[ii * NTH_M + k] = GDD;
+            d.GR[ii * NTH_M + k] = GR * (1.f / PULL_SCALE);
+            d.GD[ii * NTH_M + k] = GD * (1.f / PULL_SCALE);
+            d.GDD[ii * NTH_M + k] = GDD * (1.f / PULL_SCALE);
         }
-        SLOT(j) = GR * sc;
-        SLOT(NH + j) = GD * sc;
-        if constexpr (T == 4) SLOT(2 * NH + j) = GDD * sc;
+        SLOT(j) = GR;
+        SLOT(NH + j) = GD;
+        if constexpr (T == 4) SLOT(2 * NH + j) = GDD;
     }
-    fx += __shfl_xor(fx, 32);
-    fy += __shfl_xor(fy, 32);
-    tt += __shfl_xor(tt, 32);
+    fx = (fx + __shfl_xor(fx, 32)) * (1.f / PULL_SCALE);
+    fy = (fy + __shfl_xor(fy, 32)) * (1.f / PULL_SCALE);
+    tt = (tt + __shfl_xor(tt, 32)) * (u * (1.f / PULL_SCALE));
 
     // ---- pullback on MFMA: glabᵀ (bodies x coefs) = Gᵀ·S over 2T K-chunks of 16 rows
     f16v_t gacc = {};
@@ -393,7 +406,7 @@ template <int K, int T, int NCLS, bool BF>
 __global__ __launch_bounds__(64) void k_forces_mfma(const Params<float> p, const Ptrs<float> d,
                                                     const uint16_t *asyn, const uint16_t *bpull, int use_status,
                                                     int debug) {
-    k_forces_mfma_impl<K, T, NCLS, BF>(p, d, asyn, bpull, use_status, debug, blockIdx.x);
+    k_forces_mfma_impl<K, T, NCLS, BF>(p, d, asyn, bpull, use_status, debug, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 template <int K, int T, int NCLS, bool BF>
