@@ -305,6 +305,18 @@ __device__ __forceinline__ void k_forces_fast_impl(const PP &p, const DP &d, int
 #pragma unroll
     for (int c = 0; c < NC; c++) gl[c] = T(0);
     T fx = T(0), fy = T(0), tt = T(0);
+    constexpr int NCL = NCLS > 0 ? NCLS : 1;
+    T Jd[NCL];
+#pragma unroll
+    for (int c = 0; c < NCL; c++) Jd[c] = p.Jcls[c] * p.dphi;
+    // Planes 2.. hold counts of 4 or more (a node inside four neighbours):
+    // they are read only when some lane of the wave has one.
+    bool hi = false;
+#pragma unroll
+    for (int c = 0; c < NCL; c++)
+#pragma unroll
+        for (int q = 2; q < NPLANE; q++) hi |= pl[c][q] != 0;
+    const bool hi_any = __ballot(hi) != 0;
     for (int k = 0; k < nth; k++) {
         T R, dR, ddR;
         synth_k<T, K>(S, nth, ldc, k, lab, need_dd, r0, R, dR, ddR);
@@ -319,15 +331,17 @@ __device__ __forceinline__ void k_forces_fast_impl(const PP &p, const DP &d, int
             GDD = Wb * j2;
         }
         if (contact) {
-            T g = T(0);
+            T pp = T(0), g = T(0);
 #pragma unroll
-            for (int c = 0; c < (NCLS > 0 ? NCLS : 1); c++) {
-                uint32_t n = 0;
+            for (int c = 0; c < NCL; c++) {
+                uint32_t n = node_bit(pl[c][0], k) | (node_bit(pl[c][1], k) << 1);
+                if (hi_any) {
 #pragma unroll
-                for (int qq = 0; qq < NPLANE; qq++) n |= node_bit(pl[c][qq], k) << qq;
-                g += p.Jcls[c] * (T)n;
+                    for (int qq = 2; qq < NPLANE; qq++) n |= node_bit(pl[c][qq], k) << qq;
+                }
+                pp += Jd[c] * (T)n;
+                if (debug) g += p.Jcls[c] * (T)n;
             }
-            const T pp = g * p.dphi;
             const T c = CS[2 * k], s = CS[2 * k + 1];
             GR += pp * R;
             fx += pp * (R * c + dR * s);

@@ -315,6 +315,17 @@ __device__ __forceinline__ void k_forces_mfma_impl(const PP &p, const DP &d,
     const float sc = live ? PULL_SCALE : 0.f;
     const float Wps = Wp * sc, Wbs = Wb * sc * inv_u * inv_u;
     const float pf = p.dphi * u * sc;
+    float Jpf[NCLS];
+#pragma unroll
+    for (int c = 0; c < NCLS; c++) Jpf[c] = p.Jcls[c] * pf;
+    // Planes 2.. hold counts of 4 or more (a node inside four neighbours):
+    // they are read only when some lane of the wave has one.
+    bool hi = false;
+#pragma unroll
+    for (int c = 0; c < NCLS; c++)
+#pragma unroll
+        for (int q = 2; q < NPLANE; q++) hi |= plh[c][q] != 0;
+    const bool hi_any = __ballot(hi) != 0;
 #pragma unroll
     for (int j = 0; j < NH; j++) {
         const int k = h * NH + j;
@@ -329,30 +340,47 @@ __device__ __forceinline__ void k_forces_mfma_impl(const PP &p, const DP &d,
             GDD = Wbs * j2;
         }
         if (contact) {
-            float g = 0.f;
+            float pp = 0.f;
 #pragma unroll
             for (int c = 0; c < NCLS; c++) {
-                uint32_t n = 0;
+                uint32_t n = __builtin_amdgcn_ubfe(plh[c][0], (uint32_t)j, 1u) |
+                             (__builtin_amdgcn_ubfe(plh[c][1], (uint32_t)j, 1u) << 1);
+                if (hi_any) {
 #pragma unroll
-                for (int qq = 0; qq < NPLANE; qq++) n |= __builtin_amdgcn_ubfe(plh[c][qq], (uint32_t)j, 1u) << qq;
-                g += p.Jcls[c] * (float)n;
+                    for (int qq = 2; qq < NPLANE; qq++) n |= __builtin_amdgcn_ubfe(plh[c][qq], (uint32_t)j, 1u) << qq;
+                }
+                pp += Jpf[c] * (float)n;
             }
-            const float pp = g * pf;
             const float c = scs[0][k], s = scs[1][k];
             GR += pp * a;
             fx += pp * (a * c + ad * s);
             fy += pp * (a * s - ad * c);
             tt += (-pp) * a * ad;
-            if (debug && live) d.P[ii * NTH_M + k] = g;
-        }
-        if (debug && live) {
-            d.GR[ii * NTH_M + k] = GR * (1.f / PULL_SCALE);
-            d.GD[ii * NTH_M + k] = GD * (1.f / PULL_SCALE);
-            d.GDD[ii * NTH_M + k] = GDD * (1.f / PULL_SCALE);
         }
         SLOT(j) = GR;
         SLOT(NH + j) = GD;
         if constexpr (T == 4) SLOT(2 * NH + j) = GDD;
+    }
+    if (debug && live) {
+        // contour gradients and contact density at the lane's nodes (tests)
+#pragma unroll
+        for (int j = 0; j < NH; j++) {
+            const int64_t o = ii * NTH_M + h * NH + j;
+            d.GR[o] = SLOT(j) * (1.f / PULL_SCALE);
+            d.GD[o] = SLOT(NH + j) * (1.f / PULL_SCALE);
+            d.GDD[o] = T == 4 ? SLOT(2 * NH + j) * (1.f / PULL_SCALE) : 0.f;
+            if (contact) {
+                float g = 0.f;
+#pragma unroll
+                for (int c = 0; c < NCLS; c++) {
+                    uint32_t n = 0;
+#pragma unroll
+                    for (int qq = 0; qq < NPLANE; qq++) n |= __builtin_amdgcn_ubfe(plh[c][qq], (uint32_t)j, 1u) << qq;
+                    g += p.Jcls[c] * (float)n;
+                }
+                d.P[o] = g;
+            }
+        }
     }
     fx = (fx + __shfl_xor(fx, 32)) * (1.f / PULL_SCALE);
     fy = (fy + __shfl_xor(fy, 32)) * (1.f / PULL_SCALE);
