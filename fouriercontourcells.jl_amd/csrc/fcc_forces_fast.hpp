@@ -80,8 +80,9 @@ __device__ __forceinline__ void node_window(float dx, float dy, float lim2, floa
     const int khi = (int)ceilf((psi + delta) * inv_dphi);
     if (khi - klo + 1 >= nth) return;
     wlen = khi - klo + 1;
-    k0 = klo % nth;
-    if (k0 < 0) k0 += nth;
+    // ψ ∈ [-π, π] and δ ≤ π/2 + 2e-3 give klo ∈ [-3nθ/4 - 1, nθ/2]: one
+    // wrap brings it into [0, nθ)
+    k0 = klo < 0 ? klo + nth : klo;
 }
 
 // Bit k of a 64-bit node mask, k uniform over the wave.

@@ -280,10 +280,19 @@ __device__ __forceinline__ void k_forces_mfma_impl(const PP &p, const DP &d,
             for (int c = 0; c < NCLS; c++) {
                 uint64_t m = cls == c ? hit : 0;
 #pragma unroll
-                for (int qq = 0; qq < NPLANE; qq++) {
+                for (int qq = 0; qq < 2; qq++) {
                     const uint64_t carry = pl[c][qq] & m;
                     pl[c][qq] ^= m;
                     m = carry;
+                }
+                // a carry into plane 2 (a node reaching a count of 4) is rare
+                if (__ballot(m != 0)) {
+#pragma unroll
+                    for (int qq = 2; qq < NPLANE; qq++) {
+                        const uint64_t carry = pl[c][qq] & m;
+                        pl[c][qq] ^= m;
+                        m = carry;
+                    }
                 }
             }
         }
