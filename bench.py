@@ -222,6 +222,20 @@ def measured_traffic(kernel):
     return None
 
 
+def measured_issue(kernel):
+    """(VALU issue fraction, MFMA busy fraction) of `kernel` from the committed
+    rocprofv3 --pmc SQ counter summary (profiles/forces_N262144_sq_counters.txt),
+    or (None, None)."""
+    path = os.path.join(ROOT, "profiles", "forces_N262144_sq_counters.txt")
+    if not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 3 and f[0] == kernel:
+            return float(f[1]), float(f[2])
+    return None, None
+
+
 def contact_pairs(sim):
     """(tested, in-range) (node, neighbour) pairs of the current state: the
     reference tests every listed neighbour at every node and evaluates R_j(ψ)
@@ -425,6 +439,7 @@ def main():
     flops = forces_flops(sim, tested, inr)
     fkernel = "k_forces_mfma" if args.precision.startswith("mixed") else "k_forces_fast"
     achieved = flops / (fms * 1e-3) / 1e12
+    valu_frac, mfma_frac = measured_issue(fkernel) if sim.N == 262144 else (None, None)
     upd_bytes = 340 * sim.N  # per body-step, 188 B read + 152 B written
     print(f"[bench] forces {fms * 1e3:.1f} us, update {ums * 1e3:.1f} us (back-to-back launches; "
           f"last step of each chunk in the run: {run_fms * 1e3:.1f} / {run_ums * 1e3:.1f} us), "
@@ -458,7 +473,8 @@ def main():
         "roofline": {"kernel": fkernel, "bound": "mfma", "achieved": achieved,
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                      "traffic": measured_traffic(fkernel) if sim.N == 262144 else None,
-                     "launch_ms": fms, "launch_ms_in_run": run_fms, "flops_per_launch": flops},
+                     "launch_ms": fms, "launch_ms_in_run": run_fms, "flops_per_launch": flops,
+                     "valu_issue_frac": valu_frac, "mfma_busy_frac": mfma_frac},
         "roofline_update": {"kernel": "k_update", "bound": "hbm",
                             "traffic": measured_traffic("k_update") if sim.N == 262144 else None,
                             "achieved": upd_bytes / (ums * 1e-3) / 1e9, "peak": PEAK_HBM_GBS,
