@@ -295,3 +295,49 @@ def test_linear_repulsion_matches_oracle(hip, oracle_engine, precision):
         F.step(g, 20)
         np.testing.assert_allclose(g.bodies.x, a.bodies.x, rtol=1e-8)
         np.testing.assert_allclose(g.bodies.coef, a.bodies.coef, rtol=1e-8, atol=1e-12)
+
+
+def clusters(k=12, sites=4, spacing=5.0, jitter=0.15, seed=9, maxmode=6):
+    """Clusters of k deformed bodies around the sites of a square lattice
+    (sites farther apart than the cutoff): a node is inside up to k - 1 other
+    bodies, so the contact counts reach every bit plane of the forces
+    kernels' counters (the planes of counts 4 and more, and their carries,
+    are only touched when some lane of a wave has such a count)."""
+    rng = np.random.default_rng(seed)
+    L = sites * spacing
+    sx, sy = np.meshgrid((np.arange(sites) + 0.5) * spacing, (np.arange(sites) + 0.5) * spacing)
+    n = sites * sites * k
+    x = np.repeat(sx.ravel(), k) + jitter * rng.standard_normal(n)
+    y = np.repeat(sy.ravel(), k) + jitter * rng.standard_normal(n)
+    nc = 2 * (maxmode - 1) + 1
+    coef = np.zeros((nc, n), order="F")
+    coef[0] = 1.0
+    coef[1:, :] = 0.04 * rng.standard_normal((nc - 1, n))
+    return F.Bodies(x, y, rng.uniform(0, 2 * np.pi, n), coef, np.ones(n)), F.PeriodicBox(L, L)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "mixed_fp16", "mixed_bf16"])
+def test_dense_clusters_contact_counts(hip, oracle_engine, precision):
+    say("dense clusters", precision)
+    b, box = clusters()
+    kw = dict(types=(F.BodyType(),), energies=(F.Tension(0.01), F.Overlap(1.0)), debug=True)
+    a = F.refresh(F.Simulation(b, box, engine=oracle_engine, **kw))
+    g = F.refresh(F.Simulation(b, box, precision=precision, **kw))
+    ga, gg = a.grad, g.grad
+    assert ga.P.max() >= 8 and np.count_nonzero(ga.P >= 4) > 1000
+    if precision == "fp64":
+        for f in ("P", "GR", "GD", "gx", "gy", "gth", "gcoef", "Ebody"):
+            np.testing.assert_allclose(getattr(gg, f), getattr(ga, f), rtol=1e-10, atol=1e-10, err_msg=f)
+        return
+    # FP32 contact: a node within rounding of a neighbour's contour may flip
+    differ = gg.P != ga.P
+    say("  flips", int(differ.sum()), "of", int(np.count_nonzero(ga.P)), "contact nodes")
+    assert differ.sum() <= 2e-3 * np.count_nonzero(ga.P)
+    assert gg.P.max() == ga.P.max()
+    clean = ~differ.any(axis=0)
+    assert clean.sum() >= 0.9 * len(clean)
+    for f in ("gx", "gy", "gth"):
+        ref, got = getattr(ga, f)[clean], getattr(gg, f)[clean]
+        np.testing.assert_allclose(got, ref, rtol=2e-4, atol=2e-4 * np.max(np.abs(ref)), err_msg=f)
+    ref, got = ga.gcoef[:, clean], gg.gcoef[:, clean]
+    np.testing.assert_allclose(got, ref, rtol=2e-4, atol=2e-4 * np.max(np.abs(ref)))
