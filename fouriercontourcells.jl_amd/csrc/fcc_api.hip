@@ -616,7 +616,7 @@ struct Sim final : fcc_sim {
         }
         if (fast_ok()) {
 #define FCC_FAST(KV, NC)                                                                                 \
-    if (nth == 40) FCC_LAUNCH((k_forces_fast<T, 40, KV, NC>), g, 64, 0, stream, eb, ee, p, d, use_status, dbg); \
+    if (nth == 40) FCC_LAUNCH((k_forces_fast2<T, KV, NC>), blocks(N, 32), 64, 0, stream, eb, ee, p, d, use_status, dbg); \
     else FCC_LAUNCH((k_forces_fast<T, 0, KV, NC>), g, 64, 0, stream, eb, ee, p, d, use_status, dbg);
             if (K == 5) {
                 if (p.ncls <= 1) { FCC_FAST(5, 1) } else { FCC_FAST(5, 2) }

@@ -184,7 +184,7 @@ struct Ensemble final : fcc_ensemble {
         }
         if (var == 1) {
 #define FCC_FE(KV, NC)                                                                                        \
-    if (a.nth == 40) FCC_LAUNCH((k_forces_fast_ens<T, 40, KV, NC>), gf, 64, 0, stream, eb, ee, dP, dD, tf, 2, dbg); \
+    if (a.nth == 40) FCC_LAUNCH((k_forces_fast2_ens<T, KV, NC>), gf, 64, 0, stream, eb, ee, dP, dD, tf, 2, dbg); \
     else FCC_LAUNCH((k_forces_fast_ens<T, 0, KV, NC>), gf, 64, 0, stream, eb, ee, dP, dD, tf, 2, dbg);
             if (a.K == 5) {
                 if (ncls <= 1) { FCC_FE(5, 1) } else { FCC_FE(5, 2) }
@@ -266,7 +266,10 @@ struct Ensemble final : fcc_ensemble {
         if (var != tiles_var || lc.update_block != tiles_lc.update_block || lc.xcd_order != tiles_lc.xcd_order) {
             std::vector<int64_t> sizes(R);
             for (size_t r = 0; r < R; r++) sizes[r] = m[r]->N;
-            const std::vector<Tile> a = tile_table(sizes, var == 2 ? 32 : 64, lc.xcd_order != 0);
+            // 32 bodies per forces workgroup for the two-lane kernels (MFMA, and the
+            // FP32 / FP64 one at nθ = 40), 64 otherwise
+            const bool two_lane = var == 2 || (var == 1 && m[0]->nth == 40);
+            const std::vector<Tile> a = tile_table(sizes, two_lane ? 32 : 64, lc.xcd_order != 0);
             const std::vector<Tile> b = tile_table(sizes, lc.update_block, lc.xcd_order != 0);
             HIPCHK(hipStreamSynchronize(stream));
             if (a.size() > cap_f) {

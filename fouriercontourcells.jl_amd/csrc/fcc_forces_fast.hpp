@@ -1,4 +1,6 @@
-// fcc_forces_fast.hpp — the forces kernel of the hot path (FP32 / FP64 VALU).
+// fcc_forces_fast.hpp — the forces kernels of the hot path (FP32 / FP64 VALU):
+// k_forces_fast2 (two lanes per body, nθ = 40) and k_forces_fast (one lane per
+// body, any nθ ≤ 64).
 //
 // Same work as k_forces (body terms, Overlap contact, pullback; src/gradients.jl,
 // src/energies/contact.jl) with the shape of the hot configurations fixed at
@@ -392,6 +394,325 @@ __global__ __launch_bounds__(64) void k_forces_fast_ens(const Params<T> *ps, con
     const FCC_CONST Params<T> &p = as_const(ps)[tl.rep];
     const FCC_CONST Ptrs<T> &d = as_const(ds)[tl.rep];
     k_forces_fast_impl<T, NTH_, K, NCLS>(p, d, use_status, debug, tl.blk);
+}
+
+// Bit-sliced sum of two per-node counters (NPLANE planes each).
+__device__ __forceinline__ void planes_add(uint64_t (&a)[NPLANE], const uint64_t (&b)[NPLANE]) {
+    uint64_t c = 0;
+#pragma unroll
+    for (int q = 0; q < NPLANE; q++) {
+        const uint64_t s = a[q] ^ b[q] ^ c;
+        c = (a[q] & b[q]) | (c & (a[q] ^ b[q]));
+        a[q] = s;
+    }
+}
+
+__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
+    const uint32_t lo = __shfl_xor((uint32_t)v, m), hi = __shfl_xor((uint32_t)(v >> 32), m);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// The same work with two lanes per body (nθ = 40): 32 bodies per wave, lane
+// (b, h) takes nodes 20h..20h+19 of body b in the node passes and neighbour
+// slots q ≡ h (mod 2) over the whole circle in the contact pass, as the
+// mixed-precision kernel does. Node j + 20 sits at φ_j + π, where the basis
+// rows are those of node j with the odd modes negated ((-1)^n cos nφ, …): lane
+// half 1 negates the odd modes of its lab row once, so both halves read the
+// same wave-uniform basis rows through the scalar cache, and the pullback of
+// half 1 takes the same signs before the halves are summed.
+template <typename T, int K, int NCLS, typename PP, typename DP>
+__device__ __forceinline__ void k_forces_fast2_impl(const PP &p, const DP &d, int use_status, int debug,
+                                                     int64_t blk) {
+    constexpr int NTH = 40, NH2 = 20;
+    constexpr int NC = 2 * K + 1;
+    constexpr int NCL = NCLS > 0 ? NCLS : 1;
+    constexpr int W = row_width<NC, T>();
+    __shared__ T sR[NTH][32];
+    __shared__ T scs[2][NTH];
+    const int lane = threadIdx.x;
+    const int b = lane & 31, h = lane >> 5;
+    for (int k = lane; k < NTH; k += 64) {
+        scs[0][k] = d.cs[2 * k];
+        scs[1][k] = d.cs[2 * k + 1];
+    }
+    const int64_t i = (int64_t)blk * 32 + b;
+    if (forces_frozen(use_status, d.st, blk, lane)) return;  // uniform over the grid
+    const bool live = i < p.N;
+    const int64_t ii = live ? i : 0;
+    const FCC_CONST T *S = as_const(d.Sc);
+    const int ldc = p.ldc;
+    const T sg = h ? T(-1) : T(1);  // sign of the odd modes at node j + 20h
+    T lab[NC];
+    {
+        T a[W];
+        row_load<T, W>(a, d.lab + ii * ldc);
+#pragma unroll
+        for (int c = 0; c < NC; c++) lab[c] = a[c];
+#pragma unroll
+        for (int m = 2; m <= K; m += 2) {  // mode m + 1 odd
+            lab[m] *= sg;
+            lab[K + m] *= sg;
+        }
+    }
+    const T r0 = live ? d.R0[ii] : T(1);
+    const int ti = live ? d.type[ii] : 0;
+    const uint32_t tbit = 1u << ti;
+    const bool need_dd = p.need_dd != 0;
+    const bool contact = p.ncontact != 0;
+
+    // pass 1: perimeter and bending integrals over both halves
+    T Qp = T(0), Qb = T(0);
+    for (int j = 0; j < NH2; j++) {
+        T R, dR, ddR;
+        synth_k<T, K>(S, NTH, ldc, j, lab, need_dd, r0, R, dR, ddR);
+        if (contact) sR[h * NH2 + j][b] = R;
+        Qp += f_sqrt(R * R + dR * dR);
+        if (need_dd) Qb += term_q<T>(FCC_TERM_BENDING, R, dR, ddR);
+    }
+    Qp += __shfl_xor(Qp, 32);
+    Qb += __shfl_xor(Qb, 32);
+    T Wp = T(0), Wb = T(0), Et[MAXB];
+#pragma unroll
+    for (int q = 0; q < MAXB; q++) {
+        Et[q] = T(0);
+        if (q < p.nbody && (p.bmask[q] & tbit)) {
+            T dF;
+            const bool bend = p.bkind[q] == FCC_TERM_BENDING;
+            Et[q] = term_E<T>(p.bkind[q], p.bp0[q], p.bp1[q], (bend ? Qb : Qp) * p.dphi, r0, dF);
+            if (bend)
+                Wb += dF * p.dphi;
+            else
+                Wp += dF * p.dphi;
+        }
+    }
+    T E = T(0);
+#pragma unroll
+    for (int q = MAXB - 1; q >= 0; q--)
+        if (q < p.nbody) E = Et[q] + E;
+    __syncthreads();  // sR, scs complete
+
+    // contact: hit counts per node and coupling class
+    uint64_t pl[NCL][NPLANE];
+#pragma unroll
+    for (int c = 0; c < NCL; c++)
+#pragma unroll
+        for (int q = 0; q < NPLANE; q++) pl[c][q] = 0;
+    if (contact && live) {
+        const double xi = d.x[ii], yi = d.y[ii];
+        const T rmi = d.rmax[ii];
+        const int pi = d.parent[ii];
+        const T Lxf = (T)p.Lx, Lyf = (T)p.Ly;
+        const bool wrap2 = p.small_box != 0;
+        const float inv_dphi = (float)(1.0 / (double)p.dphi);
+        int cnt = d.ncount[ii];
+        if (cnt > d.cap) cnt = d.cap;
+        // the neighbour's position, bounds and lab row in one round trip, the
+        // next neighbour's index one iteration ahead
+        const int32_t *nrow = d.nidx + ii;
+        int jn = h < cnt ? nrow[(int64_t)h * p.N] : 0;
+        for (int q = h; q < cnt; q += 2) {
+            const int j = jn;
+            const double xj = d.x[j], yj = d.y[j];
+            const T rmaxj = d.rmax[j], rminj = d.rmin[j], r0j = d.R0[j];
+            T labj[W];
+            row_load<T, W>(labj, d.lab + (int64_t)j * ldc);
+            if (q + 2 < cnt) jn = nrow[(int64_t)(q + 2) * p.N];
+            int cls = 0;
+            if (!p.cls_uniform) {
+                cls = p.ccls[pi == d.parent[j] ? 1 : 0][ti][d.type[j]];
+                if (cls < 0) continue;  // zero coupling: no contribution
+            }
+            const T dxij = (T)min_image_nb(xi - xj, p.Lx, p.invLx, wrap2);
+            const T dyij = (T)min_image_nb(yi - yj, p.Ly, p.invLy, wrap2);
+            const T lim = rmaxj + p.range;
+            const T lim2 = lim * lim;
+            const T D2 = dxij * dxij + dyij * dyij;
+            const T reach = (rmi + lim) * T(1.0001) + T(1e-6);
+            if (D2 > reach * reach) continue;
+            int k0, wlen;
+            node_window((float)dxij, (float)dyij, (float)lim2, (float)D2, inv_dphi, NTH, k0, wlen);
+            // sure-inside radius: R_j(ψ) ≥ rmin_j exactly, and the evaluated
+            // R_j differs from the exact one by far less than the margin
+            T rin = rminj * (T(1) - (sizeof(T) == 8 ? T(1e-12) : T(2e-5))) - (sizeof(T) == 8 ? T(1e-13) : T(1e-6));
+            const T rin2 = rin > T(0) ? rin * rin : T(-1);
+            // node-indexed masks (bit k = node k): within lim of r_j and not
+            // surely inside (mask); surely inside, then hits (hit)
+            uint64_t mask = 0, hit = 0;
+            int k = k0;
+            for (int t = 0; t < wlen; t++) {
+                const T Rk = sR[k][b];
+                T dx = dxij + Rk * scs[0][k];
+                T dy = dyij + Rk * scs[1][k];
+                if (wrap2) {
+                    dx = min_image(dx, Lxf);
+                    dy = min_image(dy, Lyf);
+                }
+                const T d2 = dx * dx + dy * dy;
+                const uint64_t bit = 1ull << k;
+                if (d2 <= lim2) mask |= bit;
+                if (d2 < rin2) hit |= bit;
+                if (++k == NTH) k = 0;
+            }
+            mask &= ~hit;
+            if (mask) {
+                T lj[NC];
+#pragma unroll
+                for (int c = 0; c < NC; c++) lj[c] = labj[c];
+                while (mask) {
+                    const int kk = __builtin_ctzll(mask);
+                    mask &= mask - 1;
+                    const T Rk = sR[kk][b];
+                    T dx = dxij + Rk * scs[0][kk];
+                    T dy = dyij + Rk * scs[1][kk];
+                    if (wrap2) {
+                        dx = min_image(dx, Lxf);
+                        dy = min_image(dy, Lyf);
+                    }
+                    const T d2 = fmax(dx * dx + dy * dy, tiny_d2<T>());
+                    const T rs = f_rsqrt(d2);
+                    const T dist = d2 * rs;
+                    const T cp = dx * rs, sp = dy * rs;
+                    if (r0j * contour_horner<T, K>(lj, cp, sp) - dist >= T(0)) hit |= 1ull << kk;
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < NCL; c++) {
+                uint64_t m = cls == c ? hit : 0;
+#pragma unroll
+                for (int qq = 0; qq < 2; qq++) {
+                    const uint64_t carry = pl[c][qq] & m;
+                    pl[c][qq] ^= m;
+                    m = carry;
+                }
+                // a carry into plane 2 (a node reaching a count of 4) is rare
+                if (__ballot(m != 0)) {
+#pragma unroll
+                    for (int qq = 2; qq < NPLANE; qq++) {
+                        const uint64_t carry = pl[c][qq] & m;
+                        pl[c][qq] ^= m;
+                        m = carry;
+                    }
+                }
+            }
+        }
+    }
+    // both lanes of a body: add the partner's counters
+#pragma unroll
+    for (int c = 0; c < NCL; c++) {
+        uint64_t other[NPLANE];
+#pragma unroll
+        for (int q = 0; q < NPLANE; q++) other[q] = shfl_xor64(pl[c][q], 32);
+        planes_add(pl[c], other);
+    }
+    // the lane's 20 nodes of each counter plane at bits 0..19
+    uint32_t plh[NCL][NPLANE];
+#pragma unroll
+    for (int c = 0; c < NCL; c++)
+#pragma unroll
+        for (int q = 0; q < NPLANE; q++) plh[c][q] = (uint32_t)(pl[c][q] >> (h * NH2));
+
+    // pass 2: contour gradients, contact force / torque, pullback
+    T gl[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) gl[c] = T(0);
+    T fx = T(0), fy = T(0), tt = T(0);
+    T Jd[NCL];
+#pragma unroll
+    for (int c = 0; c < NCL; c++) Jd[c] = p.Jcls[c] * p.dphi;
+    // Planes 2.. hold counts of 4 or more (a node inside four neighbours):
+    // they are read only when some lane of the wave has one.
+    bool hi = false;
+#pragma unroll
+    for (int c = 0; c < NCL; c++)
+#pragma unroll
+        for (int q = 2; q < NPLANE; q++) hi |= plh[c][q] != 0;
+    const bool hi_any = __ballot(hi) != 0;
+    for (int j = 0; j < NH2; j++) {
+        const int k = h * NH2 + j;
+        T R, dR, ddR;
+        synth_k<T, K>(S, NTH, ldc, j, lab, need_dd, r0, R, dR, ddR);
+        if (contact) R = sR[k][b];
+        const T rs = f_rsqrt(R * R + dR * dR);
+        T GR = Wp * (R * rs), GD = Wp * (dR * rs), GDD = T(0);
+        if (need_dd) {
+            T j0, j1, j2;
+            term_J<T>(FCC_TERM_BENDING, R, dR, ddR, j0, j1, j2);
+            GR += Wb * j0;
+            GD += Wb * j1;
+            GDD = Wb * j2;
+        }
+        if (contact) {
+            T pp = T(0), g = T(0);
+#pragma unroll
+            for (int c = 0; c < NCL; c++) {
+                uint32_t n = __builtin_amdgcn_ubfe(plh[c][0], (uint32_t)j, 1u) |
+                             (__builtin_amdgcn_ubfe(plh[c][1], (uint32_t)j, 1u) << 1);
+                if (hi_any) {
+#pragma unroll
+                    for (int qq = 2; qq < NPLANE; qq++) n |= __builtin_amdgcn_ubfe(plh[c][qq], (uint32_t)j, 1u) << qq;
+                }
+                pp += Jd[c] * (T)n;
+                if (debug) g += p.Jcls[c] * (T)n;
+            }
+            const T c = scs[0][k], s = scs[1][k];
+            GR += pp * R;
+            fx += pp * (R * c + dR * s);
+            fy += pp * (R * s - dR * c);
+            tt += (-pp) * R * dR;
+            if (debug && live) d.P[ii * NTH + k] = g;
+        }
+        const FCC_CONST T *C = S + (size_t)j * ldc;
+        const FCC_CONST T *D = S + (size_t)(NTH + j) * ldc;
+        gl[0] += GR;
+#pragma unroll
+        for (int c = 1; c < NC; c++) gl[c] = fma(C[c], GR, fma(D[c], GD, gl[c]));
+        if (need_dd) {
+            const FCC_CONST T *DD = S + (size_t)(2 * NTH + j) * ldc;
+#pragma unroll
+            for (int c = 1; c < NC; c++) gl[c] = fma(DD[c], GDD, gl[c]);
+        }
+        if (debug && live) {
+            d.GR[ii * NTH + k] = GR;
+            d.GD[ii * NTH + k] = GD;
+            d.GDD[ii * NTH + k] = GDD;
+        }
+    }
+    // half 1 summed with the rows of nodes 0..19: its odd modes change sign
+#pragma unroll
+    for (int m = 2; m <= K; m += 2) {
+        gl[m] *= sg;
+        gl[K + m] *= sg;
+    }
+#pragma unroll
+    for (int c = 0; c < NC; c++) gl[c] += __shfl_xor(gl[c], 32);
+    fx += __shfl_xor(fx, 32);
+    fy += __shfl_xor(fy, 32);
+    tt += __shfl_xor(tt, 32);
+    if (live && h == 0) {
+        T a[W];
+#pragma unroll
+        for (int c = 0; c < W; c++) a[c] = c < NC ? gl[c] : T(0);
+        row_store<T, W>(a, d.glab + ii * ldc);
+        d.gx[ii] = fx;
+        d.gy[ii] = fy;
+        d.gth[ii] = tt;
+        d.Ebody[ii] = E;
+    }
+}
+
+template <typename T, int K, int NCLS>
+__global__ __launch_bounds__(64) void k_forces_fast2(const Params<T> p, const Ptrs<T> d, int use_status, int debug) {
+    k_forces_fast2_impl<T, K, NCLS>(p, d, use_status, debug, xcd_tile(blockIdx.x, gridDim.x));
+}
+
+template <typename T, int K, int NCLS>
+__global__ __launch_bounds__(64) void k_forces_fast2_ens(const Params<T> *ps, const Ptrs<T> *ds, const Tile *tiles,
+                                                         int use_status, int debug) {
+    const Tile tl = load_uniform(tiles, blockIdx.x);
+    if (tl.rep < 0) return;
+    const FCC_CONST Params<T> &p = as_const(ps)[tl.rep];
+    const FCC_CONST Ptrs<T> &d = as_const(ds)[tl.rep];
+    k_forces_fast2_impl<T, K, NCLS>(p, d, use_status, debug, tl.blk);
 }
 
 }  // namespace fcc

@@ -75,22 +75,6 @@ __device__ __forceinline__ typename MfmaType<BF>::v_t load_op(const uint16_t *p)
     return v;
 }
 
-// Bit-sliced sum of two per-node counters (NPLANE planes each).
-__device__ __forceinline__ void planes_add(uint64_t (&a)[NPLANE], const uint64_t (&b)[NPLANE]) {
-    uint64_t c = 0;
-#pragma unroll
-    for (int q = 0; q < NPLANE; q++) {
-        const uint64_t s = a[q] ^ b[q] ^ c;
-        c = (a[q] & b[q]) | (c & (a[q] ^ b[q]));
-        a[q] = s;
-    }
-}
-
-__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
-    const uint32_t lo = __shfl_xor((uint32_t)v, m), hi = __shfl_xor((uint32_t)(v >> 32), m);
-    return ((uint64_t)hi << 32) | lo;
-}
-
 // Synthesis on MFMA: lane (b, h) supplies lab coefficients 8h..8h+7 of body
 // b (pre-scaled by SYN_SCALE); acc[t] receives tile t of S·(SYN_SCALE·lab)
 // (slots σ = 16t + r). The contour is R = u·acc with u = R0 / SYN_SCALE; the
