@@ -437,7 +437,7 @@ def main():
         ps.close()
 
     flops = forces_flops(sim, tested, inr)
-    fkernel = "k_forces_mfma" if args.precision.startswith("mixed") else "k_forces_fast"
+    fkernel = "k_forces_mfma" if args.precision.startswith("mixed") else "k_forces_fast2"
     achieved = flops / (fms * 1e-3) / 1e12
     valu_frac, mfma_frac = measured_issue(fkernel) if sim.N == 262144 else (None, None)
     upd_bytes = 340 * sim.N  # per body-step, 188 B read + 152 B written
