@@ -440,7 +440,7 @@ def main():
     fkernel = "k_forces_mfma" if args.precision.startswith("mixed") else "k_forces_fast2"
     achieved = flops / (fms * 1e-3) / 1e12
     valu_frac, mfma_frac = measured_issue(fkernel) if sim.N == 262144 else (None, None)
-    upd_bytes = 340 * sim.N  # per body-step, 188 B read + 152 B written
+    upd_bytes = 280 * sim.N  # per body-step, 188 B read + 92 B written
     print(f"[bench] forces {fms * 1e3:.1f} us, update {ums * 1e3:.1f} us (back-to-back launches; "
           f"last step of each chunk in the run: {run_fms * 1e3:.1f} / {run_ums * 1e3:.1f} us), "
           f"pairs tested {tested}, in range {inr}", file=sys.stderr, flush=True)

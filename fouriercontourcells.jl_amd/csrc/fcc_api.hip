@@ -963,11 +963,25 @@ struct Sim final : fcc_sim {
         return FCC_OK;
     }
 
+    // The time step keeps the lab rows as the shape state; the body-frame
+    // coefficients are written to `coef` only for state reads and checkpoints.
+    int materialize_coef() {
+        const unsigned g = blocks(N, 256);
+        switch (KM) {
+        case 5: hipLaunchKernelGGL((k_coef_from_lab<T, 5>), g, 256, 0, stream, p, d); break;
+        case 7: hipLaunchKernelGGL((k_coef_from_lab<T, 7>), g, 256, 0, stream, p, d); break;
+        default: hipLaunchKernelGGL((k_coef_from_lab<T, 15>), g, 256, 0, stream, p, d);
+        }
+        HIPCHK(hipGetLastError());
+        return FCC_OK;
+    }
+
     int get_bodies(fcc_bodies *b) override {
         std::vector<double> x, y;
         std::vector<int32_t> ix, iy, ty, pa;
         std::vector<T> th, r0, co;
         int rc;
+        if ((rc = materialize_coef())) return rc;
         if ((rc = download(x, d.x, N)) || (rc = download(y, d.y, N)) || (rc = download(ix, d.imgx, N)) ||
             (rc = download(iy, d.imgy, N)) || (rc = download(ty, d.type, N)) || (rc = download(pa, d.parent, N)) ||
             (rc = download(th, d.th, N)) || (rc = download(r0, d.R0, N)) || (rc = download(co, d.coef, (size_t)N * ldc)))
@@ -1415,6 +1429,7 @@ struct Sim final : fcc_sim {
         char *out = (char *)buf;
         std::memcpy(out, &h, sizeof h);
         size_t off = sizeof h;
+        if (int rc = materialize_coef()) return rc;
         for (auto &a : ckpt_arrays()) {
             HIPCHK(hipMemcpyAsync(out + off, a.first, a.second, hipMemcpyDeviceToHost, stream));
             off += a.second;

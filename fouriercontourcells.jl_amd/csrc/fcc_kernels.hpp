@@ -316,6 +316,25 @@ __device__ __forceinline__ T lab_frame(const T (&c)[2 * KM + 1], T th, T r0, int
     return r0 * bound;
 }
 
+// Inverse of lab_frame on the coefficient rows: the body-frame coefficients
+// (a_n, b_n) from the lab-frame row l (rotation by -nθ). The lab rows are the
+// stored state; the body-frame coefficients are derived from them where they
+// are needed (the update, body_frame!, state reads).
+template <typename T, int KM>
+__device__ __forceinline__ void lab_to_coef(T (&c)[2 * KM + 1], T th, int K) {
+#pragma clang fp contract(off)
+    T cn[KM + 1], sn[KM + 1];
+    harmonics<T, KM>(th, K, cn, sn);
+#pragma unroll
+    for (int m = 1; m <= KM; m++) {
+        if (m <= K) {
+            const T al = c[m], bl = c[KM + m];
+            c[m] = al * cn[m] + bl * sn[m];
+            c[KM + m] = bl * cn[m] - al * sn[m];
+        }
+    }
+}
+
 // Construction / refresh: enforce_shape! then lab_frame! (src/simulation.jl:102-103).
 template <typename T, int KM, bool F64S>
 __global__ __launch_bounds__(256) void k_init(const Params<T> p, const Ptrs<T> d) {
@@ -364,10 +383,25 @@ __global__ __launch_bounds__(256) void k_body_frame(const Params<T> p, const Ptr
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= p.N) return;
     T gl[NCM], c[NCM], gc[NCM];
+    const T th = d.th[i];
     load_row<T, KM>(gl, d.glab + i * p.ldc, p.K);
-    load_row<T, KM>(c, d.coef + i * p.ldc, p.K);
-    body_frame<T, KM>(gl, c, d.th[i], d.R0[i], p.K, gc);
+    load_row<T, KM>(c, d.lab + i * p.ldc, p.K);
+    lab_to_coef<T, KM>(c, th, p.K);
+    body_frame<T, KM>(gl, c, th, d.R0[i], p.K, gc);
     store_row<T, KM>(gc, d.gcoef + i * p.ldc, p.K);
+}
+
+// Body-frame coefficients of the current state into `coef` (before state
+// reads and checkpoints; the time step itself keeps only the lab rows).
+template <typename T, int KM>
+__global__ __launch_bounds__(256) void k_coef_from_lab(const Params<T> p, const Ptrs<T> d) {
+    constexpr int NCM = 2 * KM + 1;
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= p.N) return;
+    T c[NCM];
+    load_row<T, KM>(c, d.lab + i * p.ldc, p.K);
+    lab_to_coef<T, KM>(c, d.th[i], p.K);
+    store_row<T, KM>(c, d.coef + i * p.ldc, p.K);
 }
 
 // substrate_displacement (src/dynamics.jl:84-99) of body c.
@@ -586,10 +620,10 @@ __device__ __forceinline__ void k_update_impl(const PP &p, const DP &d, int use_
     }
     T gl[NCM], c[NCM], gc[NCM];
     load_coef<T, KM, EXACT>(gl, d.glab + i * p.ldc, K);
-    load_coef<T, KM, EXACT>(c, d.coef + i * p.ldc, K);
+    load_coef<T, KM, EXACT>(c, d.lab + i * p.ldc, K);
+    lab_to_coef<T, KM>(c, th, K);
     const T thn = shape_step<T, KM, F64S>(p, ti, K, gl, c, th, r0, d.gth[i], d2, gc);
     d.th[i] = thn;
-    store_coef<T, KM, EXACT>(d.coef + i * p.ldc, c, K);
     if (debug) store_coef<T, KM, EXACT>(d.gcoef + i * p.ldc, gc, K);
     if (i == 0) d.st->stop = d.st->clock >= d.st->target ? 1 : 0;
     place_body<T, KM, EXACT>(p, d, i, K, c, thn, r0, d0, d1);
