@@ -132,6 +132,9 @@ int fcc_destroy(fcc_sim *sim);
  * and invalidate the neighbour list (the test helper refresh!,
  * test/runtests.jl:9-15, minus the gradient evaluation). */
 int fcc_set_bodies(fcc_sim *sim, const fcc_bodies *bodies);
+/* The time step stores the shape as lab-frame rows; the body-frame `coef`
+ * returned here is derived from them (rotation by -nθ, within rounding of
+ * the coefficients the step computed). */
 int fcc_get_bodies(fcc_sim *sim, fcc_bodies *bodies);
 int fcc_set_dt(fcc_sim *sim, double dt);
 
