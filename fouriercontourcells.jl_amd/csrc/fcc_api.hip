@@ -53,6 +53,8 @@ static inline unsigned blocks(int64_t n, int b) { return (unsigned)((n + b - 1) 
             hipLaunchKernelGGL(KERN, dim3(G), dim3(B), SH, S, __VA_ARGS__);                          \
     } while (0)
 
+constexpr int64_t FOUR_LANES_BELOW = 131072;
+
 // Steps per long graph (fcc_run and ensemble runs replay it for all but the
 // last few steps of a chunk).
 constexpr int GRAPH_STEPS = 8;
@@ -615,8 +617,13 @@ struct Sim final : fcc_sim {
             }
         }
         if (fast_ok()) {
+// Below FOUR_LANES_BELOW bodies (fewer than 4096 waves of 32 bodies, one per
+// SIMD slot of occupancy 4) the FP32 / FP64 kernel takes four lanes per body.
 #define FCC_FAST(KV, NC)                                                                                 \
-    if (nth == 40) FCC_LAUNCH((k_forces_fast2<T, KV, NC>), blocks(N, 32), 64, 0, stream, eb, ee, p, d, use_status, dbg); \
+    if (nth == 40 && N < FOUR_LANES_BELOW)                                                               \
+        FCC_LAUNCH((k_forces_fast2<T, KV, NC, 4>), blocks(N, 16), 64, 0, stream, eb, ee, p, d, use_status, dbg); \
+    else if (nth == 40)                                                                                  \
+        FCC_LAUNCH((k_forces_fast2<T, KV, NC, 2>), blocks(N, 32), 64, 0, stream, eb, ee, p, d, use_status, dbg); \
     else FCC_LAUNCH((k_forces_fast<T, 0, KV, NC>), g, 64, 0, stream, eb, ee, p, d, use_status, dbg);
             if (K == 5) {
                 if (p.ncls <= 1) { FCC_FAST(5, 1) } else { FCC_FAST(5, 2) }

@@ -66,7 +66,10 @@ struct Ensemble final : fcc_ensemble {
     Ptrs<T> *dD = nullptr;
     Tile *tf = nullptr, *tu = nullptr;  // tiles of the forces / update launches
     size_t cap_f = 0, cap_u = 0;
-    unsigned gf = 0, gu = 0;
+    // forces tiles: gf4 of the four-lane FP32 / FP64 kernel (replicas below
+    // FOUR_LANES_BELOW bodies, as each would run alone), then gf of the
+    // other kernel
+    unsigned gf4 = 0, gf = 0, gu = 0;
     std::vector<Params<T>> hP, upP;
     std::vector<Ptrs<T>> hD, upD;
     bool uploaded = false;
@@ -184,8 +187,13 @@ struct Ensemble final : fcc_ensemble {
         }
         if (var == 1) {
 #define FCC_FE(KV, NC)                                                                                        \
-    if (a.nth == 40) FCC_LAUNCH((k_forces_fast2_ens<T, KV, NC>), gf, 64, 0, stream, eb, ee, dP, dD, tf, 2, dbg); \
-    else FCC_LAUNCH((k_forces_fast_ens<T, 0, KV, NC>), gf, 64, 0, stream, eb, ee, dP, dD, tf, 2, dbg);
+    if (a.nth == 40) {                                                                                        \
+        if (gf4) FCC_LAUNCH((k_forces_fast2_ens<T, KV, NC, 4>), gf4, 64, 0, stream, eb, gf ? nullptr : ee, dP, dD, \
+                            tf, 2, dbg);                                                                      \
+        if (gf) FCC_LAUNCH((k_forces_fast2_ens<T, KV, NC, 2>), gf, 64, 0, stream, gf4 ? nullptr : eb, ee, dP, dD, \
+                           tf + gf4, 2, dbg);                                                                 \
+    } else                                                                                                    \
+        FCC_LAUNCH((k_forces_fast_ens<T, 0, KV, NC>), gf, 64, 0, stream, eb, ee, dP, dD, tf, 2, dbg);
             if (a.K == 5) {
                 if (ncls <= 1) { FCC_FE(5, 1) } else { FCC_FE(5, 2) }
             } else {
@@ -264,12 +272,19 @@ struct Ensemble final : fcc_ensemble {
             uploaded = true;
         }
         if (var != tiles_var || lc.update_block != tiles_lc.update_block || lc.xcd_order != tiles_lc.xcd_order) {
-            std::vector<int64_t> sizes(R);
+            std::vector<int64_t> sizes(R), small(R, 0), large(R, 0);
             for (size_t r = 0; r < R; r++) sizes[r] = m[r]->N;
-            // 32 bodies per forces workgroup for the two-lane kernels (MFMA, and the
-            // FP32 / FP64 one at nθ = 40), 64 otherwise
-            const bool two_lane = var == 2 || (var == 1 && m[0]->nth == 40);
-            const std::vector<Tile> a = tile_table(sizes, two_lane ? 32 : 64, lc.xcd_order != 0);
+            // Bodies per forces workgroup: 16 for the four-lane FP32 / FP64 kernel
+            // (replicas below FOUR_LANES_BELOW bodies at nθ = 40, the kernel each
+            // would get alone), 32 for the two-lane kernels (MFMA, and the FP32 /
+            // FP64 one at nθ = 40), 64 otherwise.
+            const bool fast40 = var == 1 && m[0]->nth == 40;
+            for (size_t r = 0; r < R; r++) (fast40 && sizes[r] < FOUR_LANES_BELOW ? small : large)[r] = sizes[r];
+            const int tile = var == 2 || fast40 ? 32 : 64;
+            std::vector<Tile> a = tile_table(small, 16, lc.xcd_order != 0);
+            const size_t n4 = a.size();
+            const std::vector<Tile> a2 = tile_table(large, tile, lc.xcd_order != 0);
+            a.insert(a.end(), a2.begin(), a2.end());
             const std::vector<Tile> b = tile_table(sizes, lc.update_block, lc.xcd_order != 0);
             HIPCHK(hipStreamSynchronize(stream));
             if (a.size() > cap_f) {
@@ -286,7 +301,8 @@ struct Ensemble final : fcc_ensemble {
             }
             HIPCHK(hipMemcpy(tf, a.data(), sizeof(Tile) * a.size(), hipMemcpyHostToDevice));
             HIPCHK(hipMemcpy(tu, b.data(), sizeof(Tile) * b.size(), hipMemcpyHostToDevice));
-            gf = (unsigned)a.size();
+            gf4 = (unsigned)n4;
+            gf = (unsigned)(a.size() - n4);
             gu = (unsigned)b.size();
             tiles_var = var;
             tiles_lc = lc;

@@ -412,36 +412,50 @@ __device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
     return ((uint64_t)hi << 32) | lo;
 }
 
-// The same work with two lanes per body (nθ = 40): 32 bodies per wave, lane
-// (b, h) takes nodes 20h..20h+19 of body b in the node passes and neighbour
-// slots q ≡ h (mod 2) over the whole circle in the contact pass, as the
-// mixed-precision kernel does. Node j + 20 sits at φ_j + π, where the basis
-// rows are those of node j with the odd modes negated ((-1)^n cos nφ, …): lane
-// half 1 negates the odd modes of its lab row once, so both halves read the
-// same wave-uniform basis rows through the scalar cache, and the pullback of
-// half 1 takes the same signs before the halves are summed.
-template <typename T, int K, int NCLS, typename PP, typename DP>
+// Mode n's (cos, sin) pair of a coefficient row, in place, for the turn by
+// nα with nα a multiple r of π/2 (r = 0..3): cos nα, sin nα ∈ {0, ±1}, so the
+// rotation only swaps and negates (exact). `inv` applies the inverse turn.
+template <typename T>
+__device__ __forceinline__ void quarter_turn(T &c, T &s, int r, bool inv) {
+    const T a = c, b = s;
+    if (inv && (r & 1)) r ^= 2;
+    c = r == 0 ? a : r == 1 ? b : r == 2 ? -a : -b;
+    s = r == 0 ? b : r == 1 ? -a : r == 2 ? -b : a;
+}
+
+// The same work with LPB = 2 or 4 lanes per body (nθ = 40): 64 / LPB bodies
+// per wave, lane (b, h) takes nodes 40h/LPB .. 40(h+1)/LPB - 1 of body b in
+// the node passes and neighbour slots q ≡ h (mod LPB) over the whole circle in
+// the contact pass (the mixed-precision kernel's layout for LPB = 2; LPB = 4
+// doubles the waves of small systems). Node j + 40h/LPB sits at φ_j + hα,
+// α = 2π/LPB, where the basis rows are those of node j with each mode's (cos,
+// sin) pair turned by n·hα, a multiple of π/2: lane group h turns its lab row
+// once, so all groups read the same wave-uniform basis rows through the scalar
+// cache, and its pullback sums take the inverse turn before the groups are
+// added.
+template <typename T, int K, int NCLS, int LPB, typename PP, typename DP>
 __device__ __forceinline__ void k_forces_fast2_impl(const PP &p, const DP &d, int use_status, int debug,
                                                      int64_t blk) {
-    constexpr int NTH = 40, NH2 = 20;
+    static_assert(LPB == 2 || LPB == 4, "two or four lanes per body");
+    constexpr int NTH = 40, NB = 64 / LPB, NH2 = NTH / LPB;
     constexpr int NC = 2 * K + 1;
     constexpr int NCL = NCLS > 0 ? NCLS : 1;
     constexpr int W = row_width<NC, T>();
-    __shared__ T sR[NTH][32];
+    __shared__ T sR[NTH][NB];
     __shared__ T scs[2][NTH];
     const int lane = threadIdx.x;
-    const int b = lane & 31, h = lane >> 5;
+    const int b = lane & (NB - 1), h = lane / NB;
     for (int k = lane; k < NTH; k += 64) {
         scs[0][k] = d.cs[2 * k];
         scs[1][k] = d.cs[2 * k + 1];
     }
-    const int64_t i = (int64_t)blk * 32 + b;
+    const int64_t i = (int64_t)blk * NB + b;
     if (forces_frozen(use_status, d.st, blk, lane)) return;  // uniform over the grid
     const bool live = i < p.N;
     const int64_t ii = live ? i : 0;
     const FCC_CONST T *S = as_const(d.Sc);
     const int ldc = p.ldc;
-    const T sg = h ? T(-1) : T(1);  // sign of the odd modes at node j + 20h
+    const int qt = h * (4 / LPB);  // quarter turns of this lane group's first node
     T lab[NC];
     {
         T a[W];
@@ -449,10 +463,7 @@ __device__ __forceinline__ void k_forces_fast2_impl(const PP &p, const DP &d, in
 #pragma unroll
         for (int c = 0; c < NC; c++) lab[c] = a[c];
 #pragma unroll
-        for (int m = 2; m <= K; m += 2) {  // mode m + 1 odd
-            lab[m] *= sg;
-            lab[K + m] *= sg;
-        }
+        for (int m = 1; m <= K; m++) quarter_turn<T>(lab[m], lab[K + m], ((m + 1) * qt) & 3, false);
     }
     const T r0 = live ? d.R0[ii] : T(1);
     const int ti = live ? d.type[ii] : 0;
@@ -469,8 +480,11 @@ __device__ __forceinline__ void k_forces_fast2_impl(const PP &p, const DP &d, in
         Qp += f_sqrt(R * R + dR * dR);
         if (need_dd) Qb += term_q<T>(FCC_TERM_BENDING, R, dR, ddR);
     }
-    Qp += __shfl_xor(Qp, 32);
-    Qb += __shfl_xor(Qb, 32);
+#pragma unroll
+    for (int o = NB; o < 64; o <<= 1) {
+        Qp += __shfl_xor(Qp, o);
+        Qb += __shfl_xor(Qb, o);
+    }
     T Wp = T(0), Wb = T(0), Et[MAXB];
 #pragma unroll
     for (int q = 0; q < MAXB; q++) {
@@ -510,13 +524,13 @@ __device__ __forceinline__ void k_forces_fast2_impl(const PP &p, const DP &d, in
         // next neighbour's index one iteration ahead
         const int32_t *nrow = d.nidx + ii;
         int jn = h < cnt ? nrow[(int64_t)h * p.N] : 0;
-        for (int q = h; q < cnt; q += 2) {
+        for (int q = h; q < cnt; q += LPB) {
             const int j = jn;
             const double xj = d.x[j], yj = d.y[j];
             const T rmaxj = d.rmax[j], rminj = d.rmin[j], r0j = d.R0[j];
             T labj[W];
             row_load<T, W>(labj, d.lab + (int64_t)j * ldc);
-            if (q + 2 < cnt) jn = nrow[(int64_t)(q + 2) * p.N];
+            if (q + LPB < cnt) jn = nrow[(int64_t)(q + LPB) * p.N];
             int cls = 0;
             if (!p.cls_uniform) {
                 cls = p.ccls[pi == d.parent[j] ? 1 : 0][ti][d.type[j]];
@@ -596,15 +610,17 @@ __device__ __forceinline__ void k_forces_fast2_impl(const PP &p, const DP &d, in
             }
         }
     }
-    // both lanes of a body: add the partner's counters
+    // the lanes of a body: add the other groups' counters
 #pragma unroll
-    for (int c = 0; c < NCL; c++) {
-        uint64_t other[NPLANE];
+    for (int o = NB; o < 64; o <<= 1)
 #pragma unroll
-        for (int q = 0; q < NPLANE; q++) other[q] = shfl_xor64(pl[c][q], 32);
-        planes_add(pl[c], other);
-    }
-    // the lane's 20 nodes of each counter plane at bits 0..19
+        for (int c = 0; c < NCL; c++) {
+            uint64_t other[NPLANE];
+#pragma unroll
+            for (int q = 0; q < NPLANE; q++) other[q] = shfl_xor64(pl[c][q], o);
+            planes_add(pl[c], other);
+        }
+    // the lane's nodes of each counter plane at bits 0..NH2-1
     uint32_t plh[NCL][NPLANE];
 #pragma unroll
     for (int c = 0; c < NCL; c++)
@@ -677,17 +693,17 @@ __device__ __forceinline__ void k_forces_fast2_impl(const PP &p, const DP &d, in
             d.GDD[ii * NTH + k] = GDD;
         }
     }
-    // half 1 summed with the rows of nodes 0..19: its odd modes change sign
+    // the group summed with the rows of nodes 0..NH2-1: inverse turn per mode
 #pragma unroll
-    for (int m = 2; m <= K; m += 2) {
-        gl[m] *= sg;
-        gl[K + m] *= sg;
+    for (int m = 1; m <= K; m++) quarter_turn<T>(gl[m], gl[K + m], ((m + 1) * qt) & 3, true);
+#pragma unroll
+    for (int o = NB; o < 64; o <<= 1) {
+#pragma unroll
+        for (int c = 0; c < NC; c++) gl[c] += __shfl_xor(gl[c], o);
+        fx += __shfl_xor(fx, o);
+        fy += __shfl_xor(fy, o);
+        tt += __shfl_xor(tt, o);
     }
-#pragma unroll
-    for (int c = 0; c < NC; c++) gl[c] += __shfl_xor(gl[c], 32);
-    fx += __shfl_xor(fx, 32);
-    fy += __shfl_xor(fy, 32);
-    tt += __shfl_xor(tt, 32);
     if (live && h == 0) {
         T a[W];
 #pragma unroll
@@ -700,19 +716,19 @@ __device__ __forceinline__ void k_forces_fast2_impl(const PP &p, const DP &d, in
     }
 }
 
-template <typename T, int K, int NCLS>
+template <typename T, int K, int NCLS, int LPB>
 __global__ __launch_bounds__(64) void k_forces_fast2(const Params<T> p, const Ptrs<T> d, int use_status, int debug) {
-    k_forces_fast2_impl<T, K, NCLS>(p, d, use_status, debug, xcd_tile(blockIdx.x, gridDim.x));
+    k_forces_fast2_impl<T, K, NCLS, LPB>(p, d, use_status, debug, xcd_tile(blockIdx.x, gridDim.x));
 }
 
-template <typename T, int K, int NCLS>
+template <typename T, int K, int NCLS, int LPB>
 __global__ __launch_bounds__(64) void k_forces_fast2_ens(const Params<T> *ps, const Ptrs<T> *ds, const Tile *tiles,
                                                          int use_status, int debug) {
     const Tile tl = load_uniform(tiles, blockIdx.x);
     if (tl.rep < 0) return;
     const FCC_CONST Params<T> &p = as_const(ps)[tl.rep];
     const FCC_CONST Ptrs<T> &d = as_const(ds)[tl.rep];
-    k_forces_fast2_impl<T, K, NCLS>(p, d, use_status, debug, tl.blk);
+    k_forces_fast2_impl<T, K, NCLS, LPB>(p, d, use_status, debug, tl.blk);
 }
 
 }  // namespace fcc
