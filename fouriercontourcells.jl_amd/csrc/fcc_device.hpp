@@ -215,6 +215,13 @@ __device__ __forceinline__ T min_image(T d, T L) {
     return d - L * rint(d / L);
 }
 
+// min_image for |d| < L/4 without the division: d/L rounds into (-1/4, 1/4),
+// rint gives ±0 and d − L·(±0) = d (up to the sign of a zero d, which a
+// squared offset does not see); other offsets take min_image.
+__device__ __forceinline__ double min_image_small(double d, double L) {
+    return fabs(d) < 0.25 * L ? d : min_image(d, L);
+}
+
 // minimum_image of a neighbour offset |d| < L/2 - cutoff or > L/2 + cutoff
 // (boxes that are not small): d·(1/L) rounds to the integer d/L rounds to, so
 // the result equals min_image's.
@@ -222,9 +229,16 @@ __device__ __forceinline__ double min_image_nb(double d, double L, double invL, 
     return small ? min_image(d, L) : d - L * rint(d * invL);
 }
 
-// wrap (src/box.jl:21-30).
+// wrap (src/box.jl:21-30). For 0 < x < L the correctly rounded x/L is at
+// most 1 − 2^-53 (the largest double below L is L(1 − δ) with δ ≥ 2^-53), so
+// s = 0 and x − 0·L = x: that case returns x without the division
+// (bit-identical).
 __device__ __forceinline__ double wrap(double x, double L, int32_t &shift) {
 #pragma clang fp contract(off)
+    if (x > 0.0 && x < L) {
+        shift = 0;
+        return x;
+    }
     double s = floor(x / L);
     double xp = x - s * L;
     if (xp >= L) {

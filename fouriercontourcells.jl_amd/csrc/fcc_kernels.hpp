@@ -238,8 +238,9 @@ __global__ __launch_bounds__(64) void k_forces_ens(const Params<T> *ps, const Pt
 
 // ----------------------------------------------------------- per-body math
 
-// (cos nθ, sin nθ) for n = 2..K+1: direct in FP64, by rotation from (cos θ,
-// sin θ) in FP32 (geometry.jl:271-273 recurrence).
+// (cos nθ, sin nθ) for n = 2..K+1 in slots 1..K: direct in FP64, by rotation
+// from (cos θ, sin θ) in FP32 (geometry.jl:271-273 recurrence), where slot 0
+// also receives (cos θ, sin θ) (FP64 leaves slot 0 unset).
 template <typename T, int KM>
 __device__ __forceinline__ void harmonics(T th, int K, T (&cn)[KM + 1], T (&sn)[KM + 1]) {
 #pragma clang fp contract(off)
@@ -250,6 +251,8 @@ __device__ __forceinline__ void harmonics(T th, int K, T (&cn)[KM + 1], T (&sn)[
     } else {
         T c1, s1;
         sincosf(th, &s1, &c1);
+        cn[0] = c1;
+        sn[0] = s1;
         T c = c1, s = s1;
 #pragma unroll
         for (int m = 1; m <= KM; m++) {
@@ -321,10 +324,9 @@ __device__ __forceinline__ T lab_frame(const T (&c)[2 * KM + 1], T th, T r0, int
 // stored state; the body-frame coefficients are derived from them where they
 // are needed (the update, body_frame!, state reads).
 template <typename T, int KM>
-__device__ __forceinline__ void lab_to_coef(T (&c)[2 * KM + 1], T th, int K) {
+__device__ __forceinline__ void lab_to_coef(T (&c)[2 * KM + 1], const T (&cn)[KM + 1], const T (&sn)[KM + 1],
+                                            int K) {
 #pragma clang fp contract(off)
-    T cn[KM + 1], sn[KM + 1];
-    harmonics<T, KM>(th, K, cn, sn);
 #pragma unroll
     for (int m = 1; m <= KM; m++) {
         if (m <= K) {
@@ -333,6 +335,13 @@ __device__ __forceinline__ void lab_to_coef(T (&c)[2 * KM + 1], T th, int K) {
             c[KM + m] = bl * cn[m] - al * sn[m];
         }
     }
+}
+
+template <typename T, int KM>
+__device__ __forceinline__ void lab_to_coef(T (&c)[2 * KM + 1], T th, int K) {
+    T cn[KM + 1], sn[KM + 1];
+    harmonics<T, KM>(th, K, cn, sn);
+    lab_to_coef<T, KM>(c, cn, sn, K);
 }
 
 // Construction / refresh: enforce_shape! then lab_frame! (src/simulation.jl:102-103).
@@ -354,15 +363,40 @@ __global__ __launch_bounds__(256) void k_init(const Params<T> p, const Ptrs<T> d
     store_row<T, KM>(l, d.lab + i * p.ldc, p.K);
 }
 
-// body_frame! (src/gradients.jl:347-362): gcoef from glab at the current state.
+// Quotients x / d of many numerators by one divisor, each correctly rounded
+// (the bits of `x / d`). FP64: the division itself. FP32: one binary64
+// reciprocal rd = RN(1/d) shared by all numerators, then RN32(x·rd). The
+// product is within 2^-51.9 (relative) of q = x/d. A quotient of two 24-bit
+// significands is never a binary32 rounding midpoint m, and q − m =
+// (x − m·d)/d with x − m·d a nonzero multiple of 2^(E−24+e_d) for q in
+// [2^E, 2^(E+1)) and d in [2^e_d, 2^(e_d+1)), so |q − m| ≥ 2^(E−48), which is
+// more than the product's error (< 2^(E−50.9)): no midpoint lies between q and
+// the product, and both round to the same binary32 value (the subnormal range
+// keeps a margin of the same kind; zero, infinite and NaN operands give the
+// IEEE results).
+template <typename T>
+struct SharedDivisor {
+    T d;
+    __device__ __forceinline__ explicit SharedDivisor(T d_) : d(d_) {}
+    __device__ __forceinline__ T div(T x) const { return x / d; }
+};
+template <>
+struct SharedDivisor<float> {
+    double rd;
+    __device__ __forceinline__ explicit SharedDivisor(float d) : rd(1.0 / (double)d) {}
+    __device__ __forceinline__ float div(float x) const { return (float)((double)x * rd); }
+};
+
+// body_frame! (src/gradients.jl:347-362): gcoef from glab at the current
+// state, given the harmonics of θ.
 template <typename T, int KM>
-__device__ __forceinline__ void body_frame(const T (&gl)[2 * KM + 1], const T (&c)[2 * KM + 1], T th, T r0,
-                                           int K, T (&gc)[2 * KM + 1]) {
+__device__ __forceinline__ void body_frame(const T (&gl)[2 * KM + 1], const T (&c)[2 * KM + 1],
+                                           const T (&cn)[KM + 1], const T (&sn)[KM + 1], T r0, int K,
+                                           T (&gc)[2 * KM + 1]) {
 #pragma clang fp contract(off)
-    T cn[KM + 1], sn[KM + 1];
-    harmonics<T, KM>(th, K, cn, sn);
     const T ga0 = r0 * gl[0];
     const T a0 = c[0];
+    const SharedDivisor<T> dv(T(2) * a0);
     gc[0] = ga0;
 #pragma unroll
     for (int m = 1; m <= KM; m++) {
@@ -371,8 +405,8 @@ __device__ __forceinline__ void body_frame(const T (&gl)[2 * KM + 1], const T (&
         if (m <= K) {
             T ga = r0 * gl[m], gb = r0 * gl[KM + m];
             T a = c[m], b = c[KM + m];
-            gc[m] = ga * cn[m] + gb * sn[m] - ga0 * a / (T(2) * a0);
-            gc[KM + m] = (-ga) * sn[m] + gb * cn[m] - ga0 * b / (T(2) * a0);
+            gc[m] = ga * cn[m] + gb * sn[m] - dv.div(ga0 * a);
+            gc[KM + m] = (-ga) * sn[m] + gb * cn[m] - dv.div(ga0 * b);
         }
     }
 }
@@ -386,8 +420,10 @@ __global__ __launch_bounds__(256) void k_body_frame(const Params<T> p, const Ptr
     const T th = d.th[i];
     load_row<T, KM>(gl, d.glab + i * p.ldc, p.K);
     load_row<T, KM>(c, d.lab + i * p.ldc, p.K);
-    lab_to_coef<T, KM>(c, th, p.K);
-    body_frame<T, KM>(gl, c, th, d.R0[i], p.K, gc);
+    T cn[KM + 1], sn[KM + 1];
+    harmonics<T, KM>(th, p.K, cn, sn);
+    lab_to_coef<T, KM>(c, cn, sn, p.K);
+    body_frame<T, KM>(gl, c, cn, sn, d.R0[i], p.K, gc);
     store_row<T, KM>(gc, d.gcoef + i * p.ldc, p.K);
 }
 
@@ -477,42 +513,87 @@ __global__ __launch_bounds__(256) void k_displacement_ens(const Params<T> *ps, c
 // does not contract): the same operations give the same bits wherever they
 // are inlined.
 
-// Displacement of a body without parent drag (src/dynamics.jl:84-99):
-// self-propulsion along θ, mobility times the centre force, translational
-// noise from Philox block 0 of draw `draw` (src/dynamics.jl:258-277).
+// Parameters of one body type used by the update, taken from the uniform
+// parameter block by a select chain over the types (no per-lane load that
+// would wait on the body's type).
+template <typename T>
+struct TypeParams {
+    T v0, mur, Dt, Dr, muth, muab;
+};
+
+// (readfirstlane keeps the compiler from folding the chain back into a load
+// indexed by the type.)
+template <typename T>
+__device__ __forceinline__ T opaque_uniform(T v) {
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+    } else {
+        const uint64_t u = __builtin_bit_cast(uint64_t, v);
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)u);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(u >> 32));
+        return __builtin_bit_cast(T, ((uint64_t)hi << 32) | lo);
+    }
+}
+
+template <typename T, typename A>
+__device__ __forceinline__ T type_param(const A &a, int ti) {
+    T at[FCC_MAX_TYPES];
+#pragma unroll
+    for (int t = 0; t < FCC_MAX_TYPES; t++) at[t] = opaque_uniform<T>(a[t]);
+    T v = at[0];
+#pragma unroll
+    for (int t = 1; t < FCC_MAX_TYPES; t++) v = ti == t ? at[t] : v;
+    return v;
+}
+
 template <typename T, typename PP>
-__device__ __forceinline__ void body_displacement(const PP &p, int64_t i, int ti, T th, T gx, T gy,
-                                                  uint64_t draw, T &d0, T &d1) {
+__device__ __forceinline__ TypeParams<T> type_params(const PP &p, int ti) {
+    TypeParams<T> tp;
+    tp.v0 = type_param<T>(p.v0, ti);
+    tp.mur = type_param<T>(p.mur, ti);
+    tp.Dt = type_param<T>(p.Dt, ti);
+    tp.Dr = type_param<T>(p.Dr, ti);
+    tp.muth = type_param<T>(p.muth, ti);
+    tp.muab = type_param<T>(p.muab, ti);
+    return tp;
+}
+
+// Displacement of a body without parent drag (src/dynamics.jl:84-99):
+// self-propulsion along θ (s, co = sin θ, cos θ), mobility times the centre
+// force, translational noise from Philox block 0 of draw `draw`
+// (src/dynamics.jl:258-277).
+template <typename T, typename PP>
+__device__ __forceinline__ void body_displacement(const PP &p, int64_t i, const TypeParams<T> &tp, T s, T co,
+                                                  T gx, T gy, uint64_t draw, T &d0, T &d1) {
 #pragma clang fp contract(off)
-    T s, co;
-    fsincos(th, &s, &co);
     T x1 = T(0), x2 = T(0);
-    if (p.Dt[ti] != T(0)) philox_pair<T>(p.key0, p.key1, (uint32_t)(i + 1), draw, 0, x1, x2);
-    const T st = sqrt(T(2) * p.Dt[ti] * p.dt);
-    d0 = p.dt * (p.v0[ti] * co - p.mur[ti] * gx) + st * x1;
-    d1 = p.dt * (p.v0[ti] * s - p.mur[ti] * gy) + st * x2;
+    if (tp.Dt != T(0)) philox_pair<T>(p.key0, p.key1, (uint32_t)(i + 1), draw, 0, x1, x2);
+    const T st = sqrt(T(2) * tp.Dt * p.dt);
+    d0 = p.dt * (tp.v0 * co - tp.mur * gx) + st * x1;
+    d1 = p.dt * (tp.v0 * s - tp.mur * gy) + st * x2;
 }
 
 // Rotational noise increment (Philox block 1 of draw `draw`).
 template <typename T, typename PP>
-__device__ __forceinline__ T rotation_noise(const PP &p, int64_t i, int ti, uint64_t draw) {
+__device__ __forceinline__ T rotation_noise(const PP &p, int64_t i, const TypeParams<T> &tp, uint64_t draw) {
 #pragma clang fp contract(off)
     T x3 = T(0), x4;
-    if (p.Dr[ti] != T(0)) philox_pair<T>(p.key0, p.key1, (uint32_t)(i + 1), draw, 1, x3, x4);
-    return sqrt(T(2) * p.Dr[ti] * p.dt) * x3;
+    if (tp.Dr != T(0)) philox_pair<T>(p.key0, p.key1, (uint32_t)(i + 1), draw, 1, x3, x4);
+    return sqrt(T(2) * tp.Dr * p.dt) * x3;
 }
 
 // Body-frame gradient, orientation and shape update with the shape
 // constraint (src/dynamics.jl:310-333, src/gradients.jl:347-362; the
-// constraint in FP64 when F64S). Returns the new θ; c holds the new
-// coefficients and gc the body-frame gradient.
+// constraint in FP64 when F64S), given the harmonics of θ. Returns the new θ;
+// c holds the new coefficients and gc the body-frame gradient.
 template <typename T, int KM, bool F64S, typename PP>
-__device__ __forceinline__ T shape_step(const PP &p, int ti, int K, const T (&gl)[2 * KM + 1], T (&c)[2 * KM + 1],
-                                        T th, T r0, T gth, T d2, T (&gc)[2 * KM + 1]) {
+__device__ __forceinline__ T shape_step(const PP &p, const TypeParams<T> &tp, int K, const T (&gl)[2 * KM + 1],
+                                        T (&c)[2 * KM + 1], const T (&cn)[KM + 1], const T (&sn)[KM + 1], T th,
+                                        T r0, T gth, T d2, T (&gc)[2 * KM + 1]) {
 #pragma clang fp contract(off)
-    body_frame<T, KM>(gl, c, th, r0, K, gc);
-    const T thn = th - p.dt * p.muth[ti] * gth + d2;
-    const T h = p.dt * p.muab[ti];
+    body_frame<T, KM>(gl, c, cn, sn, r0, K, gc);
+    const T thn = th - p.dt * tp.muth * gth + d2;
+    const T h = p.dt * tp.muab;
 #pragma unroll
     for (int m = 1; m <= KM; m++) {
         c[m] -= h * gc[m];
@@ -527,18 +608,20 @@ __device__ __forceinline__ T shape_step(const PP &p, int ti, int K, const T (&gl
 
 // Position wrap in FP64 (src/box.jl:21-30), the image counters, the lab
 // frame of the new state (src/geometry.jl:208-227) and the Verlet skin check
-// (src/neighbors.jl:409-416) of body i, given its new coefficients c and
-// orientation thn and its displacement (d0, d1).
+// (src/neighbors.jl:409-416) of body i, given its position (x, y), its new
+// coefficients c and orientation thn, its displacement (d0, d1) and the
+// skin reference (x0, y0, rmax0).
 template <typename T, int KM, bool EXACT, typename PP, typename DP>
 __device__ __forceinline__ void place_body(const PP &p, const DP &d, int64_t i, int K, const T (&c)[2 * KM + 1],
-                                           T thn, T r0, T d0, T d1) {
+                                           T thn, T r0, T d0, T d1, double x, double y, double x0, double y0,
+                                           T rmax0) {
 #pragma clang fp contract(off)
     constexpr int NCM = 2 * KM + 1;
     constexpr int W = row_width<NCM, T>();
     T l[NCM];
     int32_t sx, sy;
-    const double xw = wrap(d.x[i] + (double)d0, p.Lx, sx);
-    const double yw = wrap(d.y[i] + (double)d1, p.Ly, sy);
+    const double xw = wrap(x + (double)d0, p.Lx, sx);
+    const double yw = wrap(y + (double)d1, p.Ly, sy);
     T rmn;
     const T rm = lab_frame<T, KM>(c, thn, r0, K, l, rmn);
     d.x[i] = xw;
@@ -556,9 +639,9 @@ __device__ __forceinline__ void place_body(const PP &p, const DP &d, int64_t i, 
         store_row<T, KM>(l, d.lab + i * p.ldc, K);
     }
     if (p.skin_check) {
-        const double dx = min_image(xw - d.x0[i], p.Lx);
-        const double dy = min_image(yw - d.y0[i], p.Ly);
-        const double dr = (double)rm - (double)d.rmax0[i];
+        const double dx = min_image_small(xw - x0, p.Lx);  // only dx², dy² are used
+        const double dy = min_image_small(yw - y0, p.Ly);
+        const double dr = (double)rm - (double)rmax0;
         const double delta = sqrt(dx * dx + dy * dy) + (dr > 0.0 ? dr : 0.0);
         // one store per wave whose bodies tripped (not one per body)
         const uint64_t trips = __ballot(2.0 * delta > (double)p.skin);
@@ -595,38 +678,93 @@ __device__ __forceinline__ void load_coef(T (&c)[2 * KM + 1], const T *row, int 
     }
 }
 
+// Per-body inputs of the update.
+template <typename T, int KM>
+struct BodyIn {
+    int ti;
+    T th, r0, gth;
+    T f0, f1, f2;  // centre force (gx, gy), or the displacement (f2 unused without it)
+    double x, y, x0, y0;
+    T rmax0;
+    T gl[2 * KM + 1], c[2 * KM + 1];
+};
+
+// Issues every load of body i, before any branch on loaded data: one memory
+// round trip covers the body's whole state (behind the noise draws, which
+// branch on the type's diffusion constants, each load would wait apart).
+template <typename T, int KM, bool EXACT, typename PP, typename DP>
+__device__ __forceinline__ void load_body(const PP &p, const DP &d, int64_t i, int K, int use_disp,
+                                          BodyIn<T, KM> &b) {
+    b.ti = d.type[i];
+    b.th = d.th[i];
+    b.r0 = d.R0[i];
+    b.gth = d.gth[i];
+    b.x = d.x[i];
+    b.y = d.y[i];
+    const T *const q0 = use_disp ? d.disp + 3 * i : d.gx + i;
+    const T *const q1 = use_disp ? d.disp + 3 * i + 1 : d.gy + i;
+    const T *const q2 = use_disp ? d.disp + 3 * i + 2 : d.gx + i;
+    b.f0 = *q0;
+    b.f1 = *q1;
+    b.f2 = *q2;
+    load_coef<T, KM, EXACT>(b.gl, d.glab + i * p.ldc, K);
+    load_coef<T, KM, EXACT>(b.c, d.lab + i * p.ldc, K);
+    b.x0 = d.x0[i];  // the skin reference is allocated with the neighbour list
+    b.y0 = d.y0[i];
+    b.rmax0 = d.rmax0[i];
+}
+
+// The update of body i from its loaded inputs: displacement and noise, θ and
+// shape step, wrap, lab frame, skin check and stores. sin θ, cos θ and the
+// harmonics of θ are computed once for the displacement, the body-frame
+// coefficients and the body-frame gradient.
+template <typename T, int KM, bool EXACT, bool F64S, typename PP, typename DP>
+__device__ __forceinline__ void finish_body(const PP &p, const DP &d, int64_t i, int K, int use_disp, int debug,
+                                            uint64_t draw, BodyIn<T, KM> &b) {
+    constexpr int NCM = 2 * KM + 1;
+    const TypeParams<T> tp = type_params<T>(p, b.ti);
+    T cn[KM + 1], sn[KM + 1];
+    harmonics<T, KM>(b.th, K, cn, sn);
+    T d0, d1, d2;
+    if (use_disp) {
+        d0 = b.f0;
+        d1 = b.f1;
+        d2 = b.f2;
+    } else {
+        T s, co;
+        if constexpr (sizeof(T) == 4) {
+            s = sn[0];
+            co = cn[0];
+        } else {
+            fsincos(b.th, &s, &co);
+        }
+        body_displacement<T>(p, i, tp, s, co, b.f0, b.f1, draw, d0, d1);
+        d2 = rotation_noise<T>(p, i, tp, draw);
+    }
+    T gc[NCM];
+    lab_to_coef<T, KM>(b.c, cn, sn, K);
+    const T thn = shape_step<T, KM, F64S>(p, tp, K, b.gl, b.c, cn, sn, b.th, b.r0, b.gth, d2, gc);
+    d.th[i] = thn;
+    if (debug) store_coef<T, KM, EXACT>(d.gcoef + i * p.ldc, gc, K);
+    if (i == 0) d.st->stop = d.st->clock >= d.st->target ? 1 : 0;
+    place_body<T, KM, EXACT>(p, d, i, K, b.c, thn, b.r0, d0, d1, b.x, b.y, b.x0, b.y0, b.rmax0);
+}
+
 // Fused Euler–Maruyama update (src/dynamics.jl:310-333) with the
 // displacement and Philox noise of src/dynamics.jl:84-99,258-277 (bodies
 // without parent drag), the body-frame gradient, the shape constraint (FP64
 // when F64S), position wrapping in FP64, the lab frame of the new state and
-// the Verlet skin check (src/neighbors.jl:409-416).
+// the Verlet skin check (src/neighbors.jl:409-416); one body per lane.
 template <typename T, int KM, bool EXACT, bool F64S, int UB, typename PP, typename DP>
 __device__ __forceinline__ void k_update_impl(const PP &p, const DP &d, int use_disp, int debug, int64_t blk) {
-    constexpr int NCM = 2 * KM + 1;
     const int64_t i = (int64_t)blk * (UB ? UB : (int)blockDim.x) + threadIdx.x;
-    if (i >= p.N || d.st->update != 0) return;
-    const uint64_t draw = (uint64_t)(d.st->clock - 1);
+    if (i >= p.N) return;
     const int K = EXACT ? KM : p.K;
-    const int ti = d.type[i];
-    const T th = d.th[i], r0 = d.R0[i];
-    T d0, d1, d2;
-    if (use_disp) {
-        d0 = d.disp[3 * i];
-        d1 = d.disp[3 * i + 1];
-        d2 = d.disp[3 * i + 2];
-    } else {
-        body_displacement<T>(p, i, ti, th, d.gx[i], d.gy[i], draw, d0, d1);
-        d2 = rotation_noise<T>(p, i, ti, draw);
-    }
-    T gl[NCM], c[NCM], gc[NCM];
-    load_coef<T, KM, EXACT>(gl, d.glab + i * p.ldc, K);
-    load_coef<T, KM, EXACT>(c, d.lab + i * p.ldc, K);
-    lab_to_coef<T, KM>(c, th, K);
-    const T thn = shape_step<T, KM, F64S>(p, ti, K, gl, c, th, r0, d.gth[i], d2, gc);
-    d.th[i] = thn;
-    if (debug) store_coef<T, KM, EXACT>(d.gcoef + i * p.ldc, gc, K);
-    if (i == 0) d.st->stop = d.st->clock >= d.st->target ? 1 : 0;
-    place_body<T, KM, EXACT>(p, d, i, K, c, thn, r0, d0, d1);
+    BodyIn<T, KM> b;
+    load_body<T, KM, EXACT>(p, d, i, K, use_disp, b);
+    if (d.st->update != 0) return;
+    const uint64_t draw = (uint64_t)(d.st->clock - 1);
+    finish_body<T, KM, EXACT, F64S>(p, d, i, K, use_disp, debug, draw, b);
 }
 
 template <typename T, int KM, bool EXACT, bool F64S>
