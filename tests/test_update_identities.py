@@ -1,4 +1,6 @@
-"""The FP32 update divides many numerators by one divisor (the body-frame
+"""Identities the update kernel relies on to skip work bit for bit.
+
+The FP32 update divides many numerators by one divisor (the body-frame
 gradient's ga0·a / (2·a0), src/gradients.jl:347-362) as RN32(x · RN64(1/d)):
 one binary64 reciprocal shared by all numerators. The kernels rely on that
 being bit for bit the IEEE binary32 quotient x / d. This checks the identity on
@@ -59,3 +61,39 @@ def test_hard_cases():
                          rng.uniform(0.25, 4.0, m).astype(np.float32)])
     with np.errstate(divide="ignore", over="ignore", invalid="ignore", under="ignore"):
         assert same_bits(shared(xe, de), ieee(xe, de))
+
+
+# ---- the FP64 position wrap (src/box.jl:21-30) returns x unchanged for 0 < x < L
+# without evaluating floor(x / L); that relies on x / L rounding below 1 there.
+
+def ref_wrap(x, L):
+    s = np.floor(x / L)
+    xp = x - s * L
+    over = xp >= L
+    return np.where(over, 0.0, xp), np.where(over, s + 1.0, s)
+
+
+def test_wrap_inside_the_box_is_identity():
+    rng = np.random.default_rng(11)
+    Ls = np.concatenate([rng.uniform(1.0, 2000.0, 200), [975.168, 844.521, 62.531, 1.0, 2.0, 1024.0, 3.0]])
+    for L in Ls:
+        top = np.nextafter(L, 0.0)
+        x = np.concatenate([rng.uniform(0.0, L, 2000), [top, np.nextafter(top, 0.0), 5e-324, 1e-300,
+                                                        np.nextafter(L / 2, 0.0), L / 2]])
+        x = x[(x > 0.0) & (x < L)]
+        xp, s = ref_wrap(x, L)
+        assert np.all(s == 0.0)
+        assert np.array_equal(xp.view(np.uint64), x.view(np.uint64))
+
+
+# ---- the skin check's minimum image (src/box.jl:14) is d itself for |d| < L/4
+# (only d² is used, so the sign of a zero d does not matter)
+
+def test_min_image_small_offsets():
+    rng = np.random.default_rng(12)
+    for L in np.concatenate([rng.uniform(1.0, 2000.0, 100), [975.168, 4.0]]):
+        q = 0.25 * L
+        d = np.concatenate([rng.uniform(-q, q, 5000), [np.nextafter(q, 0.0), -np.nextafter(q, 0.0), 0.0, 1e-300]])
+        d = d[np.abs(d) < q]
+        full = d - L * np.rint(d / L)
+        assert np.array_equal(full * full, d * d)
