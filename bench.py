@@ -305,11 +305,14 @@ def bench_ensemble(steps, warmup, independent_steps, workers=16, independent=Tru
     launch, tune_ms = ens.autotune(20)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    fms, ums = ens.run_timed(steps)
+    ens.run(steps)
     torch.cuda.synchronize()
     sec = time.perf_counter() - t0
     value = total * steps / sec
     builds = sum(m.counters().builds for m in ens.members)
+    # kernel breakdown from a short timed run after the measured one (the
+    # timed form runs each chunk's last step eagerly between events)
+    fms, ums = ens.run_timed(max(1, steps // 10))
     ens.close()
     print(f"[bench] ensemble: {len(reps)} replicas, {total} bodies, {value / 1e6:.1f} M body-steps/s "
           f"({sec / steps * 1e3:.4f} ms/step; forces {fms * 1e3:.1f} us, update {ums * 1e3:.1f} us), "
