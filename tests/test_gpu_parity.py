@@ -341,3 +341,46 @@ def test_dense_clusters_contact_counts(hip, oracle_engine, precision):
         np.testing.assert_allclose(got, ref, rtol=2e-4, atol=2e-4 * np.max(np.abs(ref)), err_msg=f)
     ref, got = ga.gcoef[:, clean], gg.gcoef[:, clean]
     np.testing.assert_allclose(got, ref, rtol=2e-4, atol=2e-4 * np.max(np.abs(ref)))
+
+
+def edge_bodies():
+    """A dilute deformed monolayer, shifted off the lattice origin, with the
+    bodies nearest each box edge moved exactly onto x = 0, y = 0 or just below
+    x = L, y = L and oriented out of or along the box."""
+    b, box = F.hexagonal_monolayer(6, 6, packing=0.6, seed=4)
+    rng = np.random.default_rng(8)
+    b.coef[1:, :] = 0.02 * rng.standard_normal((b.coef.shape[0] - 1, len(b)))
+    b.x = np.mod(b.x + 0.137, box.Lx)
+    b.y = np.mod(b.y + 0.093, box.Ly)
+    top_x, top_y = np.nextafter(box.Lx, 0.0), np.nextafter(box.Ly, 0.0)
+    ox, oy = np.argsort(b.x), np.argsort(b.y)
+    for i, th in zip(ox[:3], (math.pi, 0.0, 1.25 * math.pi)):
+        b.x[i], b.theta[i] = 0.0, th
+    for i, th in zip(ox[-3:], (0.0, math.pi, 0.25 * math.pi)):
+        b.x[i], b.theta[i] = top_x, th
+    for i, th in zip(oy[:2], (-math.pi / 2, math.pi / 2)):
+        b.y[i], b.theta[i] = 0.0, th
+    for i, th in zip(oy[-2:], (math.pi / 2, -math.pi / 2)):
+        b.y[i], b.theta[i] = top_y, th
+    return b, box
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_wrap_at_the_box_edges_matches_oracle(hip, oracle_engine, precision):
+    # box.jl:21-30 through the update's FP64 wrap, for bodies exactly on the
+    # box edges (the wrap's division-free case 0 < x < L and the full one)
+    say("wrap edges", precision)
+    b, box = edge_bodies()
+    kw = dict(types=(F.BodyType(v0=2.0),), energies=(F.Tension(0.01), F.Overlap(1.0)), dt=0.005, skin=0.3)
+    dev = F.Simulation(b, box, precision=precision, engine=hip, **kw)
+    ref = F.Simulation(b, box, engine=oracle_engine, **kw)
+    tol = 1e-12 if precision == "fp64" else 1e-7
+    for n in (1, 2, 3):
+        F.step(dev, 1)
+        F.step(ref, 1)
+        bd, br = dev.bodies, ref.bodies
+        assert np.array_equal(bd.imgx, br.imgx) and np.array_equal(bd.imgy, br.imgy), n
+        assert np.all((bd.x >= 0.0) & (bd.x < box.Lx) & (bd.y >= 0.0) & (bd.y < box.Ly))
+        np.testing.assert_allclose(bd.x, br.x, rtol=0, atol=tol * box.Lx)
+        np.testing.assert_allclose(bd.y, br.y, rtol=0, atol=tol * box.Ly)
+    assert np.count_nonzero(bd.imgx) + np.count_nonzero(bd.imgy) >= 3
