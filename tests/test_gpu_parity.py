@@ -384,3 +384,34 @@ def test_wrap_at_the_box_edges_matches_oracle(hip, oracle_engine, precision):
         np.testing.assert_allclose(bd.x, br.x, rtol=0, atol=tol * box.Lx)
         np.testing.assert_allclose(bd.y, br.y, rtol=0, atol=tol * box.Ly)
     assert np.count_nonzero(bd.imgx) + np.count_nonzero(bd.imgy) >= 3
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_four_body_types_match_oracle(hip, oracle_engine, precision):
+    # BodyType parameters per type (src/bodies.jl:30-45) for the maximum number
+    # of types: each body's v0, Dr, Dt, μr, μθ, μab come from its own type
+    say("four types", precision)
+    b, box = F.hexagonal_monolayer(8, 8, packing=0.9, seed=6)
+    rng = np.random.default_rng(21)
+    b.coef[1:, :] = 0.03 * rng.standard_normal((b.coef.shape[0] - 1, len(b)))
+    b.type = (np.arange(len(b)) % 4 + 1).astype(np.int32)
+    types = (F.BodyType(v0=0.3, Dr=0.02, Dt=0.0, mur=1.0, mutheta=1.0, muab=0.1),
+             F.BodyType(v0=0.0, Dr=0.0, Dt=0.01, mur=0.5, mutheta=2.0, muab=0.05),
+             F.BodyType(v0=0.1, Dr=0.05, Dt=0.02, mur=2.0, mutheta=0.5, muab=0.2),
+             F.BodyType(v0=0.2, Dr=0.0, Dt=0.0, mur=1.5, mutheta=1.0, muab=0.0))
+    kw = dict(types=types, energies=(F.Tension(0.01), F.Overlap(1.0)), dt=0.005, skin=0.3, seed=5)
+    dev = F.Simulation(b, box, precision=precision, engine=hip, **kw)
+    ref = F.Simulation(b, box, engine=oracle_engine, **kw)
+    F.step(dev, 5)
+    F.step(ref, 5)
+    bd, br = dev.bodies, ref.bodies
+    tol = 1e-10 if precision == "fp64" else 2e-5
+    np.testing.assert_allclose(bd.x, br.x, rtol=tol, atol=tol)
+    np.testing.assert_allclose(bd.y, br.y, rtol=tol, atol=tol)
+    np.testing.assert_allclose(bd.theta, br.theta, rtol=tol, atol=tol)
+    np.testing.assert_allclose(bd.coef, br.coef, rtol=0, atol=tol)
+    # the types differ in every parameter: the same run with all bodies of type 1 differs
+    b1 = F.Bodies(b.x, b.y, b.theta, b.coef, b.R0)
+    one = F.Simulation(b1, box, precision=precision, engine=hip, **kw)
+    F.step(one, 5)
+    assert not np.allclose(one.bodies.theta, bd.theta, rtol=0, atol=1e-6)
