@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "fcc_forces_mfma.hpp"
+#include "fcc_forces_mfma4.hpp"
 #include "fcc_observe.hpp"
 #include "../../include/fcc_bench.h"
 
@@ -172,6 +173,18 @@ static bool row_phys(int t, int w, int T, int &q, int &k) {
     return true;
 }
 
+// The same for row m (0..15) of output tile t of the four-lane layout: local
+// slot s = 4t + m % 4 of lane group m / 4 holds quantity s / 10 at node
+// 10·(m / 4) + s % 10.
+static bool row_phys4(int t, int m, int T, int &q, int &k) {
+    const int g = m / 4, s = 4 * t + m % 4;
+    const int nq = T == 4 ? 3 : 2;
+    if (s >= nq * NQ4) return false;
+    q = s / NQ4;
+    k = NQ4 * g + s % NQ4;
+    return true;
+}
+
 // ------------------------------------------------------------- simulation
 
 struct fcc_sim {
@@ -227,6 +240,7 @@ struct Sim final : fcc_sim {
     int32_t *ctile = nullptr;  // per-tile totals of the cell-count scan
     unsigned ntile = 1;
     uint16_t *asyn = nullptr, *bpull = nullptr;  // MFMA operand tables (mixed modes)
+    uint16_t *asyn4 = nullptr, *bpull4 = nullptr;  // the same for the four-lane layout
     int mfma_T = 0;
     bool valid = false;
     int32_t maxcount = 0;  // largest neighbour count of the last build
@@ -498,6 +512,32 @@ struct Sim final : fcc_sim {
             if ((rc = alloc(&asyn, A.size())) || (rc = alloc(&bpull, B.size()))) return rc;
             HIPCHK(hipMemcpy(asyn, A.data(), A.size() * 2, hipMemcpyHostToDevice));
             HIPCHK(hipMemcpy(bpull, B.data(), B.size() * 2, hipMemcpyHostToDevice));
+            // four-lane layout (16x16x16 tiles): synthesis A holds row l % 16 of
+            // tile t at coefficients 4(l / 16) + j; pullback B holds row
+            // 4(l / 16) + j of tile t at coefficient l % 16
+            const int T4 = TT == 4 ? 8 : 5;
+            std::vector<uint16_t> A4((size_t)T4 * 2 * 64 * 4, 0), B4((size_t)T4 * 2 * 64 * 4, 0);
+            for (int t = 0; t < T4; t++)
+                for (int l = 0; l < 64; l++)
+                    for (int j = 0; j < 4; j++) {
+                        int q, k;
+                        uint16_t hi, lo;
+                        const int ca = 4 * (l / 16) + j;
+                        double v = 0.0;
+                        if (ca < nc && row_phys4(t, l % 16, TT, q, k)) v = basis_value(q, k, ca, K, nth);
+                        enc(v, hi, lo);
+                        A4[((size_t)(t * 2 + 0) * 64 + l) * 4 + j] = hi;
+                        A4[((size_t)(t * 2 + 1) * 64 + l) * 4 + j] = lo;
+                        const int cb = l % 16;
+                        v = 0.0;
+                        if (cb < nc && row_phys4(t, 4 * (l / 16) + j, TT, q, k)) v = basis_value(q, k, cb, K, nth);
+                        enc(v, hi, lo);
+                        B4[((size_t)(t * 2 + 0) * 64 + l) * 4 + j] = hi;
+                        B4[((size_t)(t * 2 + 1) * 64 + l) * 4 + j] = lo;
+                    }
+            if ((rc = alloc(&asyn4, A4.size())) || (rc = alloc(&bpull4, B4.size()))) return rc;
+            HIPCHK(hipMemcpy(asyn4, A4.data(), A4.size() * 2, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(bpull4, B4.data(), B4.size() * 2, hipMemcpyHostToDevice));
             if (ldc != LDC_M) return fail(FCC_ERR_STATE, "MFMA forces kernel expects coefficient rows of %d", LDC_M);
             mfma_T = TT;
         }
@@ -601,9 +641,13 @@ struct Sim final : fcc_sim {
             if (mfma_T && fast_ok()) {
                 const unsigned gm = blocks(N, 32);
                 const bool bf = cfg.precision == FCC_MIXED_BF16;
+// Below FOUR_LANES_BELOW bodies the four-lane kernel (16 bodies per wave).
+#define FCC_MF2(KV, TV, NCV, BFV)                                                                        \
+    if (N < FOUR_LANES_BELOW)                                                                            \
+        FCC_LAUNCH((k_forces_mfma4<KV, TV, NCV, BFV>), blocks(N, 16), 64, 0, stream, eb, ee, p, d, asyn4, bpull4, use_status, dbg); \
+    else FCC_LAUNCH((k_forces_mfma<KV, TV, NCV, BFV>), gm, 64, 0, stream, eb, ee, p, d, asyn, bpull, use_status, dbg);
 #define FCC_MF(KV, TV, NCV)                                                                             \
-    if (bf) FCC_LAUNCH((k_forces_mfma<KV, TV, NCV, true>), gm, 64, 0, stream, eb, ee, p, d, asyn, bpull, use_status, dbg); \
-    else FCC_LAUNCH((k_forces_mfma<KV, TV, NCV, false>), gm, 64, 0, stream, eb, ee, p, d, asyn, bpull, use_status, dbg);
+    if (bf) { FCC_MF2(KV, TV, NCV, true) } else { FCC_MF2(KV, TV, NCV, false) }
 #define FCC_MF_T(KV, NCV) if (mfma_T == 3) { FCC_MF(KV, 3, NCV) } else { FCC_MF(KV, 4, NCV) }
                 if (K == 5) {
                     if (p.ncls <= 1) { FCC_MF_T(5, 1) } else { FCC_MF_T(5, 2) }
@@ -612,6 +656,7 @@ struct Sim final : fcc_sim {
                 }
 #undef FCC_MF_T
 #undef FCC_MF
+#undef FCC_MF2
                 HIPCHK(hipGetLastError());
                 return FCC_OK;
             }

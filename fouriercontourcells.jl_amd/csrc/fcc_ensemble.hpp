@@ -66,7 +66,7 @@ struct Ensemble final : fcc_ensemble {
     Ptrs<T> *dD = nullptr;
     Tile *tf = nullptr, *tu = nullptr;  // tiles of the forces / update launches
     size_t cap_f = 0, cap_u = 0;
-    // forces tiles: gf4 of the four-lane FP32 / FP64 kernel (replicas below
+    // forces tiles: gf4 of the four-lane kernel (MFMA or FP32 / FP64; replicas below
     // FOUR_LANES_BELOW bodies, as each would run alone), then gf of the
     // other kernel
     unsigned gf4 = 0, gf = 0, gu = 0;
@@ -168,11 +168,12 @@ struct Ensemble final : fcc_ensemble {
         if constexpr (sizeof(T) == 4) {
             if (var == 2) {
                 const bool bf = a.cfg.precision == FCC_MIXED_BF16;
-#define FCC_MFE(KV, TV, NCV)                                                                                  \
-    if (bf) FCC_LAUNCH((k_forces_mfma_ens<KV, TV, NCV, true>), gf, 64, 0, stream, eb, ee, dP, dD, tf, a.asyn,  \
-                       a.bpull, 2, dbg);                                                               \
-    else FCC_LAUNCH((k_forces_mfma_ens<KV, TV, NCV, false>), gf, 64, 0, stream, eb, ee, dP, dD, tf, a.asyn,   \
-                    a.bpull, 2, dbg);
+#define FCC_MFE2(KV, TV, NCV, BFV)                                                                            \
+    if (gf4) FCC_LAUNCH((k_forces_mfma4_ens<KV, TV, NCV, BFV>), gf4, 64, 0, stream, eb, gf ? nullptr : ee, dP, dD, \
+                        tf, a.asyn4, a.bpull4, 2, dbg);                                                       \
+    if (gf) FCC_LAUNCH((k_forces_mfma_ens<KV, TV, NCV, BFV>), gf, 64, 0, stream, gf4 ? nullptr : eb, ee, dP, dD, \
+                       tf + gf4, a.asyn, a.bpull, 2, dbg);
+#define FCC_MFE(KV, TV, NCV) if (bf) { FCC_MFE2(KV, TV, NCV, true) } else { FCC_MFE2(KV, TV, NCV, false) }
 #define FCC_MFE_T(KV, NCV) if (a.mfma_T == 3) { FCC_MFE(KV, 3, NCV) } else { FCC_MFE(KV, 4, NCV) }
                 if (a.K == 5) {
                     if (ncls <= 1) { FCC_MFE_T(5, 1) } else { FCC_MFE_T(5, 2) }
@@ -181,6 +182,7 @@ struct Ensemble final : fcc_ensemble {
                 }
 #undef FCC_MFE_T
 #undef FCC_MFE
+#undef FCC_MFE2
                 HIPCHK(hipGetLastError());
                 return FCC_OK;
             }
@@ -274,11 +276,11 @@ struct Ensemble final : fcc_ensemble {
         if (var != tiles_var || lc.update_block != tiles_lc.update_block || lc.xcd_order != tiles_lc.xcd_order) {
             std::vector<int64_t> sizes(R), small(R, 0), large(R, 0);
             for (size_t r = 0; r < R; r++) sizes[r] = m[r]->N;
-            // Bodies per forces workgroup: 16 for the four-lane FP32 / FP64 kernel
-            // (replicas below FOUR_LANES_BELOW bodies at nθ = 40, the kernel each
-            // would get alone), 32 for the two-lane kernels (MFMA, and the FP32 /
-            // FP64 one at nθ = 40), 64 otherwise.
-            const bool fast40 = var == 1 && m[0]->nth == 40;
+            // Bodies per forces workgroup: 16 for the four-lane kernels (replicas
+            // below FOUR_LANES_BELOW bodies at nθ = 40, the kernel each would get
+            // alone: MFMA or FP32 / FP64), 32 for the two-lane kernels, 64
+            // otherwise.
+            const bool fast40 = (var == 1 && m[0]->nth == 40) || var == 2;
             for (size_t r = 0; r < R; r++) (fast40 && sizes[r] < FOUR_LANES_BELOW ? small : large)[r] = sizes[r];
             const int tile = var == 2 || fast40 ? 32 : 64;
             std::vector<Tile> a = tile_table(small, 16, lc.xcd_order != 0);
