@@ -66,7 +66,7 @@ static std::mutex g_capture_lock;
 // ------------------------------------------------------------- validation
 
 // Simulation(...) checks (src/simulation.jl:71-118, src/basis.jl:36-38,
-// src/neighbors.jl:334) and the native-kernel limits.
+// src/neighbors.jl:123) and the native-kernel limits.
 static int validate(const fcc_config *cfg, const fcc_bodies *b) {
     if (!cfg || !b) return fail(FCC_ERR_ARGUMENT, "null argument");
     if (cfg->n_bodies < 1) return fail(FCC_ERR_ARGUMENT, "need at least one body");
@@ -230,6 +230,9 @@ struct Sim final : fcc_sim {
     int64_t N = 0;
     int K = 0, nc = 0, ldc = 0, nth = 0, KM = 0;
     bool mixed = false, debug = false, has_contact = false, has_pdrag = false, has_linrep = false;
+    // Systems below this many bodies get the four-lane forces kernels at
+    // nθ = 40 (FOUR_LANES_BELOW; 0 with FCC_FLAG_TWO_LANES).
+    int64_t four_below = FOUR_LANES_BELOW;
     bool has_bonds = false;  // bond terms and at least one bond
     int64_t nbonds = 0;
     double skin = 0.0, range = 0.0;
@@ -298,6 +301,7 @@ struct Sim final : fcc_sim {
         KM = K <= 5 ? 5 : (K <= 7 ? 7 : 15);
         mixed = c->precision == FCC_MIXED_FP16 || c->precision == FCC_MIXED_BF16;
         debug = (c->flags & FCC_FLAG_DEBUG_BUFFERS) != 0;
+        four_below = (c->flags & FCC_FLAG_TWO_LANES) ? 0 : FOUR_LANES_BELOW;
         skin = c->skin;
         if (ext_stream) {
             stream = ext_stream;
@@ -542,7 +546,7 @@ struct Sim final : fcc_sim {
             mfma_T = TT;
         }
 
-        // cell grid (src/simulation.jl:90, src/neighbors.jl:238-244)
+        // cell grid (src/simulation.jl:90, src/neighbors.jl:27-33)
         double maxR0 = 0.0;
         for (int64_t i = 0; i < N; i++) maxR0 = std::max(maxR0, b->R0[i]);
         const double cutoff = 2.0 * maxR0 * (1.0 + std::sqrt(2.0 / 3.0)) + range + skin;
@@ -565,7 +569,7 @@ struct Sim final : fcc_sim {
         HIPCHK(hipMemsetAsync(perm, 0, sizeof(int32_t) * N, stream));
         HIPCHK(hipMemsetAsync(cstart, 0, sizeof(int32_t) * nce, stream));
         HIPCHK(hipMemsetAsync(ccnt, 0, sizeof(int32_t) * nce, stream));
-        // neighbour list (capacity 32, src/neighbors.jl:332-338)
+        // neighbour list (capacity 32, src/neighbors.jl:121-127)
         d.cap = 32;
         if ((rc = alloc(&d.nidx, (size_t)d.cap * N)) || (rc = alloc(&d.ncount, N)) || (rc = alloc(&d.x0, N)) ||
             (rc = alloc(&d.y0, N)) || (rc = alloc(&d.rmax0, N)))
@@ -643,7 +647,7 @@ struct Sim final : fcc_sim {
                 const bool bf = cfg.precision == FCC_MIXED_BF16;
 // Below FOUR_LANES_BELOW bodies the four-lane kernel (16 bodies per wave).
 #define FCC_MF2(KV, TV, NCV, BFV)                                                                        \
-    if (N < FOUR_LANES_BELOW)                                                                            \
+    if (N < four_below)                                                                            \
         FCC_LAUNCH((k_forces_mfma4<KV, TV, NCV, BFV>), blocks(N, 16), 64, 0, stream, eb, ee, p, d, asyn4, bpull4, use_status, dbg); \
     else FCC_LAUNCH((k_forces_mfma<KV, TV, NCV, BFV>), gm, 64, 0, stream, eb, ee, p, d, asyn, bpull, use_status, dbg);
 #define FCC_MF(KV, TV, NCV)                                                                             \
@@ -665,7 +669,7 @@ struct Sim final : fcc_sim {
 // Below FOUR_LANES_BELOW bodies (fewer than 4096 waves of 32 bodies, one per
 // SIMD slot of occupancy 4) the FP32 / FP64 kernel takes four lanes per body.
 #define FCC_FAST(KV, NC)                                                                                 \
-    if (nth == 40 && N < FOUR_LANES_BELOW)                                                               \
+    if (nth == 40 && N < four_below)                                                               \
         FCC_LAUNCH((k_forces_fast2<T, KV, NC, 4>), blocks(N, 16), 64, 0, stream, eb, ee, p, d, use_status, dbg); \
     else if (nth == 40)                                                                                  \
         FCC_LAUNCH((k_forces_fast2<T, KV, NC, 2>), blocks(N, 32), 64, 0, stream, eb, ee, p, d, use_status, dbg); \

@@ -251,7 +251,7 @@ __device__ __forceinline__ double wrap(double x, double L, int32_t &shift) {
 
 // ------------------------------------------------------------- body terms
 
-// contour_integrands of the built-ins (src/energies/body_terms.jl:10,31;
+// contour_integrands of the built-ins (src/energies/body_terms.jl:12,33;
 // Bending test/test_energies.jl:9).
 template <typename T>
 __device__ __forceinline__ T term_q(int kind, T R, T dR, T ddR) {
@@ -264,7 +264,7 @@ __device__ __forceinline__ T term_q(int kind, T R, T dR, T ddR) {
     return sqrt(R * R + dR * dR);
 }
 
-// contour_integrand_gradients (src/energies/body_terms.jl:12-15; closed forms
+// contour_integrand_gradients (src/energies/body_terms.jl:14-17; closed forms
 // of the ForwardDiff derivatives of the other two).
 template <typename T>
 __device__ __forceinline__ void term_J(int kind, T R, T dR, T ddR, T &j0, T &j1, T &j2) {
@@ -285,7 +285,7 @@ __device__ __forceinline__ void term_J(int kind, T R, T dR, T ddR, T &j0, T &j1,
     j2 = T(0);
 }
 
-// energy and energy_gradient (src/energies/body_terms.jl:11,16,32).
+// energy and energy_gradient (src/energies/body_terms.jl:13,18,34).
 template <typename T>
 __device__ __forceinline__ T term_E(int kind, T p0, T p1, T Q, T r0, T &dF) {
     if (kind == FCC_TERM_PERIMETER_ELASTICITY) {
@@ -350,7 +350,7 @@ __device__ __forceinline__ void row_store(const T (&l)[W], T *row) {
     for (int q = 0; q < (int)(W * sizeof(T) / 16); q++) s[q] = tmp[q];
 }
 
-// contour_radius_dir without the R0 factor (src/geometry.jl:282-291).
+// contour_radius_dir without the R0 factor (src/geometry.jl:147-156).
 template <typename T, int KM>
 __device__ __forceinline__ T contour_dir(const T (&l)[2 * KM + 1], int K, T c1, T s1) {
     T c = c1, s = s1, r = l[0];

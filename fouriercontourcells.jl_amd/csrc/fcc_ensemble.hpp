@@ -281,7 +281,7 @@ struct Ensemble final : fcc_ensemble {
             // alone: MFMA or FP32 / FP64), 32 for the two-lane kernels, 64
             // otherwise.
             const bool fast40 = (var == 1 && m[0]->nth == 40) || var == 2;
-            for (size_t r = 0; r < R; r++) (fast40 && sizes[r] < FOUR_LANES_BELOW ? small : large)[r] = sizes[r];
+            for (size_t r = 0; r < R; r++) (fast40 && sizes[r] < m[r]->four_below ? small : large)[r] = sizes[r];
             const int tile = var == 2 || fast40 ? 32 : 64;
             std::vector<Tile> a = tile_table(small, 16, lc.xcd_order != 0);
             const size_t n4 = a.size();

@@ -7,7 +7,7 @@
 // compile time:
 //  - K (shape modes) is a template constant: no per-mode guards;
 //  - Tension and PerimeterElasticity share the integrand √(R² + R'²)
-//    (src/energies/body_terms.jl:10,31), so one perimeter integral and one
+//    (src/energies/body_terms.jl:12,33), so one perimeter integral and one
 //    combined weight serve every such term; Bending has its own;
 //  - the contact density is kept as per-node hit counts, one bit-sliced
 //    counter (NP bit planes over the nθ nodes) per Overlap coupling class, in
@@ -100,7 +100,7 @@ template <typename T>
 __device__ __forceinline__ T tiny_d2() { return sizeof(T) == 4 ? T(1e-30) : T(1e-300); }
 
 // R_j(ψ)/R0 = a0 + Σ_{m=1..K} αm cos (m+1)ψ + βm sin (m+1)ψ as Re Σ cn z^n,
-// z = (c1, s1), by complex Horner (src/geometry.jl:282-291 evaluates the
+// z = (c1, s1), by complex Horner (src/geometry.jl:147-156 evaluates the
 // same sum by rotating (cos nψ, sin nψ)).
 template <typename T, int K>
 __device__ __forceinline__ T contour_horner(const T (&l)[2 * K + 1], T zr, T zi) {

@@ -239,7 +239,7 @@ __global__ __launch_bounds__(64) void k_forces_ens(const Params<T> *ps, const Pt
 // ----------------------------------------------------------- per-body math
 
 // (cos nθ, sin nθ) for n = 2..K+1 in slots 1..K: direct in FP64, by rotation
-// from (cos θ, sin θ) in FP32 (geometry.jl:271-273 recurrence), where slot 0
+// from (cos θ, sin θ) in FP32 (src/geometry.jl:132-138 recurrence), where slot 0
 // also receives (cos θ, sin θ) (FP64 leaves slot 0 unset).
 template <typename T, int KM>
 __device__ __forceinline__ void harmonics(T th, int K, T (&cn)[KM + 1], T (&sn)[KM + 1]) {
@@ -266,7 +266,7 @@ __device__ __forceinline__ void harmonics(T th, int K, T (&cn)[KM + 1], T (&sn)[
     }
 }
 
-// enforce_shape! (src/dynamics.jl:174-194) on a register row, in U precision.
+// enforce_shape! (src/dynamics.jl:44-64) on a register row, in U precision.
 template <typename U, typename T, int KM>
 __device__ __forceinline__ void enforce_shape(T (&c)[2 * KM + 1], int K) {
 #pragma clang fp contract(off)
@@ -295,7 +295,7 @@ __device__ __forceinline__ void enforce_shape(T (&c)[2 * KM + 1], int K) {
     c[0] = (T)sqrt(U(1) - q / U(2));
 }
 
-// lab_frame! (src/geometry.jl:208-227) on registers.
+// lab_frame! (src/geometry.jl:73-92) on registers.
 template <typename T, int KM>
 __device__ __forceinline__ T lab_frame(const T (&c)[2 * KM + 1], T th, T r0, int K, T (&l)[2 * KM + 1],
                                        T &rmin) {
@@ -368,8 +368,10 @@ __global__ __launch_bounds__(256) void k_init(const Params<T> p, const Ptrs<T> d
 // reciprocal rd = RN(1/d) shared by all numerators, then RN32(x·rd). The
 // product is within 2^-51.9 (relative) of q = x/d. A quotient of two 24-bit
 // significands is never a binary32 rounding midpoint m, and q − m =
-// (x − m·d)/d with x − m·d a nonzero multiple of 2^(E−24+e_d) for q in
-// [2^E, 2^(E+1)) and d in [2^e_d, 2^(e_d+1)), so |q − m| ≥ 2^(E−48), which is
+// (x − m·d)/d for q in [2^E, 2^(E+1)) and d in [2^e_d, 2^(e_d+1)): m is an
+// odd multiple of 2^(E−24), d a multiple of 2^(e_d−23) and x (≥ 2^(E+e_d)) a
+// multiple of 2^(E+e_d−23), so x − m·d is a nonzero multiple of
+// 2^(E+e_d−47) and |q − m| ≥ 2^(E+e_d−47) / 2^(e_d+1) = 2^(E−48), which is
 // more than the product's error (< 2^(E−50.9)): no midpoint lies between q and
 // the product, and both round to the same binary32 value (the subnormal range
 // keeps a margin of the same kind; zero, infinite and NaN operands give the
@@ -463,7 +465,7 @@ __device__ __forceinline__ void substrate_disp(const PP &p, const DP &d, int64_t
     uy = p.dt * (p.v0[tc] * s - p.mur[tc] * fy) + st * x2;
 }
 
-// displacement_kernel! (src/dynamics.jl:258-277), used when some body type
+// displacement_kernel! (src/dynamics.jl:128-147), used when some body type
 // has parent drag: a child needs its parent's θ before the update.
 template <typename T, int UB, typename PP, typename DP>
 __device__ __forceinline__ void k_displacement_impl(const PP &p, const DP &d, int64_t blk) {
@@ -561,7 +563,7 @@ __device__ __forceinline__ TypeParams<T> type_params(const PP &p, int ti) {
 // Displacement of a body without parent drag (src/dynamics.jl:84-99):
 // self-propulsion along θ (s, co = sin θ, cos θ), mobility times the centre
 // force, translational noise from Philox block 0 of draw `draw`
-// (src/dynamics.jl:258-277).
+// (src/dynamics.jl:128-147).
 template <typename T, typename PP>
 __device__ __forceinline__ void body_displacement(const PP &p, int64_t i, const TypeParams<T> &tp, T s, T co,
                                                   T gx, T gy, uint64_t draw, T &d0, T &d1) {
@@ -583,7 +585,7 @@ __device__ __forceinline__ T rotation_noise(const PP &p, int64_t i, const TypePa
 }
 
 // Body-frame gradient, orientation and shape update with the shape
-// constraint (src/dynamics.jl:310-333, src/gradients.jl:347-362; the
+// constraint (src/dynamics.jl:180-203, src/gradients.jl:347-362; the
 // constraint in FP64 when F64S), given the harmonics of θ. Returns the new θ;
 // c holds the new coefficients and gc the body-frame gradient.
 template <typename T, int KM, bool F64S, typename PP>
@@ -607,8 +609,8 @@ __device__ __forceinline__ T shape_step(const PP &p, const TypeParams<T> &tp, in
 }
 
 // Position wrap in FP64 (src/box.jl:21-30), the image counters, the lab
-// frame of the new state (src/geometry.jl:208-227) and the Verlet skin check
-// (src/neighbors.jl:409-416) of body i, given its position (x, y), its new
+// frame of the new state (src/geometry.jl:73-92) and the Verlet skin check
+// (src/neighbors.jl:198-205) of body i, given its position (x, y), its new
 // coefficients c and orientation thn, its displacement (d0, d1) and the
 // skin reference (x0, y0, rmax0).
 template <typename T, int KM, bool EXACT, typename PP, typename DP>
@@ -750,11 +752,11 @@ __device__ __forceinline__ void finish_body(const PP &p, const DP &d, int64_t i,
     place_body<T, KM, EXACT>(p, d, i, K, b.c, thn, b.r0, d0, d1, b.x, b.y, b.x0, b.y0, b.rmax0);
 }
 
-// Fused Euler–Maruyama update (src/dynamics.jl:310-333) with the
-// displacement and Philox noise of src/dynamics.jl:84-99,258-277 (bodies
+// Fused Euler–Maruyama update (src/dynamics.jl:180-203) with the
+// displacement and Philox noise of src/dynamics.jl:84-99,128-147 (bodies
 // without parent drag), the body-frame gradient, the shape constraint (FP64
 // when F64S), position wrapping in FP64, the lab frame of the new state and
-// the Verlet skin check (src/neighbors.jl:409-416); one body per lane.
+// the Verlet skin check (src/neighbors.jl:198-205); one body per lane.
 template <typename T, int KM, bool EXACT, bool F64S, int UB, typename PP, typename DP>
 __device__ __forceinline__ void k_update_impl(const PP &p, const DP &d, int use_disp, int debug, int64_t blk) {
     const int64_t i = (int64_t)blk * (UB ? UB : (int)blockDim.x) + threadIdx.x;

@@ -32,6 +32,7 @@ FCC_TERM_LINEAR_REPULSION = 17
 FCC_TERM_HARMONIC_BOND = 32
 
 FCC_FLAG_DEBUG_BUFFERS = 1
+FCC_FLAG_TWO_LANES = 2
 
 PRECISIONS = {
     "fp64": FCC_FP64,

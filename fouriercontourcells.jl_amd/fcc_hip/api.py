@@ -358,12 +358,14 @@ class Simulation:
     `precision` replaces `T`: "fp64" (T = Float64), "fp32" (T = Float32),
     "mixed" / "mixed_fp16" / "mixed_bf16" (MFMA contour contractions).
     `engine` is the C-ABI implementation (default: libfcc_hip.so).
+    `two_lanes` launches the two-lane forces kernels at every size (the
+    library picks four lanes per body below 131072 bodies), for tests.
     """
 
     def __init__(self, bodies, box, types, energies=(), dt=0.005, ntheta=40, precision="fp64",
-                 seed=1, skin=0.3, debug=False, engine=None, bonds=()):
+                 seed=1, skin=0.3, debug=False, engine=None, bonds=(), two_lanes=False):
         self._setup(engine if engine is not None else _abi.hip_engine(), bodies, box, types, energies,
-                    dt, ntheta, precision, seed, skin, debug)
+                    dt, ntheta, precision, seed, skin, debug, two_lanes)
         h = ctypes.c_void_p()
         _check(self.engine, self.engine.create(ctypes.byref(self._cfg),
                                                ctypes.byref(self._bodies.struct()), ctypes.byref(h)))
@@ -394,7 +396,8 @@ class Simulation:
                                                   pairs.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
         self.bonds = [tuple(int(v) for v in b) for b in pairs]
 
-    def _setup(self, engine, bodies, box, types, energies, dt, ntheta, precision, seed, skin, debug):
+    def _setup(self, engine, bodies, box, types, energies, dt, ntheta, precision, seed, skin, debug,
+               two_lanes=False):
         self.engine = engine
         self.box = box
         self.types = tuple(types)
@@ -405,7 +408,7 @@ class Simulation:
         self.precision = precision
         self._bodies = bodies.copy()
         self._owner = None
-        flags = _abi.FCC_FLAG_DEBUG_BUFFERS if debug else 0
+        flags = (_abi.FCC_FLAG_DEBUG_BUFFERS if debug else 0) | (_abi.FCC_FLAG_TWO_LANES if two_lanes else 0)
         self._cfg = _config(bodies, box, types, energies, dt, ntheta, precision, seed, skin, flags)
 
     def close(self):
@@ -534,7 +537,7 @@ class Ensemble:
     """
 
     def __init__(self, replicas, ntheta=40, precision="fp32", debug=False, tune=False, tune_steps=20,
-                 engine=None):
+                 engine=None, two_lanes=False):
         self.engine = engine if engine is not None else _abi.hip_engine()
         replicas = list(replicas)
         if not replicas:
@@ -546,7 +549,7 @@ class Ensemble:
         for r, rp in enumerate(replicas):
             sim = Simulation.__new__(Simulation)
             sim._setup(self.engine, rp.bodies, rp.box, rp.types, rp.energies, rp.dt, ntheta, precision,
-                       rp.seed, rp.skin, debug)
+                       rp.seed, rp.skin, debug, two_lanes)
             cfgs[r] = sim._cfg
             structs[r] = sim._bodies.struct()
             self.members.append(sim)

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define FCC_ABI_VERSION 3
+#define FCC_ABI_VERSION 4
 
 #define FCC_MAX_TYPES 4        /* body types per simulation */
 #define FCC_MAX_TERMS 8        /* energy terms per simulation */
@@ -67,7 +67,11 @@ enum fcc_term_kind {
 
 /* Simulation flags. */
 enum fcc_flags {
-    FCC_FLAG_DEBUG_BUFFERS = 1  /* keep P, GR, GD, GDD and gcoef for inspection */
+    FCC_FLAG_DEBUG_BUFFERS = 1, /* keep P, GR, GD, GDD and gcoef for inspection */
+    FCC_FLAG_TWO_LANES = 2      /* forces kernels with two lanes per body at every N
+                                   (by default four lanes below 131072 bodies at
+                                   n_theta = 40); same results up to rounding, for
+                                   tests of the two-lane kernels on small systems */
 };
 
 /* BodyType (src/bodies.jl:30-45). */

@@ -225,7 +225,7 @@ static void basis_init(fcco_sim *s) {
 
 /* -------------------------------------------------------------- geometry */
 
-/* lab_frame!, src/geometry.jl:208-227. */
+/* lab_frame!, src/geometry.jl:73-92. */
 static void lab_frame(fcco_sim *s, int64_t i, double thi) {
     int K = s->K, nc = s->nc;
     double r0 = s->R0[i];
@@ -249,7 +249,7 @@ static void lab_frame(fcco_sim *s, int64_t i, double thi) {
     s->rmax[i] = r0 * bound;
 }
 
-/* synthesize! on the CPU (three products), src/geometry.jl:248-257. */
+/* synthesize! on the CPU (three products), src/geometry.jl:113-122. */
 static void synthesize(fcco_sim *s) {
     int nth = s->nth, nc = s->nc;
     int64_t N = s->N;
@@ -270,14 +270,14 @@ static void synthesize(fcco_sim *s) {
     }
 }
 
-/* update_geometry!, src/geometry.jl:240-245. */
+/* update_geometry!, src/geometry.jl:105-110. */
 static void update_geometry(fcco_sim *s) {
 #pragma omp parallel for num_threads(g_threads) schedule(static)
     for (int64_t i = 0; i < s->N; i++) lab_frame(s, i, s->th[i]);
     synthesize(s);
 }
 
-/* contour_radius_dir, src/geometry.jl:282-291. */
+/* contour_radius_dir, src/geometry.jl:147-156. */
 static double contour_radius_dir(const double *labj, int K, double R0, double c1, double s1) {
     double c = c1, sn = s1, r = labj[0];
     for (int m = 1; m <= K; m++) {
@@ -294,7 +294,7 @@ static double contour_radius_dir(const double *labj, int K, double R0, double c1
 
 #define SHAPE_BOUND 0.816496580927726 /* sqrt(2/3), src/dynamics.jl:26 */
 
-/* enforce_shape!, src/dynamics.jl:174-194. */
+/* enforce_shape!, src/dynamics.jl:44-64. */
 static void enforce_shape(double *c, int K) {
     double sm = 0.0;
     for (int m = 1; m <= K; m++) sm += sqrt(c[m] * c[m] + c[K + m] * c[K + m]);
@@ -320,7 +320,7 @@ static int applies(const fcc_term *t, int32_t type0) {
     return 0;
 }
 
-/* contour_integrands of the built-in body terms (body_terms.jl:10,31;
+/* contour_integrands of the built-in body terms (body_terms.jl:12,33;
  * Bending: test/test_energies.jl:9). */
 static double integrand(const fcc_term *t, double R, double dR, double ddR) {
     switch (t->kind) {
@@ -336,7 +336,7 @@ static double integrand(const fcc_term *t, double R, double dR, double ddR) {
     return 0.0;
 }
 
-/* contour_integrand_gradients (Tension body_terms.jl:12-15; the others by
+/* contour_integrand_gradients (Tension body_terms.jl:14-17; the others by
  * their closed forms, which ForwardDiff reproduces to rounding). */
 static void integrand_grad(const fcc_term *t, double R, double dR, double ddR, double J[3]) {
     switch (t->kind) {
@@ -368,7 +368,7 @@ static void integrand_grad(const fcc_term *t, double R, double dR, double ddR, d
     J[0] = J[1] = J[2] = 0.0;
 }
 
-/* energy and energy_gradient (body_terms.jl:11,16,32; interface.jl:192). */
+/* energy and energy_gradient (body_terms.jl:13,18,34; interface.jl:192). */
 static double term_energy(const fcc_term *t, double Q, double R0, double *dF) {
     switch (t->kind) {
     case FCC_TERM_TENSION:
@@ -689,7 +689,7 @@ static int neighbors_rebuild(fcco_sim *s) {
     return FCC_OK;
 }
 
-/* skin_flag!, src/neighbors.jl:409-416. */
+/* skin_flag!, src/neighbors.jl:198-205. */
 static int skin_exceeded(const fcco_sim *s, int64_t i, double x, double y, double rmax) {
     double dx = fcco_minimum_image(x - s->x0[i], s->Lx);
     double dy = fcco_minimum_image(y - s->y0[i], s->Ly);
@@ -699,7 +699,7 @@ static int skin_exceeded(const fcco_sim *s, int64_t i, double x, double y, doubl
 }
 
 /* update_neighbors!(sim; force), src/simulation.jl:132-142 (skin_check!
- * src/neighbors.jl:422-426). */
+ * src/neighbors.jl:211-215). */
 int fcco_update_neighbors(fcco_sim *s, int32_t force) {
     if (!force && s->valid && s->skin > 0.0) {
         int flag = 0;
@@ -741,7 +741,7 @@ static void substrate_displacement(const fcco_sim *s, int64_t i, const double *x
     *dy = s->dt * (p->v0 * sn - p->mur * fy) + st * x2;
 }
 
-/* displacement_kernel!, src/dynamics.jl:258-277 (noise from a 3 x N matrix
+/* displacement_kernel!, src/dynamics.jl:128-147 (noise from a 3 x N matrix
  * or generated, PhiloxNoise src/random.jl:88-100). */
 static void displacement(fcco_sim *s, const double *xi_noise) {
     s->status[STATUS_UPDATE] = s->status[STATUS_REBUILD];
@@ -781,7 +781,7 @@ static void displacement(fcco_sim *s, const double *xi_noise) {
     }
 }
 
-/* euler_maruyama_kernel!, src/dynamics.jl:310-333. `geom`: update lab/rmax;
+/* euler_maruyama_kernel!, src/dynamics.jl:180-203. `geom`: update lab/rmax;
  * `skin`: fused skin check; `clock`: tick. */
 static void euler_maruyama(fcco_sim *s, int frame, int geom, int skin, int clock) {
     if (s->status[STATUS_UPDATE] != 0) return;
@@ -947,7 +947,7 @@ int fcco_compute_gradients(fcco_sim *s, int32_t frame) {
     return FCC_OK;
 }
 
-/* advance!, src/simulation.jl:181-191, with integrate! src/dynamics.jl:350-362. */
+/* advance!, src/simulation.jl:181-191, with integrate! src/dynamics.jl:220-232. */
 static void advance(fcco_sim *s) {
     fcco_compute_gradients(s, 0);
     int skin = s->ncontact > 0 && s->valid && s->skin > 0.0;
@@ -1027,7 +1027,7 @@ static int load_bodies(fcco_sim *s, const fcc_bodies *b) {
 }
 
 /* Simulation(...) validation, src/simulation.jl:71-118, basis.jl:36-38,
- * neighbors.jl:334. */
+ * neighbors.jl:123. */
 static int validate(const fcc_config *cfg, const fcc_bodies *b) {
     if (cfg->n_bodies < 1) return fail(FCC_ERR_ARGUMENT, "need at least one body");
     if (cfg->n_modes < 1) return fail(FCC_ERR_ARGUMENT, "maxmode must be ≥ 2");
@@ -1125,7 +1125,7 @@ int fcco_create(const fcc_config *cfg, const fcc_bodies *b, fcco_sim **out) {
     }
     basis_init(s);
     load_bodies(s, b);
-    /* cutoff, src/simulation.jl:90; CellList src/neighbors.jl:238-244 */
+    /* cutoff, src/simulation.jl:90; CellList src/neighbors.jl:27-33 */
     double maxR0 = 0.0;
     for (int64_t i = 0; i < N; i++)
         if (s->R0[i] > maxR0) maxR0 = s->R0[i];
@@ -1140,7 +1140,7 @@ int fcco_create(const fcc_config *cfg, const fcc_bodies *b, fcco_sim **out) {
     s->start = xcalloc(nce, sizeof(int32_t), &bad);
     s->stop = xcalloc(nce, sizeof(int32_t), &bad);
     s->scratch = xcalloc(nce, sizeof(int32_t), &bad);
-    /* NeighborList, src/neighbors.jl:332-338 (capacity 32) */
+    /* NeighborList, src/neighbors.jl:121-127 (capacity 32) */
     s->cap = 32;
     s->nidx = xcalloc((size_t)s->cap * n, sizeof(int32_t), &bad);
     s->ncount = xcalloc(n, sizeof(int32_t), &bad);

@@ -52,7 +52,7 @@ int fcco_get_counters(fcco_sim *s, int64_t *step, double *t, int64_t *clock,
 /* Overwrite the gradients gx, gy (N each) before fcco_integrate_noise. */
 int fcco_set_center_gradients(fcco_sim *s, const double *gx, const double *gy);
 /* integrate!(bodies, grad, disp, ξ, ...) with a 3 x N noise matrix and
- * frame = false (src/dynamics.jl:350-362): used by the parent-drag tests. */
+ * frame = false (src/dynamics.jl:220-232): used by the parent-drag tests. */
 int fcco_integrate_noise(fcco_sim *s, const double *xi);
 /* Build the cell and neighbour lists from given positions and bounds (no
  * simulation state): the bit-exactness check of the device lists. */

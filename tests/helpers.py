@@ -12,7 +12,7 @@ import numpy as np
 
 def contour_direct(lab, R0, K, psi):
     """R_j(ψ) = R0 [a0 + Σ αn cos nψ + βn sin nψ] by direct cos/sin (no
-    recurrence), geometry.jl:259-278 restated via test_contact.jl:149."""
+    recurrence), src/geometry.jl:124-156 restated via test_contact.jl:149."""
     psi = np.asarray(psi, dtype=np.float64)
     r = np.full_like(psi, lab[0])
     for m in range(1, K + 1):
@@ -22,7 +22,7 @@ def contour_direct(lab, R0, K, psi):
 
 
 def rdirect_body(coef, theta, R0, phi):
-    """Body contour from body-frame coefficients, test_geometry.jl:200."""
+    """Body contour from body-frame coefficients, test/test_geometry.jl:35."""
     K = (coef.shape[0] - 1) // 2
     phi = np.asarray(phi, dtype=np.float64)
     r = np.full_like(phi, coef[0])

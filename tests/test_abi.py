@@ -106,7 +106,7 @@ def test_hexagonal_monolayer_geometry():
 
 
 def test_with_nuclei_and_vcat():
-    # test/test_parents.jl:252-271
+    # test/test_parents.jl:160-178
     coef = np.zeros((11, 2))
     coef[0] = 1
     a = F.Bodies([1.0, 2.0], [1.0, 2.0], np.zeros(2), coef, np.ones(2), parent=[1, 1])

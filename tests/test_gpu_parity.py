@@ -99,19 +99,21 @@ LAYOUT_ENERGIES = (F.Tension(0.1), F.OnTypes(F.PerimeterElasticity(0.2, 3.7), 2)
                    F.Overlap([[1.0, 1.0], [1.0, 1.0]], related=[[1.0, -2.0], [-2.0, 1.0]]))
 
 
-def layout_sim(engine, ntheta=40, precision="fp64", v0=0.2, skin=0.3, seed=7):
+def layout_sim(engine, ntheta=40, precision="fp64", v0=0.2, skin=0.3, seed=7, **kw):
     b, box = layout_bodies()
     types = (F.BodyType(v0=v0, Dr=0.1), F.BodyType(Dr=0.1, drag="parent"))
     return F.Simulation(b, box, types=types, energies=LAYOUT_ENERGIES, ntheta=ntheta, precision=precision,
-                        seed=seed, skin=skin, debug=True, engine=engine)
+                        seed=seed, skin=skin, debug=True, engine=engine, **kw)
 
 
-@pytest.mark.parametrize("ntheta", [40, 37])
-def test_gradients_match_oracle_fp64(hip, oracle_engine, ntheta):
+# two_lanes: the two-lane forces kernels (which the library otherwise takes
+# from 131072 bodies on) on the same small systems
+@pytest.mark.parametrize("ntheta,two_lanes", [(40, False), (40, True), (37, False)])
+def test_gradients_match_oracle_fp64(hip, oracle_engine, ntheta, two_lanes):
     # test/test_cuda.jl:10-20: rtol 1e-10, atol 1e-10
-    say("gradients fp64 ntheta", ntheta)
+    say("gradients fp64 ntheta", ntheta, "two lanes" if two_lanes else "")
     a = F.refresh(layout_sim(oracle_engine, ntheta))
-    b = F.refresh(layout_sim(hip, ntheta))
+    b = F.refresh(layout_sim(hip, ntheta, two_lanes=two_lanes))
     ga, gb = a.grad, b.grad
     assert np.count_nonzero(ga.P > 0) > 100
     np.testing.assert_allclose(b.geom.R, a.geom.R, rtol=1e-12, atol=1e-12)
@@ -316,13 +318,14 @@ def clusters(k=12, sites=4, spacing=5.0, jitter=0.15, seed=9, maxmode=6):
     return F.Bodies(x, y, rng.uniform(0, 2 * np.pi, n), coef, np.ones(n)), F.PeriodicBox(L, L)
 
 
+@pytest.mark.parametrize("two_lanes", [False, True])
 @pytest.mark.parametrize("precision", ["fp64", "fp32", "mixed_fp16", "mixed_bf16"])
-def test_dense_clusters_contact_counts(hip, oracle_engine, precision):
-    say("dense clusters", precision)
+def test_dense_clusters_contact_counts(hip, oracle_engine, precision, two_lanes):
+    say("dense clusters", precision, "two lanes" if two_lanes else "")
     b, box = clusters()
     kw = dict(types=(F.BodyType(),), energies=(F.Tension(0.01), F.Overlap(1.0)), debug=True)
     a = F.refresh(F.Simulation(b, box, engine=oracle_engine, **kw))
-    g = F.refresh(F.Simulation(b, box, precision=precision, **kw))
+    g = F.refresh(F.Simulation(b, box, precision=precision, two_lanes=two_lanes, **kw))
     ga, gg = a.grad, g.grad
     assert ga.P.max() >= 8 and np.count_nonzero(ga.P >= 4) > 1000
     if precision == "fp64":

@@ -81,7 +81,7 @@ def test_rotational_diffusion(oracle_engine):
 
 
 def test_same_seed_same_trajectory(oracle_engine):
-    # test/test_random.jl:86-92
+    # test/test_random.jl:51-57
     def mk(seed):
         b, box = F.hexagonal_monolayer(4, 4)
         return F.Simulation(b, box, types=(F.BodyType(v0=0.1, Dr=0.5, Dt=0.01),),

@@ -36,7 +36,7 @@ def test_host_philox_matches_oracle():
 
 @pytest.mark.parametrize("prec", ["fp32", "fp64"])
 def test_normal_noise_statistics(oracle_engine, prec):
-    # test/test_random.jl:48-69
+    # test/test_random.jl:13-34
     xi = fcc_hip.philox_noise(100_000, 42, 7, prec, engine=oracle_engine)
     assert xi.dtype == (np.float32 if prec == "fp32" else np.float64)
     for r in range(3):
@@ -53,7 +53,7 @@ def test_normal_noise_statistics(oracle_engine, prec):
 
 
 def test_draw_index_uses_64_bits(oracle_engine):
-    # test/test_random.jl:81-84
+    # test/test_random.jl:46-49
     a = fcc_hip.philox_noise(1, 1, 1, "fp64", engine=oracle_engine)
     b = fcc_hip.philox_noise(1, 1, 1 + (1 << 32), "fp64", engine=oracle_engine)
     assert not np.array_equal(a, b)
