@@ -1809,6 +1809,57 @@ int fcc_philox_noise(uint64_t seed, uint64_t draw, int64_t n, int32_t precision,
     return FCC_OK;
 }
 
+// Staging buffer of fcc_philox_noise_noalloc: a fixed array of the code
+// object (3 normals of FCC_NOALLOC_MAX_NOISE bodies in FP64, 6 MiB), one copy
+// per GPU, held until the process ends. Its host-side name is not a device
+// address: host code uses the address hipGetSymbolAddress returns for the
+// current device (noalloc_scratch).
+static __device__ __align__(256) double scratch[3 * FCC_NOALLOC_MAX_NOISE];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[NOALLOC_MAX_DEVICES];
+
+// Device address of `scratch` on the current device, looked up on the first
+// call on that device (with it current) and kept per device.
+static int noalloc_scratch(void **out) {
+    int dev = -1;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+        return fail(FCC_ERR_DEVICE, "device %d beyond the %d devices of the scratch table", dev, NOALLOC_MAX_DEVICES);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(FCC_ERR_DEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return FCC_OK;
+}
+
+int fcc_philox_noise_noalloc(uint64_t seed, uint64_t draw, int64_t n, int32_t precision, void *out) {
+    if (n < 0 || !out) return fail(FCC_ERR_ARGUMENT, "bad noise arguments");
+    if (precision != FCC_FP64 && precision != FCC_FP32) return fail(FCC_ERR_ARGUMENT, "precision must be FP64 or FP32");
+    if (n > FCC_NOALLOC_MAX_NOISE)
+        return fail(FCC_ERR_ARGUMENT, "n = %lld exceeds the %d bodies of the fixed noise buffer",
+                    (long long)n, FCC_NOALLOC_MAX_NOISE);
+    if (n == 0) return FCC_OK;
+    void *dv = nullptr;
+    int rc = noalloc_scratch(&dv);
+    if (rc) return rc;
+    const size_t bytes = (size_t)3 * n * (precision == FCC_FP64 ? 8 : 4);
+    const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    if (precision == FCC_FP64)
+        hipLaunchKernelGGL((k_noise<double>), blocks(n, 256), 256, 0, 0, n, k0, k1, draw, (double *)dv);
+    else
+        hipLaunchKernelGGL((k_noise<float>), blocks(n, 256), 256, 0, 0, n, k0, k1, draw, (float *)dv);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpy(out, dv, bytes, hipMemcpyDeviceToHost));
+    return FCC_OK;
+}
+
 int fcc_philox4x32(const uint32_t *ctr, const uint32_t *key, int64_t n, uint32_t *out) {
     if (n < 0 || !ctr || !key || !out) return fail(FCC_ERR_ARGUMENT, "bad philox arguments");
     if (n == 0) return FCC_OK;

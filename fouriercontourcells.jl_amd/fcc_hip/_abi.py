@@ -34,6 +34,8 @@ FCC_TERM_HARMONIC_BOND = 32
 FCC_FLAG_DEBUG_BUFFERS = 1
 FCC_FLAG_TWO_LANES = 2
 
+FCC_NOALLOC_MAX_NOISE = 262144
+
 PRECISIONS = {
     "fp64": FCC_FP64,
     "fp32": FCC_FP32,
@@ -157,6 +159,7 @@ _DEVICE_ONLY = {
     "run": [_vp, _i64],
     "philox4x32": [ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32), _i64,
                    ctypes.POINTER(ctypes.c_uint32)],
+    "philox_noise_noalloc": [ctypes.c_uint64, ctypes.c_uint64, _i64, _i32, _vp],
     "abi_version": [],
     "time_step_kernels": [_vp, _i64, _dp, _dp],
     "run_timed": [_vp, _i64, _dp, _dp],

@@ -208,6 +208,18 @@ int fcc_get_counters(fcc_sim *sim, int64_t *step, double *t, int64_t *clock,
 int fcc_philox_noise(uint64_t seed, uint64_t draw, int64_t n, int32_t precision,
                      void *out);
 
+/* fcc_philox_noise without any device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (serving
+ * loops, harnesses that forbid hipMalloc). The device staging buffer is a
+ * fixed array of the library, one per GPU (3 x FCC_NOALLOC_MAX_NOISE doubles,
+ * 6 MiB, held until the process ends); n above FCC_NOALLOC_MAX_NOISE returns
+ * FCC_ERR_ARGUMENT. Runs on the current device's null stream and returns when
+ * `out` is filled; every call on one GPU shares its buffer, so calls that
+ * overlap on one GPU (from several threads) race. */
+#define FCC_NOALLOC_MAX_NOISE 262144
+int fcc_philox_noise_noalloc(uint64_t seed, uint64_t draw, int64_t n, int32_t precision,
+                             void *out);
+
 /* Raw Philox4x32-10 blocks on the device (src/random.jl:25-36): ctr [4 x n],
  * key [2 x n] -> out [4 x n]. */
 int fcc_philox4x32(const uint32_t *ctr, const uint32_t *key, int64_t n,
