@@ -123,3 +123,35 @@ def test_checkpoint_checks():
     blob2 = s2.checkpoint()
     with pytest.raises(F.ArgumentError):
         F.Simulation(b2, box, **two).restore(blob2)  # other body types
+
+
+def test_resumed_run_matches_oracle_fp64(oracle_engine):
+    # The resumed run against the FP64 oracle's uninterrupted one (device
+    # clock, src/simulation.jl:99; Verlet reference, src/neighbors.jl:234-240):
+    # 150 steps, checkpoint, a fresh simulation restored from it, 50 more
+    # steps; the oracle runs 200 steps. Rotational and translational noise
+    # on, thin skin: the continued run draws the noise of draws 151..200 and
+    # rebuilds where the uninterrupted run does.
+    say("resume vs oracle")
+    b, box = F.hexagonal_monolayer(8, 8, packing=0.95, maxmode=6, seed=5)
+    kw = dict(types=(F.BodyType(v0=0.3, Dr=0.5, Dt=0.01),), energies=(F.Tension(0.01), F.Overlap(1.0)),
+              seed=5, skin=0.05)
+    ref = F.Simulation(b, box, engine=oracle_engine, **kw)
+    F.step(ref, 200)
+    part = F.Simulation(b, box, precision="fp64", **kw)
+    F.run(part, 150)
+    blob = part.checkpoint()
+    part.close()
+    dev = F.Simulation(b, box, precision="fp64", **kw)
+    dev.restore(blob)
+    F.run(dev, 50)
+    bd, br = dev.bodies, ref.bodies
+    cd, cr = dev.counters(), ref.counters()
+    say(f"  builds device {cd.builds}, oracle {cr.builds}; max |dx| {np.max(np.abs(bd.x - br.x)):.2e}")
+    np.testing.assert_allclose(bd.x, br.x, rtol=1e-8)
+    np.testing.assert_allclose(bd.y, br.y, rtol=1e-8)
+    np.testing.assert_allclose(bd.theta, br.theta, rtol=1e-8, atol=1e-12)
+    np.testing.assert_allclose(bd.coef, br.coef, rtol=1e-8, atol=1e-12)
+    assert np.array_equal(bd.imgx, br.imgx) and np.array_equal(bd.imgy, br.imgy)
+    assert cd.clock == cr.clock == 200 and cd.step == cr.step == 200
+    assert cd.builds == cr.builds and cr.builds > 5

@@ -181,3 +181,21 @@ def test_timed_run_equals_run():
     assert f > 0 and u > 0
     for a, b in zip(e1.members, e2.members):
         assert_same(a, b)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "mixed_fp16"])
+def test_large_and_small_replicas_in_one_step(precision):
+    # a replica of 132496 bodies (two-lane forces kernel, from 131072 bodies
+    # on) next to small ones (four-lane kernel): one ensemble step launches
+    # both kernels, the second over the tiles after the first's
+    say("large + small replicas", precision)
+    mm = 6 if precision == "fp32" else 8
+    reps = [fast_replica(seed=41, nx=6, ny=6, maxmode=mm), fast_replica(seed=42, nx=364, ny=364, maxmode=mm),
+            fast_replica(seed=43, nx=20, ny=14, maxmode=mm)]
+    assert len(reps[1].bodies) >= 131072 > max(len(reps[0].bodies), len(reps[2].bodies))
+    seq = [F.run(alone(rp, precision), 30) for rp in reps]
+    ens = F.Ensemble(reps, precision=precision)
+    ens.run(30)
+    for a, b in zip(seq, ens.members):
+        assert_same(a, b)
+    assert seq[1].counters().builds > 1

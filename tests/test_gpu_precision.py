@@ -20,6 +20,7 @@ import numpy as np
 import pytest
 
 import fcc_hip as F
+from helpers import force_scales, plain_force_errors, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -46,32 +47,6 @@ def gradients(b, box, precision, engine=None, energies=None, **kw):
                        precision=precision, debug=True, engine=engine, **kw)
     F.refresh(sim)
     return sim, sim.grad
-
-
-def rel_err(a, ref, scale):
-    return np.abs(a - ref) / np.maximum(np.maximum(np.abs(ref), scale), 1e-300)
-
-
-def force_scales(sim, g):
-    """Σk |Pk| Δφ |nk| and Σk |Pk| Δφ |Rk R'k| per body from the oracle."""
-    geo = sim.geom
-    dphi = 2 * math.pi / sim.ntheta
-    phi = dphi * np.arange(sim.ntheta)[:, None]
-    R, dR = geo.R, geo.dR
-    nx = R * np.cos(phi) + dR * np.sin(phi)
-    ny = R * np.sin(phi) - dR * np.cos(phi)
-    w = np.abs(g.P) * dphi
-    return np.sum(w * np.hypot(nx, ny), axis=0), np.sum(w * np.abs(R * dR), axis=0)
-
-
-def plain_force_errors(dev, ref, ok):
-    """The unnormalised force metrics beside the scale-normalised one, on the
-    bodies without a flipped node: per-body |ΔF| / |F| (median, 99th
-    percentile, maximum) and the field error ‖ΔF‖ / ‖F‖ over all of them."""
-    dF = np.hypot(dev.gx - ref.gx, dev.gy - ref.gy)[ok]
-    Fn = np.hypot(ref.gx, ref.gy)[ok]
-    per = dF / np.maximum(Fn, 1e-300)
-    return np.median(per), np.quantile(per, 0.99), per.max(), np.linalg.norm(dF) / np.linalg.norm(Fn)
 
 
 # two_lanes: the two-lane kernels (taken from 131072 bodies on) at this size
