@@ -9,12 +9,17 @@ step over all bodies; steps run in graph mode (captured hipGraph replayed in
 chunks, one host sync per chunk) with the state resident in HBM.
 
 Prints one JSON line: the mixed-precision path's throughput (`value`), the
-plain FP32 path's for comparison, the roofline of the dominant kernel (forces)
-and of the update kernel from HIP-event timings taken here (back-to-back
-launches, fcc_time_kernels), the FP64 CPU baseline (the oracle, OpenMP over
-bodies) on a bounded sample of the same workload, and the batched-ensemble
-workload (64 replicas of uneven size in one launch per kernel and step) next
-to the same replicas run as independent simulations.
+plain FP32 and the mixed BF16 paths' for comparison, the roofline of the
+dominant kernel (forces: its FP32 vector work against the vector peak, its
+contractions against the matrix-core peak) and of the update kernel from HIP
+events over the timed run (fcc_run_timed; back-to-back figures at its end
+state from fcc_time_kernels beside them), the list-build stages, the
+FP64 CPU baseline (the oracle, OpenMP over bodies) on a bounded sample of the
+same workload, and the batched-ensemble workload (64 replicas of uneven size
+in one launch per kernel and step) next to the same replicas run as
+independent simulations. The timed run and the kernel timing are roctx
+ranges ("timed", "kernel_timing") for rocprofv3 --marker-trace. Fields read from committed profiles name their
+file (`traffic_source`, `issue_source`).
 """
 import argparse
 import json
@@ -32,8 +37,27 @@ import numpy as np  # noqa: E402
 
 import fcc_hip as F  # noqa: E402
 
-PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 (vendor figure, dense matrix / packed vector)
+PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 (vendor figure; the vector rate assumes packed FP32)
+PEAK_FP32_UNPACKED_TFLOPS = 78.65  # the vector rate without packed FP32 (this build compiles without it)
+PEAK_FP16_MFMA_TFLOPS = 2500.0     # MI355X dense FP16 / BF16 matrix cores
 PEAK_HBM_GBS = 8000.0      # MI355X HBM3E
+DTYPES = {"fp64": "f64", "fp32": "f32", "mixed": "mixed_fp16", "mixed_fp16": "mixed_fp16",
+          "mixed_bf16": "mixed_bf16"}
+TRAFFIC_FILE = "profiles/timestep_N262144_hbm_traffic.txt"
+ISSUE_FILE = "profiles/forces_N262144_sq_counters.txt"
+
+
+def marker(name=None):
+    """Open (name) or close (None) a roctx range through torch, so that a
+    rocprofv3 --marker-trace of this run delimits the timed region."""
+    try:
+        import torch
+        if name is None:
+            torch.cuda.nvtx.range_pop()
+        else:
+            torch.cuda.nvtx.range_push(name)
+    except Exception:
+        pass
 
 
 def make_sim(n, precision, seed=1234):
@@ -164,6 +188,13 @@ def kernel_durations(sim, reps=50):
     return f.value, u.value
 
 
+def clock_warmup(sim):
+    """State-preserving kernel work (fcc_time_kernels restores the state it
+    advances) before a timed region: an idle GPU runs at lower clocks for the
+    first milliseconds of work, longer than a few warm-up steps take."""
+    kernel_durations(sim)
+
+
 def binning_durations(sim, reps=20):
     """Cell-list and neighbour-list build of the current state, `reps` times
     (fcc_time_rebuild): average milliseconds of the stages cell index, scan,
@@ -177,16 +208,21 @@ def binning_durations(sim, reps=20):
 
 def binning_roofline(sim, stage_ms, builds, steps):
     """Algorithmic HBM bytes per stage of one list build (N bodies, C cells,
-    S = total neighbour entries) and the achieved GB/s: cell index 4C + 28N
-    (count clear; x, y read, cell id write, count atomic), scan 20C (two
-    passes), scatter 16N, cell sort 8C + 8N, neighbours 28N + 4S + 8C (own x,
-    y, rmax, perm; list and counts written; cell starts and counts)."""
+    S = total neighbour entries, r = bytes of a bound: 4 in FP32 / mixed, 8 in
+    FP64) and the achieved GB/s: cell index 4C + 28N (count clear; x, y
+    read, cell id write, count atomic), scan 20C (two passes), scatter 16N,
+    cell sort 8C + 8N + 2(16 + r)N (segments sorted in place; x, y, rmax
+    gathered into cell-sorted copies), neighbours 8C + (16 + r)N own state +
+    (16 + r)N sorted copies read once + 4(S + N) perm of the candidates in
+    range + 4S list + 4N counts + (16 + r)N Verlet reference (with a skin)."""
     N = sim.N
     cells = sim.cells
     C = cells.ncx * cells.ncy
     S = int(sim.neighbors.count.sum())
+    r = 8 if sim.precision == "fp64" else 4
     names = ["cell_index", "scan", "scatter", "cell_sort", "neighbors"]
-    nbytes = [4 * C + 28 * N, 20 * C, 16 * N, 8 * C + 8 * N, 28 * N + 4 * S + 8 * C]
+    nbytes = [4 * C + 28 * N, 20 * C, 16 * N, 8 * C + 8 * N + 2 * (16 + r) * N,
+              8 * C + 3 * (16 + r) * N + 4 * (S + N) + 4 * S + 4 * N]
     stages = {}
     for n, ms, b in zip(names, stage_ms, nbytes):
         stages[n] = {"ms": ms, "bytes": b, "achieved": b / (ms * 1e-3) / 1e9, "unit": "GB/s",
@@ -211,8 +247,8 @@ def calibrate_copy(nbytes=1 << 30, reps=5):
 
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
-    summary (profiles/timestep_N262144_hbm_traffic.txt), or None."""
-    path = os.path.join(ROOT, "profiles", "timestep_N262144_hbm_traffic.txt")
+    summary (TRAFFIC_FILE), or None."""
+    path = os.path.join(ROOT, TRAFFIC_FILE)
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -224,9 +260,8 @@ def measured_traffic(kernel):
 
 def measured_issue(kernel):
     """(VALU issue fraction, MFMA busy fraction) of `kernel` from the committed
-    rocprofv3 --pmc SQ counter summary (profiles/forces_N262144_sq_counters.txt),
-    or (None, None)."""
-    path = os.path.join(ROOT, "profiles", "forces_N262144_sq_counters.txt")
+    rocprofv3 --pmc SQ counter summary (ISSUE_FILE), or (None, None)."""
+    path = os.path.join(ROOT, ISSUE_FILE)
     if not os.path.exists(path):
         return None, None
     for line in open(path):
@@ -260,13 +295,14 @@ def contact_pairs(sim):
 
 
 def forces_flops(sim, tested, inr):
-    """Algorithmic FLOPs of one forces launch: contour contractions
-    2·[2·(3nθ)·ncoef] per body (synthesis + pullback, src/geometry.jl:114-120,
-    src/gradients.jl:376-382), contact 12 per tested and 7 + 8K per in-range
-    (node, neighbour) pair, and 10 per node for the Tension integrand and its
-    gradient."""
+    """Algorithmic FLOPs of one forces launch as (contractions, vector work):
+    contour contractions 2·[2·(3nθ)·ncoef] per body (synthesis + pullback,
+    src/geometry.jl:113-122, src/gradients.jl:374-386; on the matrix cores in
+    the mixed modes), and the vector work: contact 12 per tested and 7 + 8K
+    per in-range (node, neighbour) pair, 10 per node for the Tension
+    integrand and its gradient."""
     N, nth, nc, K = sim.N, sim.ntheta, sim.ncoef, (sim.ncoef - 1) // 2
-    return N * 2 * (2 * 3 * nth * nc) + 12 * tested + (7 + 8 * K) * inr + 10 * nth * N
+    return N * 2 * (2 * 3 * nth * nc), 12 * tested + (7 + 8 * K) * inr + 10 * nth * N
 
 
 def ensemble_replicas(nrep=64, seed=4):
@@ -417,9 +453,17 @@ def main():
     print(f"[bench] workload N={args.n ** 2} precision={args.precision}", file=sys.stderr, flush=True)
     sim = make_sim(args.n, args.precision)
     F.run(sim, args.warmup)
+    clock_warmup(sim)
     tested0, inr0 = contact_pairs(sim)
+    clock_warmup(sim)
+    marker("timed")
     sec, run_fms, run_ums = timed_run(sim, args.steps)
-    fms, ums = kernel_durations(sim)
+    marker()
+    # kernel timing right after the timed run (an idle GPU clocks down; the
+    # pair count below is host work), at its end state
+    marker("kernel_timing")
+    end_fms, end_ums = kernel_durations(sim)
+    marker()
     tested1, inr1 = contact_pairs(sim)
     tested, inr = (tested0 + tested1) / 2, (inr0 + inr1) / 2
     c = sim.counters()
@@ -430,22 +474,31 @@ def main():
     print(f"[bench] {args.precision}: {value / 1e6:.1f} M body-steps/s ({sec / args.steps * 1e3:.4f} ms/step), "
           f"builds {c.builds}", file=sys.stderr, flush=True)
 
-    plain = None
-    if not args.no_plain and args.precision != "fp32":
-        ps = make_sim(args.n, "fp32")
-        F.run(ps, args.warmup)
-        psec, pfms, pums = timed_run(ps, args.steps)
-        plain = ps.N * args.steps / psec
-        print(f"[bench] fp32: {plain / 1e6:.1f} M body-steps/s", file=sys.stderr, flush=True)
-        ps.close()
+    def other_path(prec):
+        s2 = make_sim(args.n, prec)
+        F.run(s2, args.warmup)
+        clock_warmup(s2)
+        psec, _, _ = timed_run(s2, args.steps)
+        v = s2.N * args.steps / psec
+        print(f"[bench] {prec}: {v / 1e6:.1f} M body-steps/s", file=sys.stderr, flush=True)
+        s2.close()
+        return v
 
-    flops = forces_flops(sim, tested, inr)
-    fkernel = "k_forces_mfma" if args.precision.startswith("mixed") else "k_forces_fast2"
-    achieved = flops / (fms * 1e-3) / 1e12
+    plain = None if args.no_plain or args.precision == "fp32" else other_path("fp32")
+    bf16 = other_path("mixed_bf16") if args.precision in ("mixed", "mixed_fp16") and not args.no_plain else None
+
+    mixed = args.precision.startswith("mixed")
+    # kernel durations sampled over the timed run: flops of its mean state
+    fl_mfma, fl_valu = forces_flops(sim, tested, inr)
+    if not mixed:  # the contractions run on the vector units too
+        fl_valu, fl_mfma = fl_valu + fl_mfma, 0.0
+    fkernel = "k_forces_mfma" if mixed else "k_forces_fast2"
+    t_f = run_fms * 1e-3
+    a_valu, a_mfma = fl_valu / t_f / 1e12, fl_mfma / t_f / 1e12
     valu_frac, mfma_frac = measured_issue(fkernel) if sim.N == 262144 else (None, None)
     upd_bytes = 280 * sim.N  # per body-step, 188 B read + 92 B written
-    print(f"[bench] forces {fms * 1e3:.1f} us, update {ums * 1e3:.1f} us (back-to-back launches; "
-          f"last step of each chunk in the run: {run_fms * 1e3:.1f} / {run_ums * 1e3:.1f} us), "
+    print(f"[bench] forces {run_fms * 1e3:.1f} us, update {run_ums * 1e3:.1f} us (samples over the timed run; "
+          f"back-to-back at its end state: {end_fms * 1e3:.1f} / {end_ums * 1e3:.1f} us), "
           f"pairs tested {tested}, in range {inr}", file=sys.stderr, flush=True)
 
     ensemble = None
@@ -453,6 +506,13 @@ def main():
         ensemble = bench_ensemble(args.ensemble_steps, min(args.warmup, 200), max(1, args.ensemble_steps // 4))
 
     cpu = None if args.no_cpu else cpu_baseline(args.n, args.cpu_seconds)
+    on_profile = sim.N == 262144
+    timing_note = ("HIP events on the simulation's stream around the last step of each graph chunk of the "
+                   "timed run, launched eagerly with the graph's kernels and arguments (fcc_run_timed; roctx "
+                   "range 'timed'); flops of the mean of its first and last state")
+    end_note = ("captured graph of 50 back-to-back launches at the end state of the timed run, right after it, "
+                "with ~400 untimed launches first (fcc_time_kernels; roctx range 'kernel_timing'); the forces "
+                "series re-evaluates one state, whose data stay in L2 between launches")
 
     line = {
         "metric": "body-steps/sec at N=262144 (mixed vs plain FP32, 1 GPU) @ force err ≤1e-4",
@@ -465,7 +525,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if args.precision != "fp64" else "f64",
+        "dtype": DTYPES.get(args.precision, args.precision),
         "data": "synthetic",
         "config": {"workload": f"hexagonal_monolayer({args.n},{args.n}; packing=1.0), N={sim.N}, "
                                f"modes 2..8, ntheta=40, Tension(0.01)+Overlap(1.0), v0=0.1, Dr=0.01, "
@@ -473,16 +533,36 @@ def main():
                    "precision": args.precision, "N": sim.N},
         "plain_fp32_value": plain,
         "mixed_over_plain": (value / plain) if plain else None,
-        "roofline": {"kernel": fkernel, "bound": "mfma", "achieved": achieved,
-                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
-                     "traffic": measured_traffic(fkernel) if sim.N == 262144 else None,
-                     "launch_ms": fms, "launch_ms_in_run": run_fms, "flops_per_launch": flops,
-                     "valu_issue_frac": valu_frac, "mfma_busy_frac": mfma_frac},
+        "mixed_bf16_value": bf16,
+        "roofline": {"kernel": fkernel, "bound": "valu", "achieved": a_valu,
+                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": a_valu / PEAK_FP32_TFLOPS,
+                     "peak_unpacked": PEAK_FP32_UNPACKED_TFLOPS,
+                     "frac_unpacked": a_valu / PEAK_FP32_UNPACKED_TFLOPS,
+                     "traffic": measured_traffic(fkernel) if on_profile else None,
+                     "traffic_source": TRAFFIC_FILE if on_profile else None,
+                     "launch_ms": run_fms, "launch_ms_source": timing_note,
+                     "launch_ms_end_state": end_fms, "launch_ms_end_state_source": end_note,
+                     "pairs_tested": tested, "pairs_in_range": inr,
+                     "flops_per_launch": fl_valu + fl_mfma,
+                     "parts": {
+                         "valu": {"what": "contact + body terms (FP32 vector)", "flops": fl_valu,
+                                  "achieved": a_valu, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                                  "frac": a_valu / PEAK_FP32_TFLOPS},
+                         "mfma": ({"what": "contour synthesis + pullback (FP16/BF16 hi+lo matrix cores; "
+                                           "algorithmic flops counted once)",
+                                   "flops": fl_mfma, "achieved": a_mfma, "peak": PEAK_FP16_MFMA_TFLOPS,
+                                   "unit": "TFLOP/s", "frac": a_mfma / PEAK_FP16_MFMA_TFLOPS}
+                                  if mixed else None)},
+                     "valu_issue_frac": valu_frac, "mfma_busy_frac": mfma_frac,
+                     "issue_source": ISSUE_FILE if on_profile else None},
         "roofline_update": {"kernel": "k_update", "bound": "hbm",
-                            "traffic": measured_traffic("k_update") if sim.N == 262144 else None,
-                            "achieved": upd_bytes / (ums * 1e-3) / 1e9, "peak": PEAK_HBM_GBS,
-                            "unit": "GB/s", "frac": upd_bytes / (ums * 1e-3) / 1e9 / PEAK_HBM_GBS,
-                            "launch_ms": ums, "launch_ms_in_run": run_ums},
+                            "traffic": measured_traffic("k_update") if on_profile else None,
+                            "traffic_source": TRAFFIC_FILE if on_profile else None,
+                            "achieved": upd_bytes / (run_ums * 1e-3) / 1e9, "peak": PEAK_HBM_GBS,
+                            "unit": "GB/s", "frac": upd_bytes / (run_ums * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                            "bytes_per_launch": upd_bytes, "launch_ms": run_ums, "launch_ms_source": timing_note,
+                            "launch_ms_end_state": end_ums,
+                            "launch_ms_end_state_source": end_note + "; update = (steps - forces) / 50"},
         "roofline_binning": binning,
         "cpu_baseline": cpu,
         "ensemble": ensemble,

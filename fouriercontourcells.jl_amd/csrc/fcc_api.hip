@@ -241,6 +241,8 @@ struct Sim final : fcc_sim {
     int64_t nce = 1;
     int32_t *cellid = nullptr, *perm = nullptr, *cstart = nullptr, *ccnt = nullptr, *cfill = nullptr;
     int32_t *ctile = nullptr;  // per-tile totals of the cell-count scan
+    double *xs = nullptr, *ys = nullptr;  // positions and bounds in cell-sorted order (list builds)
+    T *rs = nullptr;
     unsigned ntile = 1;
     uint16_t *asyn = nullptr, *bpull = nullptr;  // MFMA operand tables (mixed modes)
     uint16_t *asyn4 = nullptr, *bpull4 = nullptr;  // the same for the four-lane layout
@@ -563,7 +565,8 @@ struct Sim final : fcc_sim {
         p.small_box = (c->Lx > 4.0 * cutoff && c->Ly > 4.0 * cutoff) ? 0 : 1;
         ntile = blocks(nce, SCAN_TILE);
         if ((rc = alloc(&cellid, N)) || (rc = alloc(&perm, N)) || (rc = alloc(&cstart, nce)) ||
-            (rc = alloc(&ccnt, nce)) || (rc = alloc(&cfill, nce)) || (rc = alloc(&ctile, ntile)))
+            (rc = alloc(&ccnt, nce)) || (rc = alloc(&cfill, nce)) || (rc = alloc(&ctile, ntile)) ||
+            (rc = alloc(&xs, N)) || (rc = alloc(&ys, N)) || (rc = alloc(&rs, N)))
             return rc;
         HIPCHK(hipMemsetAsync(cellid, 0, sizeof(int32_t) * N, stream));
         HIPCHK(hipMemsetAsync(perm, 0, sizeof(int32_t) * N, stream));
@@ -733,7 +736,12 @@ struct Sim final : fcc_sim {
     // scan, scatter, cell sort and neighbours of the first pass.
     int rebuild(hipEvent_t *ev = nullptr) {
         const unsigned gN = blocks(N, 256), gC = blocks(nce, 256);
-        if (ev) HIPCHK(hipEventRecord(ev[0], stream));
+        if (ev) {
+            // timed build: the GPU waits ~60 µs (device wall clock, 100 MHz)
+            // while the host enqueues the stages
+            hipLaunchKernelGGL(k_delay, 1, 64, 0, stream, (int64_t)6000);
+            HIPCHK(hipEventRecord(ev[0], stream));
+        }
         hipLaunchKernelGGL(k_fill_i32, gC, 256, 0, stream, nce, ccnt, 0);
         hipLaunchKernelGGL(k_cell_index, gN, 256, 0, stream, N, d.x, d.y, ncx, ncy, wx, wy, cellid, ccnt);
         if (ev) HIPCHK(hipEventRecord(ev[1], stream));
@@ -742,13 +750,17 @@ struct Sim final : fcc_sim {
         if (ev) HIPCHK(hipEventRecord(ev[2], stream));
         hipLaunchKernelGGL(k_scatter, gN, 256, 0, stream, N, cellid, cfill, perm);
         if (ev) HIPCHK(hipEventRecord(ev[3], stream));
-        hipLaunchKernelGGL(k_cell_sort, gC, 256, 0, stream, nce, cstart, ccnt, perm);
+        hipLaunchKernelGGL((k_cell_sort<T>), gC, 256, 0, stream, nce, cstart, ccnt, perm, d.x, d.y, d.rmax, xs, ys,
+                           rs);
         HIPCHK(hipGetLastError());
+        double *x0 = skin > 0.0 ? d.x0 : nullptr, *y0 = skin > 0.0 ? d.y0 : nullptr;
+        T *r0 = skin > 0.0 ? d.rmax0 : nullptr;
         for (bool first = true;; first = false) {
             hipLaunchKernelGGL(k_clear_status, 1, 1, 0, stream, d.st);
             if (ev && first) HIPCHK(hipEventRecord(ev[4], stream));
-            hipLaunchKernelGGL((k_neighbors<T>), gN, 256, 0, stream, N, d.x, d.y, d.rmax, perm, cstart, ccnt,
-                               p.Lx, p.Ly, ncx, ncy, wx, wy, range + skin, d.cap, d.nidx, d.ncount, d.st);
+            hipLaunchKernelGGL((k_neighbors<T>), gN, 256, 0, stream, N, d.x, d.y, d.rmax, perm, cstart, ccnt, xs, ys,
+                               rs, p.Lx, p.Ly, ncx, ncy, wx, wy, range + skin, d.cap, d.nidx, d.ncount, d.st, x0, y0,
+                               r0);
             if (ev && first) HIPCHK(hipEventRecord(ev[5], stream));
             HIPCHK(hipGetLastError());
             DevStatus st;
@@ -770,8 +782,6 @@ struct Sim final : fcc_sim {
             d.cap = cap;
             gen++;
         }
-        if (skin > 0.0)
-            hipLaunchKernelGGL((k_copy_skin_ref<T>), gN, 256, 0, stream, N, d.x, d.y, d.rmax, d.x0, d.y0, d.rmax0);
         hipLaunchKernelGGL(k_clear_status, 1, 1, 0, stream, d.st);
         HIPCHK(hipGetLastError());
         valid = true;
@@ -789,12 +799,12 @@ struct Sim final : fcc_sim {
         hipLaunchKernelGGL(k_scan_tiles, ntile, SCAN_TILE, 0, sb, nce, ccnt, cstart, ctile);
         hipLaunchKernelGGL(k_scan_offsets, ntile, SCAN_TILE, 0, sb, nce, ctile, cstart, cfill);
         hipLaunchKernelGGL(k_scatter, gN, 256, 0, sb, N, cellid, cfill, perm);
-        hipLaunchKernelGGL(k_cell_sort, gC, 256, 0, sb, nce, cstart, ccnt, perm);
+        hipLaunchKernelGGL((k_cell_sort<T>), gC, 256, 0, sb, nce, cstart, ccnt, perm, d.x, d.y, d.rmax, xs, ys, rs);
         hipLaunchKernelGGL(k_begin_build, 1, 1, 0, sb, d.st);
-        hipLaunchKernelGGL((k_neighbors<T>), gN, 256, 0, sb, N, d.x, d.y, d.rmax, perm, cstart, ccnt, p.Lx, p.Ly,
-                           ncx, ncy, wx, wy, range + skin, d.cap, d.nidx, d.ncount, d.st);
-        if (skin > 0.0)
-            hipLaunchKernelGGL((k_copy_skin_ref<T>), gN, 256, 0, sb, N, d.x, d.y, d.rmax, d.x0, d.y0, d.rmax0);
+        double *x0 = skin > 0.0 ? d.x0 : nullptr, *y0 = skin > 0.0 ? d.y0 : nullptr;
+        T *r0 = skin > 0.0 ? d.rmax0 : nullptr;
+        hipLaunchKernelGGL((k_neighbors<T>), gN, 256, 0, sb, N, d.x, d.y, d.rmax, perm, cstart, ccnt, xs, ys, rs,
+                           p.Lx, p.Ly, ncx, ncy, wx, wy, range + skin, d.cap, d.nidx, d.ncount, d.st, x0, y0, r0);
         hipLaunchKernelGGL(k_finish_build, 1, 1, 0, sb, d.st, d.cap, limit);
         HIPCHK(hipGetLastError());
         return FCC_OK;
@@ -1211,16 +1221,22 @@ struct Sim final : fcc_sim {
     // then `reps` steps (forces + update kernels, which advance the state:
     // an update launched on its own again and again would integrate stale
     // forces and time a state the model never reaches). Forces = series 1 /
-    // reps, update = (series 2 - series 1) / reps. The state is saved before
-    // and restored after, so the simulation is unchanged.
+    // reps, update = (series 2 - series 1) / reps. Each series is a captured
+    // graph, as in fcc_run; the forces series is first replayed untimed
+    // (about 400 launches: an idle GPU runs at lower clocks for the first
+    // milliseconds of work).
+    // The state is saved before and restored after, so the simulation is
+    // unchanged.
     int time_kernels(int64_t reps, double *fms, double *ums) override {
         if (has_contact && !valid) return fail(FCC_ERR_STATE, "neighbour list not built (call fcc_update_neighbors)");
+        if (reps < 1) return fail(FCC_ERR_ARGUMENT, "reps must be positive");
         const auto regions = step_regions();
         size_t bytes = 0;
         for (const auto &q : regions) bytes += (q.second + 255) / 256 * 256;
         char *snap = nullptr;
         HIPCHK(hipMalloc(&snap, bytes));
         hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+        hipGraphExec_t gx[2] = {nullptr, nullptr};
         int rc = FCC_OK;
         auto copy = [&](bool save) -> int {
             size_t off = 0;
@@ -1233,23 +1249,44 @@ struct Sim final : fcc_sim {
             }
             return FCC_OK;
         };
+        // series s: reps forces launches (s = 0) or reps steps (s = 1)
+        auto capture_series = [&](int s, hipGraphExec_t *out) -> int {
+            std::lock_guard<std::mutex> lk(g_capture_lock);
+            hipGraph_t graph;
+            HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            int r = FCC_OK;
+            for (int64_t q = 0; q < reps && !r; q++)
+                if (!(r = launch_forces(0)) && s == 1) r = launch_update();
+            hipError_t e = hipStreamEndCapture(stream, &graph);
+            if (r) return r;
+            HIPCHK(e);
+            e = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            HIPCHK(e);
+            return FCC_OK;
+        };
         for (auto &e : ev)
             if (!rc && hipEventCreate(&e) != hipSuccess) rc = fail(FCC_ERR_DEVICE, "hipEventCreate failed");
+        p.skin_check = (has_contact && skin > 0.0) ? 1 : 0;
+        if (!rc) rc = capture_series(0, &gx[0]);
+        if (!rc) rc = capture_series(1, &gx[1]);
         if (!rc) rc = copy(true);
         if (!rc) {
             hipLaunchKernelGGL(k_clear_status, 1, 1, 0, stream, d.st);  // update = 0: the update kernel runs
-            p.skin_check = (has_contact && skin > 0.0) ? 1 : 0;
-            hipError_t e = hipEventRecord(ev[0], stream);
-            for (int64_t q = 0; q < reps && !rc && e == hipSuccess; q++) rc = launch_forces(0);
-            if (!rc && e == hipSuccess) e = hipEventRecord(ev[1], stream);
-            for (int64_t q = 0; q < reps && !rc && e == hipSuccess; q++)
-                if (!(rc = launch_forces(0))) rc = launch_update();
-            if (!rc && e == hipSuccess) e = hipEventRecord(ev[2], stream);
-            if (!rc && e == hipSuccess) e = hipEventSynchronize(ev[2]);
+            // untimed replays, about 20 ms of work at N = 262144 (at least one)
+            hipError_t e = hipSuccess;
+            for (int64_t w = 0; w < std::max<int64_t>(1, 400 / reps) && e == hipSuccess; w++)
+                e = hipGraphLaunch(gx[0], stream);
+            if (e == hipSuccess) e = hipEventRecord(ev[0], stream);
+            if (e == hipSuccess) e = hipGraphLaunch(gx[0], stream);
+            if (e == hipSuccess) e = hipEventRecord(ev[1], stream);
+            if (e == hipSuccess) e = hipGraphLaunch(gx[1], stream);
+            if (e == hipSuccess) e = hipEventRecord(ev[2], stream);
+            if (e == hipSuccess) e = hipEventSynchronize(ev[2]);
             float a = 0.f, b = 0.f;
-            if (!rc && e == hipSuccess) e = hipEventElapsedTime(&a, ev[0], ev[1]);
-            if (!rc && e == hipSuccess) e = hipEventElapsedTime(&b, ev[1], ev[2]);
-            if (!rc && e != hipSuccess) rc = fail(FCC_ERR_DEVICE, "kernel timing: %s", hipGetErrorString(e));
+            if (e == hipSuccess) e = hipEventElapsedTime(&a, ev[0], ev[1]);
+            if (e == hipSuccess) e = hipEventElapsedTime(&b, ev[1], ev[2]);
+            if (e != hipSuccess) rc = fail(FCC_ERR_DEVICE, "kernel timing: %s", hipGetErrorString(e));
             if (!rc) {
                 if (fms) *fms = a / (double)reps;
                 if (ums) *ums = (b - a) / (double)reps;
@@ -1258,6 +1295,8 @@ struct Sim final : fcc_sim {
             if (!rc) rc = rc2;
         }
         (void)hipStreamSynchronize(stream);
+        for (hipGraphExec_t g : gx)
+            if (g) (void)hipGraphExecDestroy(g);
         for (hipEvent_t e : ev)
             if (e) (void)hipEventDestroy(e);
         (void)hipFree(snap);

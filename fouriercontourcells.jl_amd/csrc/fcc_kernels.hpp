@@ -871,12 +871,17 @@ __global__ __launch_bounds__(256) void k_scatter(int64_t N, const int32_t *celli
 }
 
 // Order each cell's bodies by index: the counting sort becomes the stable
-// sortperm! of src/neighbors.jl:86.
+// sortperm! of src/neighbors.jl:86. The positions and bounds of the sorted
+// bodies are copied to xs, ys, rs (cell-sorted order) for the neighbour
+// kernel, whose candidates are then contiguous runs of these arrays.
+template <typename T>
 __global__ __launch_bounds__(256) void k_cell_sort(int64_t nce, const int32_t *start, const int32_t *count,
-                                                   int32_t *perm) {
+                                                   int32_t *perm, const double *x, const double *y,
+                                                   const T *rmax, double *xs, double *ys, T *rs) {
     const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (c >= nce) return;
-    int32_t *seg = perm + start[c];
+    const int32_t s0 = start[c];
+    int32_t *seg = perm + s0;
     const int n = count[c];
     for (int a = 1; a < n; a++) {
         int32_t v = seg[a];
@@ -887,19 +892,71 @@ __global__ __launch_bounds__(256) void k_cell_sort(int64_t nce, const int32_t *s
         }
         seg[b + 1] = v;
     }
+    for (int a = 0; a < n; a++) {
+        const int32_t j = seg[a];
+        xs[s0 + a] = x[j];
+        ys[s0 + a] = y[j];
+        rs[s0 + a] = rmax[j];
+    }
 }
 
 // neighbor_kernel! (src/neighbors.jl:148-174) for body i: 3×3 cells, oy
 // outer, ox inner, cell members in sorted order; entries beyond `cap` are
-// counted only. Returns the count.
+// counted only. Returns the count. Candidates are read from the cell-sorted
+// copies (xs, ys, rs at sorted position q hold the values of body perm[q] bit
+// for bit): the cells ox = -1, 0, +1 of a row that does not wrap are
+// consecutive cell ids, so their members are one run of sorted positions in
+// exactly the reference's order; a wrapping row is scanned cell by cell. A
+// run is read in batches of four candidates (position, bound and body id)
+// whose loads are issued together.
 template <typename T>
-__device__ __forceinline__ int32_t neighbors_of(int64_t i, int64_t N, const double *x, const double *y,
-                                                const T *rmax, const int32_t *perm, const int32_t *start,
-                                                const int32_t *count, double Lx, double Ly, int32_t ncx,
+__device__ __forceinline__ void neighbor_run(int64_t i, int64_t N, double xi, double yi, double ri, int32_t a,
+                                             int32_t b, const int32_t *perm, const double *xs, const double *ys,
+                                             const T *rs, double Lx, double Ly, double qLx, double qLy,
+                                             int32_t cap, int32_t *nidx, int32_t &n) {
+#pragma clang fp contract(off)
+    for (int32_t q = a; q < b; q += 4) {
+        double cx[4], cy[4];
+        T cr[4];
+        int32_t cj[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int32_t qq = q + u < b ? q + u : b - 1;  // in bounds: a <= q < b
+            cx[u] = xs[qq];
+            cy[u] = ys[qq];
+            cr[u] = rs[qq];
+            cj[u] = perm[qq];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            if (q + u >= b) break;
+            // minimum_image (src/box.jl:14); below a quarter box the rounded
+            // quotient is +-0 and the result is dx itself, so the division
+            // is skipped there (same value, bit for bit)
+            double dx = xi - cx[u];
+            if (!(fabs(dx) < qLx)) dx = dx - Lx * rint(dx / Lx);
+            double dy = yi - cy[u];
+            if (!(fabs(dy) < qLy)) dy = dy - Ly * rint(dy / Ly);
+            const double lim = ri + (double)cr[u];
+            if (dx * dx + dy * dy <= lim * lim) {
+                const int32_t j = cj[u];
+                if (j == i) continue;
+                n++;
+                if (n <= cap) nidx[(int64_t)(n - 1) * N + i] = j;
+            }
+        }
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ int32_t neighbors_of(int64_t i, int64_t N, double xi, double yi, T rmaxi,
+                                                const int32_t *perm, const int32_t *start,
+                                                const int32_t *count, const double *xs, const double *ys,
+                                                const T *rs, double Lx, double Ly, int32_t ncx,
                                                 int32_t ncy, double wx, double wy, double range, int32_t cap,
                                                 int32_t *nidx) {
 #pragma clang fp contract(off)
-    const double xi = x[i], yi = y[i], ri = (double)rmax[i] + range;
+    const double ri = (double)rmaxi + range;
     int32_t cx = (int32_t)(xi / wx), cy = (int32_t)(yi / wy);
     cx = cx < ncx - 1 ? cx : ncx - 1;
     cy = cy < ncy - 1 ? cy : ncy - 1;
@@ -907,50 +964,70 @@ __device__ __forceinline__ int32_t neighbors_of(int64_t i, int64_t N, const doub
     cy = cy > 0 ? cy : 0;
     const int32_t rx = ncx >= 3 ? 1 : 0, ry = ncy >= 3 ? 1 : 0;
     const double qLx = 0.25 * Lx, qLy = 0.25 * Ly;
+    // the row's cells are consecutive ids when cx - rx .. cx + rx stay inside
+    const bool row_run = cx - rx >= 0 && cx + rx <= ncx - 1;
     int32_t n = 0;
     for (int32_t oy = -ry; oy <= ry; oy++) {
         int32_t yy = (cy + oy) % ncy;
         yy = yy < 0 ? yy + ncy : yy;
+        if (row_run) {
+            const int32_t c0 = cx - rx + ncx * yy, c1 = cx + rx + ncx * yy;
+            neighbor_run<T>(i, N, xi, yi, ri, start[c0], start[c1] + count[c1], perm, xs, ys, rs, Lx, Ly, qLx,
+                            qLy, cap, nidx, n);
+            continue;
+        }
         for (int32_t ox = -rx; ox <= rx; ox++) {
             int32_t xx = (cx + ox) % ncx;
             xx = xx < 0 ? xx + ncx : xx;
             const int32_t cell = xx + ncx * yy;
-            const int32_t s0 = start[cell], s1 = s0 + count[cell];
-            for (int32_t q = s0; q < s1; q++) {
-                const int32_t j = perm[q];
-                if (j == i) continue;
-                // minimum_image (src/box.jl:14); below a quarter box the
-                // rounded quotient is +-0 and the result is dx itself, so
-                // the division is skipped there (same value, bit for bit)
-                double dx = xi - x[j];
-                if (!(fabs(dx) < qLx)) dx = dx - Lx * rint(dx / Lx);
-                double dy = yi - y[j];
-                if (!(fabs(dy) < qLy)) dy = dy - Ly * rint(dy / Ly);
-                const double lim = ri + (double)rmax[j];
-                if (dx * dx + dy * dy <= lim * lim) {
-                    n++;
-                    if (n <= cap) nidx[(int64_t)(n - 1) * N + i] = j;
-                }
-            }
+            const int32_t s0 = start[cell];
+            neighbor_run<T>(i, N, xi, yi, ri, s0, s0 + count[cell], perm, xs, ys, rs, Lx, Ly, qLx, qLy, cap,
+                            nidx, n);
         }
     }
     return n;
 }
 
-// The largest count goes to st->overflow.
+// The largest count goes to st->overflow (one atomic per wave). With x0
+// non-null the body's position and bound are also stored as the reference
+// state of the Verlet skin (src/neighbors.jl:234-238). Workgroups take their
+// bodies in XCD-grouped order: each XCD's L2 then holds the candidates of a
+// contiguous band of bodies.
 template <typename T>
 __global__ __launch_bounds__(256) void k_neighbors(int64_t N, const double *x, const double *y, const T *rmax,
                                                    const int32_t *perm, const int32_t *start,
-                                                   const int32_t *count, double Lx, double Ly, int32_t ncx,
+                                                   const int32_t *count, const double *xs, const double *ys,
+                                                   const T *rs, double Lx, double Ly, int32_t ncx,
                                                    int32_t ncy, double wx, double wy, double range,
                                                    int32_t cap, int32_t *nidx, int32_t *ncount,
-                                                   DevStatus *st) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= N) return;
-    const int32_t n = neighbors_of<T>(i, N, x, y, rmax, perm, start, count, Lx, Ly, ncx, ncy, wx, wy, range,
-                                      cap, nidx);
-    ncount[i] = n;
-    atomicMax(&st->overflow, n);
+                                                   DevStatus *st, double *x0, double *y0, T *rmax0) {
+    const int64_t i = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    int32_t n = 0;
+    if (i < N) {
+        const double xi = x[i], yi = y[i];
+        const T rmi = rmax[i];
+        n = neighbors_of<T>(i, N, xi, yi, rmi, perm, start, count, xs, ys, rs, Lx, Ly, ncx, ncy, wx, wy, range,
+                            cap, nidx);
+        ncount[i] = n;
+        if (x0) {
+            x0[i] = xi;
+            y0[i] = yi;
+            rmax0[i] = rmi;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) n = max(n, __shfl_xor(n, off));
+    if ((threadIdx.x & 63) == 0) atomicMax(&st->overflow, n);
+}
+
+// Keeps one wave waiting for about `ticks` of the device's constant-rate
+// wall clock (at most ~10^6 sleeps, so it always ends). Launched before a
+// timed sequence of short kernels so that the host has enqueued all of them
+// before the first timing event is reached: the events then measure kernel
+// durations, not host launch gaps.
+__global__ void k_delay(int64_t ticks) {
+    const long long t0 = wall_clock64();
+    for (int it = 0; it < 1000000 && wall_clock64() - t0 < ticks; it++) __builtin_amdgcn_s_sleep(2);
 }
 
 __global__ __launch_bounds__(256) void k_fill_i32(int64_t n, int32_t *a, int32_t v) {
@@ -1004,16 +1081,6 @@ __global__ void k_ensemble_targets(DevStatus *st, int n, int64_t steps) {
     const int64_t tg = steps < 0 ? INT64_MAX : st[r].clock + steps;
     st[r].target = tg;
     st[r].stop = st[r].clock >= tg ? 1 : 0;
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void k_copy_skin_ref(int64_t N, const double *x, const double *y,
-                                                       const T *rmax, double *x0, double *y0, T *rmax0) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= N) return;
-    x0[i] = x[i];
-    y0[i] = y[i];
-    rmax0[i] = rmax[i];
 }
 
 // ------------------------------------------------------------ noise / misc

@@ -29,9 +29,11 @@ int fcc_run_timed(fcc_sim *sim, int64_t n, double *forces_ms, double *update_ms)
 
 /* Average duration of the forces kernel and of the update kernel(s) of a
  * step, in milliseconds, from two series between HIP events on the
- * simulation's stream: `reps` back-to-back forces launches (evaluating the
- * current state), then `reps` steps (forces + update); update = (steps -
- * forces) / reps. The state is saved before and restored afterwards: the
+ * simulation's stream, each a captured graph as in fcc_run (the forces
+ * series is first replayed untimed, about 400 launches in all, for the clock
+ * ramp of an idle GPU):
+ * `reps` back-to-back forces launches (evaluating the current state), then
+ * `reps` steps (forces + update); update = (steps - forces) / reps. The state is saved before and restored afterwards: the
  * simulation is left unchanged. */
 int fcc_time_kernels(fcc_sim *sim, int64_t reps, double *forces_ms, double *update_ms);
 
@@ -39,7 +41,11 @@ int fcc_time_kernels(fcc_sim *sim, int64_t reps, double *forces_ms, double *upda
  * (as fcc_update_neighbors(sim, 1)), with HIP events between the stages;
  * stage_ms[5] receives the average duration in milliseconds of: cell index
  * (count clear + cell ids and counts), scan (two launches), scatter, cell
- * sort, neighbour list. Counts as `reps` list builds. */
+ * sort (with the cell-sorted copies of positions and bounds), neighbour list
+ * (with the Verlet reference). Each build starts behind a short device wait,
+ * so the host has enqueued its stages before the first event: the figures
+ * are kernel durations (plus the gap between the two launches of a stage).
+ * Counts as `reps` list builds. */
 int fcc_time_rebuild(fcc_sim *sim, int64_t reps, double *stage_ms);
 
 /* fcc_ensemble_run with the same eager last step per chunk between HIP
