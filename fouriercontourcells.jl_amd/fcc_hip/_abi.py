@@ -217,6 +217,11 @@ class Engine:
 
 
 def library_path():
+    """The in-tree build (csrc/libfcc_hip.so), or the file named by
+    FCC_HIP_LIBRARY (another build of the same C ABI)."""
+    other = os.environ.get("FCC_HIP_LIBRARY")
+    if other:
+        return other
     here = os.path.dirname(os.path.abspath(__file__))
     return os.path.join(os.path.dirname(here), "csrc", "libfcc_hip.so")
 
