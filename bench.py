@@ -503,7 +503,8 @@ def main():
 
     ensemble = None
     if not args.no_ensemble:
-        ensemble = bench_ensemble(args.ensemble_steps, min(args.warmup, 200), max(1, args.ensemble_steps // 4))
+        ensemble = bench_ensemble(args.ensemble_steps, min(args.warmup, 200), max(1, args.ensemble_steps // 4),
+                                  independent=not args.no_independent)
 
     cpu = None if args.no_cpu else cpu_baseline(args.n, args.cpu_seconds)
     on_profile = sim.N == 262144
