@@ -198,7 +198,8 @@ def clock_warmup(sim):
 def binning_durations(sim, reps=20):
     """Cell-list and neighbour-list build of the current state, `reps` times
     (fcc_time_rebuild): average milliseconds of the stages cell index, scan,
-    scatter, cell sort, neighbours, from HIP events between them."""
+    scatter, cell sort, neighbours, from HIP events recorded by the dispatches
+    of each stage's first and last kernel (kernel durations)."""
     out = (ctypes.c_double * 5)()
     rc = sim.engine.time_rebuild(sim._h, int(reps), out)
     if rc != 0:

@@ -734,34 +734,27 @@ struct Sim final : fcc_sim {
     // update_celllist! + update_neighbors!(nl) (src/neighbors.jl:81-92,223-242).
     // `ev` (bench timing): six events recorded around the stages cell index,
     // scan, scatter, cell sort and neighbours of the first pass.
+    // The stages of a timed build (ev non-null) record ev[2s] at the start of
+    // stage s's first kernel and ev[2s + 1] at the end of its last one, by
+    // the dispatches themselves (hipExtLaunchKernelGGL).
     int rebuild(hipEvent_t *ev = nullptr) {
         const unsigned gN = blocks(N, 256), gC = blocks(nce, 256);
-        if (ev) {
-            // timed build: the GPU waits ~60 µs (device wall clock, 100 MHz)
-            // while the host enqueues the stages
-            hipLaunchKernelGGL(k_delay, 1, 64, 0, stream, (int64_t)6000);
-            HIPCHK(hipEventRecord(ev[0], stream));
-        }
-        hipLaunchKernelGGL(k_fill_i32, gC, 256, 0, stream, nce, ccnt, 0);
-        hipLaunchKernelGGL(k_cell_index, gN, 256, 0, stream, N, d.x, d.y, ncx, ncy, wx, wy, cellid, ccnt);
-        if (ev) HIPCHK(hipEventRecord(ev[1], stream));
-        hipLaunchKernelGGL(k_scan_tiles, ntile, SCAN_TILE, 0, stream, nce, ccnt, cstart, ctile);
-        hipLaunchKernelGGL(k_scan_offsets, ntile, SCAN_TILE, 0, stream, nce, ctile, cstart, cfill);
-        if (ev) HIPCHK(hipEventRecord(ev[2], stream));
-        hipLaunchKernelGGL(k_scatter, gN, 256, 0, stream, N, cellid, cfill, perm);
-        if (ev) HIPCHK(hipEventRecord(ev[3], stream));
-        hipLaunchKernelGGL((k_cell_sort<T>), gC, 256, 0, stream, nce, cstart, ccnt, perm, d.x, d.y, d.rmax, xs, ys,
-                           rs);
+        auto E = [&](int q) -> hipEvent_t { return ev ? ev[q] : nullptr; };
+        FCC_LAUNCH(k_fill_i32, gC, 256, 0, stream, E(0), nullptr, nce, ccnt, 0);
+        FCC_LAUNCH(k_cell_index, gN, 256, 0, stream, nullptr, E(1), N, d.x, d.y, ncx, ncy, wx, wy, cellid, ccnt);
+        FCC_LAUNCH(k_scan_tiles, ntile, SCAN_TILE, 0, stream, E(2), nullptr, nce, ccnt, cstart, ctile);
+        FCC_LAUNCH(k_scan_offsets, ntile, SCAN_TILE, 0, stream, nullptr, E(3), nce, ctile, cstart, cfill);
+        FCC_LAUNCH(k_scatter, gN, 256, 0, stream, E(4), E(5), N, cellid, cfill, perm);
+        FCC_LAUNCH((k_cell_sort<T>), gC, 256, 0, stream, E(6), E(7), nce, cstart, ccnt, perm, d.x, d.y, d.rmax, xs,
+                   ys, rs);
         HIPCHK(hipGetLastError());
         double *x0 = skin > 0.0 ? d.x0 : nullptr, *y0 = skin > 0.0 ? d.y0 : nullptr;
         T *r0 = skin > 0.0 ? d.rmax0 : nullptr;
         for (bool first = true;; first = false) {
             hipLaunchKernelGGL(k_clear_status, 1, 1, 0, stream, d.st);
-            if (ev && first) HIPCHK(hipEventRecord(ev[4], stream));
-            hipLaunchKernelGGL((k_neighbors<T>), gN, 256, 0, stream, N, d.x, d.y, d.rmax, perm, cstart, ccnt, xs, ys,
-                               rs, p.Lx, p.Ly, ncx, ncy, wx, wy, range + skin, d.cap, d.nidx, d.ncount, d.st, x0, y0,
-                               r0);
-            if (ev && first) HIPCHK(hipEventRecord(ev[5], stream));
+            FCC_LAUNCH((k_neighbors<T>), gN, 256, 0, stream, first ? E(8) : nullptr, first ? E(9) : nullptr, N, d.x,
+                       d.y, d.rmax, perm, cstart, ccnt, xs, ys, rs, p.Lx, p.Ly, ncx, ncy, wx, wy, range + skin, d.cap,
+                       d.nidx, d.ncount, d.st, x0, y0, r0);
             HIPCHK(hipGetLastError());
             DevStatus st;
             HIPCHK(hipMemcpyAsync(&st, d.st, sizeof st, hipMemcpyDeviceToHost, stream));
@@ -880,7 +873,7 @@ struct Sim final : fcc_sim {
 
     // `reps` forced list rebuilds of the current state, timed per stage.
     int time_rebuild(int64_t reps, double *stage_ms) override {
-        hipEvent_t ev[6] = {};
+        hipEvent_t ev[10] = {};
         for (auto &e : ev) HIPCHK(hipEventCreate(&e));
         double acc[5] = {0, 0, 0, 0, 0};
         int rc = FCC_OK;
@@ -888,7 +881,7 @@ struct Sim final : fcc_sim {
             if ((rc = rebuild(ev))) break;
             for (int k = 0; k < 5; k++) {
                 float ms = 0.f;
-                hipError_t e = hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
+                hipError_t e = hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]);
                 if (e != hipSuccess) {
                     rc = fail(FCC_ERR_DEVICE, "hipEventElapsedTime: %s", hipGetErrorString(e));
                     break;

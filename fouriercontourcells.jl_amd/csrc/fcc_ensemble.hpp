@@ -33,6 +33,7 @@ static int check_launch(const fcc_launch *lc) {
         return fail(FCC_ERR_ARGUMENT, "update_block must be 64, 128 or 256");
     if (lc->xcd_order != 0 && lc->xcd_order != 1) return fail(FCC_ERR_ARGUMENT, "xcd_order must be 0 or 1");
     if (lc->chunk < 0) return fail(FCC_ERR_ARGUMENT, "chunk must be non-negative");
+    if (lc->lanes != 0 && lc->lanes != 2) return fail(FCC_ERR_ARGUMENT, "lanes must be 0 or 2");
     return FCC_OK;
 }
 
@@ -273,7 +274,8 @@ struct Ensemble final : fcc_ensemble {
             HIPCHK(hipMemcpyAsync(dD, upD.data(), sizeof(Ptrs<T>) * R, hipMemcpyHostToDevice, stream));
             uploaded = true;
         }
-        if (var != tiles_var || lc.update_block != tiles_lc.update_block || lc.xcd_order != tiles_lc.xcd_order) {
+        if (var != tiles_var || lc.update_block != tiles_lc.update_block || lc.xcd_order != tiles_lc.xcd_order ||
+            lc.lanes != tiles_lc.lanes) {
             std::vector<int64_t> sizes(R), small(R, 0), large(R, 0);
             for (size_t r = 0; r < R; r++) sizes[r] = m[r]->N;
             // Bodies per forces workgroup: 16 for the four-lane kernels (replicas
@@ -548,7 +550,31 @@ struct Ensemble final : fcc_ensemble {
         int rc = check_launch(in);
         if (rc) return rc;
         lc = *in;
+        apply_lanes();
         return FCC_OK;
+    }
+
+    // The lane choice of the launch shape on every replica: the four-lane
+    // threshold each uses alone and inside the ensemble (a member's own
+    // captured step is recaptured when it changes).
+    void apply_lanes() {
+        for (Sim<T> *s : m) {
+            const int64_t fb = (lc.lanes == 2 || (s->cfg.flags & FCC_FLAG_TWO_LANES)) ? 0 : FOUR_LANES_BELOW;
+            if (fb != s->four_below) {
+                s->four_below = fb;
+                s->gen++;
+            }
+        }
+    }
+
+    // Whether the lane choice changes any replica's forces kernel.
+    bool lanes_matter() const {
+        bool fast = m[0]->nth == 40;
+        for (const Sim<T> *s : m) fast = fast && s->fast_ok();
+        if (!fast || (m[0]->cfg.flags & FCC_FLAG_TWO_LANES)) return false;
+        for (const Sim<T> *s : m)
+            if (s->N < FOUR_LANES_BELOW) return true;
+        return false;
     }
 
     // ----------------------------------------------------------- autotuner
@@ -607,9 +633,12 @@ struct Ensemble final : fcc_ensemble {
         if (!rc && hipEventCreate(&e0) != hipSuccess) rc = fail(FCC_ERR_DEVICE, "hipEventCreate failed");
         if (!rc && hipEventCreate(&e1) != hipSuccess) rc = fail(FCC_ERR_DEVICE, "hipEventCreate failed");
         const int ubs[3] = {64, 128, 256};
+        const int nl = lanes_matter() ? 2 : 1;
+        for (int li = 0; li < nl && !rc; li++)
         for (int xo = 0; xo < 2 && !rc; xo++) {
             for (int u = 0; u < 3 && !rc; u++) {
-                lc = fcc_launch{ubs[u], xo, keep.chunk, 0};
+                lc = fcc_launch{ubs[u], xo, keep.chunk, nl == 1 ? keep.lanes : 2 * li};
+                apply_lanes();
                 if ((rc = prepare()) || (rc = copy_regions(regions, snap, false))) break;
                 hipError_t e = hipSuccess;
                 for (int q = 0; q < 2 && e == hipSuccess; q++) e = hipGraphLaunch(gexec, stream);
@@ -633,6 +662,7 @@ struct Ensemble final : fcc_ensemble {
         }
         if (!rc) rc = copy_regions(regions, snap, false);
         lc = rc ? keep : win;
+        apply_lanes();
         if (!rc) rc = prepare();
         (void)hipStreamSynchronize(stream);
         if (e0) (void)hipEventDestroy(e0);

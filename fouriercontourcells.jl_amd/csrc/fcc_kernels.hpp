@@ -1020,16 +1020,6 @@ __global__ __launch_bounds__(256) void k_neighbors(int64_t N, const double *x, c
     if ((threadIdx.x & 63) == 0) atomicMax(&st->overflow, n);
 }
 
-// Keeps one wave waiting for about `ticks` of the device's constant-rate
-// wall clock (at most ~10^6 sleeps, so it always ends). Launched before a
-// timed sequence of short kernels so that the host has enqueued all of them
-// before the first timing event is reached: the events then measure kernel
-// durations, not host launch gaps.
-__global__ void k_delay(int64_t ticks) {
-    const long long t0 = wall_clock64();
-    for (int it = 0; it < 1000000 && wall_clock64() - t0 < ticks; it++) __builtin_amdgcn_s_sleep(2);
-}
-
 __global__ __launch_bounds__(256) void k_fill_i32(int64_t n, int32_t *a, int32_t v) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i < n) a[i] = v;

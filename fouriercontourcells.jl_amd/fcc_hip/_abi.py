@@ -105,7 +105,7 @@ class FccLaunch(ctypes.Structure):
         ("update_block", ctypes.c_int32),
         ("xcd_order", ctypes.c_int32),
         ("chunk", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("lanes", ctypes.c_int32),
     ]
 
 

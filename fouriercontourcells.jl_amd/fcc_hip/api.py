@@ -599,12 +599,13 @@ class Ensemble:
     def launch(self):
         lc = _abi.FccLaunch()
         _check(self.engine, self.engine.ensemble_get_launch(self._h, ctypes.byref(lc)))
-        return {"update_block": lc.update_block, "xcd_order": lc.xcd_order, "chunk": lc.chunk}
+        return {"update_block": lc.update_block, "xcd_order": lc.xcd_order, "chunk": lc.chunk,
+                "lanes": lc.lanes}
 
     @launch.setter
     def launch(self, v):
         lc = _abi.FccLaunch(int(v.get("update_block", 256)), int(v.get("xcd_order", 1)),
-                            int(v.get("chunk", 0)), 0)
+                            int(v.get("chunk", 0)), int(v.get("lanes", 0)))
         _check(self.engine, self.engine.ensemble_set_launch(self._h, ctypes.byref(lc)))
 
     def observer_start(self, nbins=256, prange=(3.5, 5.5), max_samples=1024):

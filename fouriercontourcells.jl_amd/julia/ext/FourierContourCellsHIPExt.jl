@@ -430,7 +430,7 @@ end
 # and step (README.md:249-261 runs them as concurrent tasks instead)
 
 struct FccLaunch
-    update_block::Int32; xcd_order::Int32; chunk::Int32; reserved::Int32
+    update_block::Int32; xcd_order::Int32; chunk::Int32; lanes::Int32
 end
 
 mutable struct HIPEnsembleHandle

@@ -322,7 +322,10 @@ typedef struct {
     int32_t update_block; /* bodies per workgroup of the update kernel: 64, 128 or 256 */
     int32_t xcd_order;    /* 1: each XCD runs one contiguous range of tiles (8 XCDs) */
     int32_t chunk;        /* graph replays between host status reads; 0 = adaptive */
-    int32_t reserved;
+    int32_t lanes;        /* lanes per body of the n_theta = 40 forces kernels: 0 = each
+                             replica's own (four below 131072 bodies, two with
+                             FCC_FLAG_TWO_LANES), 2 = two for every replica from now on, as
+                             if created with FCC_FLAG_TWO_LANES (member handles included) */
 } fcc_launch;
 
 /* Create n replicas from cfgs[n] and bodies[n] (validated like fcc_create).
@@ -342,9 +345,11 @@ int fcc_ensemble_get_launch(fcc_ensemble *ens, fcc_launch *launch);
 int fcc_ensemble_set_launch(fcc_ensemble *ens, const fcc_launch *launch);
 
 /* Autotuner: replays `steps` ensemble steps for every candidate launch shape
- * (update_block x xcd_order) from the same state, restored after each trial
- * and at the end (the run state is unchanged), keeps the fastest and returns
- * it with its device time per step in milliseconds. */
+ * (update_block x xcd_order, and lanes 0 / 2 when some replica would take
+ * the four-lane kernels) from the same state, restored after each trial and
+ * at the end (the run state is unchanged), keeps the fastest and returns it
+ * with its device time per step in milliseconds. With lanes = 2 chosen, the
+ * replicas continue as replicas created with FCC_FLAG_TWO_LANES would. */
 int fcc_ensemble_autotune(fcc_ensemble *ens, int64_t steps, fcc_launch *best,
                           double *ms_per_step);
 

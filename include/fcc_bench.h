@@ -42,10 +42,10 @@ int fcc_time_kernels(fcc_sim *sim, int64_t reps, double *forces_ms, double *upda
  * stage_ms[5] receives the average duration in milliseconds of: cell index
  * (count clear + cell ids and counts), scan (two launches), scatter, cell
  * sort (with the cell-sorted copies of positions and bounds), neighbour list
- * (with the Verlet reference). Each build starts behind a short device wait,
- * so the host has enqueued its stages before the first event: the figures
- * are kernel durations (plus the gap between the two launches of a stage).
- * Counts as `reps` list builds. */
+ * (with the Verlet reference). The events of a stage are recorded by the
+ * dispatches of its first and last kernel (start and end), so the figures are
+ * kernel durations (plus the gap between the two launches of a two-kernel
+ * stage). Counts as `reps` list builds. */
 int fcc_time_rebuild(fcc_sim *sim, int64_t reps, double *stage_ms);
 
 /* fcc_ensemble_run with the same eager last step per chunk between HIP

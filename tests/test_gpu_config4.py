@@ -6,7 +6,7 @@ bench.py's ensemble leg), as the reference's concurrent simulations
 (README.md:249-261, test/test_cuda.jl:95-104): launch shape from the
 autotuner, 200 steps as one ensemble; the smallest and the largest replica,
 the fastest (v0 = 0.2) and one with Dr = 0.02 end bitwise where they end when
-run alone, and the smallest replica's end state has gradients within the
+run alone (with the lane count per body the autotuner chose), and the smallest replica's end state has gradients within the
 force tolerance of the FP64 oracle's on the same state."""
 import sys
 
@@ -52,7 +52,7 @@ def test_config4_workload(oracle_engine):
     for r in picks:
         rp = reps[r]
         alone = F.Simulation(rp.bodies, rp.box, types=rp.types, energies=rp.energies, dt=rp.dt, seed=rp.seed,
-                             skin=rp.skin, precision="fp32")
+                             skin=rp.skin, precision="fp32", two_lanes=launch["lanes"] == 2)
         F.run(alone, STEPS)
         (sa, ca), (se, ce) = state(alone), state(ens.members[r])
         for u, w in zip(sa, se):

@@ -27,9 +27,9 @@ def fast_replica(seed, nx=8, ny=8, v0=2.0, Dr=0.5, eta=0.01, dt=0.005, maxmode=6
                      energies=(F.Tension(eta), F.Overlap(1.0)), dt=dt, seed=seed, skin=skin)
 
 
-def alone(rp, precision):
+def alone(rp, precision, two_lanes=False):
     return F.Simulation(rp.bodies, rp.box, types=rp.types, energies=rp.energies, dt=rp.dt, seed=rp.seed,
-                        skin=rp.skin, precision=precision)
+                        skin=rp.skin, precision=precision, two_lanes=two_lanes)
 
 
 def state(s):
@@ -75,30 +75,55 @@ def test_uneven_sizes_and_parameters(precision, maxmode):
         assert_same(a, b)
 
 
-@pytest.mark.parametrize("update_block,xcd_order", list(itertools.product([64, 128, 256], [0, 1])))
-def test_every_launch_shape_gives_the_same_run(update_block, xcd_order):
-    say("launch shape", update_block, xcd_order)
+@pytest.mark.parametrize("update_block,xcd_order,lanes", list(itertools.product([64, 128, 256], [0, 1], [0, 2])))
+def test_every_launch_shape_gives_the_same_run(update_block, xcd_order, lanes):
+    # lanes = 2: the two-lane forces kernels for every replica, as replicas
+    # created with FCC_FLAG_TWO_LANES would run alone
+    say("launch shape", update_block, xcd_order, lanes)
     reps = [fast_replica(seed=s, nx=nx, ny=ny) for s, (nx, ny) in enumerate([(6, 6), (20, 14), (8, 30)])]
-    seq = [F.run(alone(rp, "fp32"), 80) for rp in reps]
+    seq = [F.run(alone(rp, "fp32", two_lanes=lanes == 2), 80) for rp in reps]
     ens = F.Ensemble(reps, precision="fp32")
-    ens.launch = {"update_block": update_block, "xcd_order": xcd_order, "chunk": 0}
-    assert ens.launch == {"update_block": update_block, "xcd_order": xcd_order, "chunk": 0}
+    launch = {"update_block": update_block, "xcd_order": xcd_order, "chunk": 0, "lanes": lanes}
+    ens.launch = launch
+    assert ens.launch == launch
     ens.run(80)
     for a, b in zip(seq, ens.members):
         assert_same(a, b)
+    # a member handle continues alone with the ensemble's lane choice
+    F.step(ens.members[1], 5)
+    F.step(seq[1], 5)
+    assert_same(seq[1], ens.members[1])
 
 
 def test_autotune_leaves_the_state_unchanged():
+    # replicas created with two lanes per body: the lane choice is not part of
+    # the search, and the tuned run continues the untuned one bitwise
     say("autotune")
     reps = [fast_replica(seed=s, nx=nx, ny=ny) for s, (nx, ny) in enumerate([(8, 8), (30, 30), (16, 40)])]
-    seq = [F.run(alone(rp, "fp32"), 50) for rp in reps]
-    ens = F.Ensemble(reps, precision="fp32")
+    seq = [F.run(alone(rp, "fp32", two_lanes=True), 50) for rp in reps]
+    ens = F.Ensemble(reps, precision="fp32", two_lanes=True)
     ens.run(20)
     launch, ms = ens.autotune(10)
     say("  best", launch, f"{ms:.4f} ms/step")
     assert launch["update_block"] in (64, 128, 256) and launch["xcd_order"] in (0, 1)
+    assert launch["lanes"] == 0
     assert ms > 0
     ens.run(30)
+    for a, b in zip(seq, ens.members):
+        assert_same(a, b)
+
+
+def test_autotuned_lane_choice_matches_replicas_alone():
+    # the autotuner also tries two lanes per body for the small replicas; the
+    # replicas then equal replicas run alone with the chosen lane count
+    say("autotune with the lane choice")
+    reps = [fast_replica(seed=s, nx=nx, ny=ny) for s, (nx, ny) in enumerate([(8, 8), (30, 30), (16, 40)])]
+    ens = F.Ensemble(reps, precision="fp32")
+    launch, ms = ens.autotune(10)
+    say("  best", launch, f"{ms:.4f} ms/step")
+    assert launch["lanes"] in (0, 2)
+    seq = [F.run(alone(rp, "fp32", two_lanes=launch["lanes"] == 2), 60) for rp in reps]
+    ens.run(60)
     for a, b in zip(seq, ens.members):
         assert_same(a, b)
 
