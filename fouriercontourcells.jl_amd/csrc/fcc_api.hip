@@ -20,6 +20,7 @@
 
 #include "fcc_forces_mfma.hpp"
 #include "fcc_forces_mfma4.hpp"
+#include "fcc_forces_wide.hpp"
 #include "fcc_observe.hpp"
 #include "../../include/fcc_bench.h"
 
@@ -625,10 +626,12 @@ struct Sim final : fcc_sim {
         return FCC_OK;
     }
 
-    // The fast kernel needs K in {5, 7}, at most two coupling classes and
-    // neighbour lists shorter than 2^NPLANE (its per-node hit counters).
+    // The fast kernel needs K in {5, 7}, at most 64 nodes, at most two coupling
+    // classes and neighbour lists shorter than 2^NPLANE (its per-node hit
+    // counters).
     bool fast_ok() const {
-        return (K == 5 || K == 7) && p.ncls <= 2 && !has_linrep && (!has_contact || maxcount < (1 << NPLANE));
+        return (K == 5 || K == 7) && nth <= FUSED_MAX_THETA && p.ncls <= 2 && !has_linrep &&
+               (!has_contact || maxcount < (1 << NPLANE));
     }
 
     // Forces of one step: the forces kernel, then the bond kernel when bond
@@ -683,6 +686,21 @@ struct Sim final : fcc_sim {
                 if (p.ncls <= 1) { FCC_FAST(7, 1) } else { FCC_FAST(7, 2) }
             }
 #undef FCC_FAST
+            HIPCHK(hipGetLastError());
+            return FCC_OK;
+        }
+        if (nth > FUSED_MAX_THETA) {
+            const size_t sh = sizeof(T) * (size_t)nth;  // contact density of the body's nodes
+#define FCC_WIDE(KMV)                                                                                    \
+    if (nth <= WIDE_WG64_THETA)                                                                          \
+        FCC_LAUNCH((k_forces_wide<T, KMV, 64>), (unsigned)N, 64, sh, stream, eb, ee, p, d, use_status, dbg); \
+    else FCC_LAUNCH((k_forces_wide<T, KMV, 256>), (unsigned)N, 256, sh, stream, eb, ee, p, d, use_status, dbg);
+            switch (KM) {
+            case 5: FCC_WIDE(5) break;
+            case 7: FCC_WIDE(7) break;
+            default: FCC_WIDE(15)
+            }
+#undef FCC_WIDE
             HIPCHK(hipGetLastError());
             return FCC_OK;
         }

@@ -11,6 +11,9 @@
 namespace fcc {
 
 constexpr int MAXB = 4;  // native body terms per simulation
+// Nodes per contour of the one-lane-per-body forces kernels (64-bit node
+// masks, per-lane LDS rows); larger contours take k_forces_wide.
+constexpr int FUSED_MAX_THETA = 64;
 
 // Device run state (src/utils.jl:50-78): `rebuild` is STATUS_REBUILD, set by
 // the fused skin check; `update` is STATUS_UPDATE, the copy of the freeze

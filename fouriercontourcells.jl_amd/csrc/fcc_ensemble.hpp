@@ -76,7 +76,7 @@ struct Ensemble final : fcc_ensemble {
     bool uploaded = false;
     fcc_launch lc{256, 1, 0, 0};
     fcc_launch tiles_lc{0, 0, 0, 0};
-    int var = -1, tiles_var = -1, ncls = 1;  // forces kernel: 2 MFMA, 1 fast, 0 generic
+    int var = -1, tiles_var = -1, ncls = 1;  // forces kernel: 3 wide, 2 MFMA, 1 fast, 0 generic
     bool bonds = false;                      // a replica has bond terms on bonds: bond kernel after forces
     hipGraphExec_t gexec = nullptr, gexec_n = nullptr;  // one step; GRAPH_STEPS steps
     bool graph_ok = false;
@@ -163,6 +163,9 @@ struct Ensemble final : fcc_ensemble {
         return FCC_OK;
     }
 
+    // The MFMA and fast kernels count hits per node in NPLANE bit planes.
+    bool fast_var() const { return var == 1 || var == 2; }
+
     int launch_forces_kernel(hipEvent_t eb, hipEvent_t ee) {
         const Sim<T> &a = *m[0];
         const int dbg = a.debug ? 1 : 0;
@@ -203,6 +206,21 @@ struct Ensemble final : fcc_ensemble {
                 if (ncls <= 1) { FCC_FE(7, 1) } else { FCC_FE(7, 2) }
             }
 #undef FCC_FE
+            HIPCHK(hipGetLastError());
+            return FCC_OK;
+        }
+        if (var == 3) {
+            const size_t sh = sizeof(T) * (size_t)a.nth;
+#define FCC_WE(KMV)                                                                                           \
+    if (a.nth <= WIDE_WG64_THETA)                                                                             \
+        FCC_LAUNCH((k_forces_wide_ens<T, KMV, 64>), gf, 64, sh, stream, eb, ee, dP, dD, tf, 2, dbg);            \
+    else FCC_LAUNCH((k_forces_wide_ens<T, KMV, 256>), gf, 256, sh, stream, eb, ee, dP, dD, tf, 2, dbg);
+            switch (a.KM) {
+            case 5: FCC_WE(5) break;
+            case 7: FCC_WE(7) break;
+            default: FCC_WE(15)
+            }
+#undef FCC_WE
             HIPCHK(hipGetLastError());
             return FCC_OK;
         }
@@ -254,7 +272,7 @@ struct Ensemble final : fcc_ensemble {
             nc = std::max(nc, s->p.ncls);
             bd = bd || s->has_bonds;
         }
-        const int v = fast ? ((sizeof(T) == 4 && m[0]->mfma_T) ? 2 : 1) : 0;
+        const int v = fast ? ((sizeof(T) == 4 && m[0]->mfma_T) ? 2 : 1) : (m[0]->nth > FUSED_MAX_THETA ? 3 : 0);
         if (v != var || nc != ncls || bd != bonds) graph_ok = false;
         var = v;
         ncls = nc;
@@ -280,11 +298,11 @@ struct Ensemble final : fcc_ensemble {
             for (size_t r = 0; r < R; r++) sizes[r] = m[r]->N;
             // Bodies per forces workgroup: 16 for the four-lane kernels (replicas
             // below FOUR_LANES_BELOW bodies at nθ = 40, the kernel each would get
-            // alone: MFMA or FP32 / FP64), 32 for the two-lane kernels, 64
-            // otherwise.
+            // alone: MFMA or FP32 / FP64), 32 for the two-lane kernels, one for
+            // contours of more than 64 nodes, 64 otherwise.
             const bool fast40 = (var == 1 && m[0]->nth == 40) || var == 2;
             for (size_t r = 0; r < R; r++) (fast40 && sizes[r] < m[r]->four_below ? small : large)[r] = sizes[r];
-            const int tile = var == 2 || fast40 ? 32 : 64;
+            const int tile = var == 3 ? 1 : (var == 2 || fast40 ? 32 : 64);
             std::vector<Tile> a = tile_table(small, 16, lc.xcd_order != 0);
             const size_t n4 = a.size();
             const std::vector<Tile> a2 = tile_table(large, tile, lc.xcd_order != 0);
@@ -481,10 +499,10 @@ struct Ensemble final : fcc_ensemble {
                     // no step since the last build: either that build kept the
                     // replica frozen (it did not fit) or its release is not in
                     // this snapshot yet
-                    if (s[r].overflow > sim.d.cap || (var != 0 && s[r].overflow >= NLIM)) special.push_back(r);
+                    if (s[r].overflow > sim.d.cap || (fast_var() && s[r].overflow >= NLIM)) special.push_back(r);
                     continue;
                 }
-                if ((rc = sim.rebuild_async(side, var != 0 ? NLIM : INT32_MAX))) return rc;
+                if ((rc = sim.rebuild_async(side, fast_var() ? NLIM : INT32_MAX))) return rc;
                 issued_builds[r]++;
                 trip[r] = clock[r];
                 interval[r] = (interval[r] + (double)since[r]) / 2;

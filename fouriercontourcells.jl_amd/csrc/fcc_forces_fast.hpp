@@ -145,7 +145,7 @@ template <typename T, int NTH_, int K, int NCLS, typename PP, typename DP>
 __device__ __forceinline__ void k_forces_fast_impl(const PP &p, const DP &d, int use_status,
                                                     int debug, int64_t blk) {
     constexpr int NC = 2 * K + 1;
-    constexpr int NTHM = NTH_ ? NTH_ : FCC_MAX_THETA;
+    constexpr int NTHM = NTH_ ? NTH_ : FUSED_MAX_THETA;
     __shared__ T sR[NTHM][64];
     __shared__ T scs[2][NTHM];
     const int nth = NTH_ ? NTH_ : p.nth;

@@ -42,7 +42,7 @@ __device__ __forceinline__ void synth(const FCC_CONST T *S, int nth, int k, cons
 template <typename T, int NTH_, int KM, typename PP, typename DP>
 __device__ __forceinline__ void k_forces_impl(const PP &p, const DP &d, int use_status,
                                                int debug, int64_t blk) {
-    constexpr int NTHM = NTH_ ? NTH_ : FCC_MAX_THETA;
+    constexpr int NTHM = NTH_ ? NTH_ : FUSED_MAX_THETA;
     constexpr int NCM = 2 * KM + 1;
     __shared__ T sR[NTHM][64];
     __shared__ T sP[NTHM][64];

@@ -30,7 +30,7 @@ extern "C" {
 #define FCC_MAX_TYPES 4        /* body types per simulation */
 #define FCC_MAX_TERMS 8        /* energy terms per simulation */
 #define FCC_MAX_MODES 15       /* K = maxmode - 1 */
-#define FCC_MAX_THETA 64       /* contour nodes handled on the device */
+#define FCC_MAX_THETA 8000     /* contour nodes handled on the device (above 64: one workgroup per body) */
 
 /* Error codes (the reference throws ArgumentError / DimensionMismatch on the
  * host, src/simulation.jl:71-118, src/basis.jl:36-38, src/bodies.jl:76-77). */

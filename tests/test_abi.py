@@ -25,6 +25,16 @@ def test_header_symbols_match_bindings():
     assert header_symbols() == _abi.HEADER_SYMBOLS
 
 
+def test_header_constants_match_bindings():
+    text = "".join(open(os.path.join(ROOT, "include", h)).read()
+                   for h in sorted(os.listdir(os.path.join(ROOT, "include"))) if h.endswith(".h"))
+    defs = dict(re.findall(r"#define (FCC_[A-Z0-9_]+) (\d+)\b", text))
+    shared = [k for k in defs if hasattr(_abi, k)]
+    assert "FCC_MAX_THETA" in shared and "FCC_MAX_MODES" in shared
+    for k in shared:
+        assert getattr(_abi, k) == int(defs[k]), k
+
+
 def test_library_exports_header_symbols():
     path = _abi.library_path()
     assert os.path.exists(path), "libfcc_hip.so not built"
@@ -67,10 +77,11 @@ def test_library_validates_before_touching_the_device():
     assert eng.ensemble_create(0, None, None, ctypes.byref(h)) == _abi.FCC_ERR_ARGUMENT
     assert "replica" in eng.last_error()
     b, box = F.hexagonal_monolayer(4, 4)
-    bad = F.api._config(b, box, (F.BodyType(),), (F.Tension(0.01),), 0.005, 80, "fp32", 1, 0.3, 0)
+    bad = F.api._config(b, box, (F.BodyType(),), (F.Tension(0.01),), 0.005, _abi.FCC_MAX_THETA + 1, "fp32", 1,
+                        0.3, 0)
     st = b.struct()
     assert eng.create(ctypes.byref(bad), ctypes.byref(st), ctypes.byref(h)) == _abi.FCC_ERR_UNSUPPORTED
-    assert "64" in eng.last_error()
+    assert str(_abi.FCC_MAX_THETA) in eng.last_error()
     bad.n_theta = 10  # must exceed 2·maxmode (src/basis.jl:38)
     assert eng.create(ctypes.byref(bad), ctypes.byref(st), ctypes.byref(h)) == _abi.FCC_ERR_ARGUMENT
     cfgs = (_abi.FccConfig * 2)()

@@ -159,11 +159,11 @@ def test_regression_fixture_on_device_fp64(hip):
         np.testing.assert_allclose(bod.coef[1:], ref[5:15], atol=1e-10, rtol=0)
 
 
-def soft_fluid(precision="fp64", v0=2.0, skin=0.05, seed=1, n=8):
+def soft_fluid(precision="fp64", v0=2.0, skin=0.05, seed=1, n=8, ntheta=40):
     b, box = F.hexagonal_monolayer(n, n, packing=0.95)
     return F.Simulation(b, box, types=(F.BodyType(v0=v0, Dr=0.5, Dt=0.01),),
                         energies=(F.Tension(0.01), F.Overlap(1.0)), dt=0.005, skin=skin, seed=seed,
-                        precision=precision)
+                        precision=precision, ntheta=ntheta)
 
 
 def state(s):
