@@ -299,9 +299,9 @@ struct Sim final : fcc_sim {
         N = c->n_bodies;
         K = c->n_modes;
         nc = 2 * K + 1;
-        ldc = nc <= 16 ? 16 : 32;
+        ldc = nc <= 16 ? 16 : (nc <= 32 ? 32 : 64);  // row stride ≥ nc (2·31 + 1 at most)
         nth = c->n_theta;
-        KM = K <= 5 ? 5 : (K <= 7 ? 7 : 15);
+        KM = K <= 5 ? 5 : (K <= 7 ? 7 : (K <= 15 ? 15 : 31));
         mixed = c->precision == FCC_MIXED_FP16 || c->precision == FCC_MIXED_BF16;
         debug = (c->flags & FCC_FLAG_DEBUG_BUFFERS) != 0;
         four_below = (c->flags & FCC_FLAG_TWO_LANES) ? 0 : FOUR_LANES_BELOW;
@@ -455,11 +455,11 @@ struct Sim final : fcc_sim {
                 const int n = m + 1;
                 const double sn = std::sin((double)n * phi), cn = std::cos((double)n * phi);
                 S[(size_t)k * SB + m] = (T)cn;
-                S[(size_t)k * SB + 16 + m] = (T)sn;
+                S[(size_t)k * SB + SH + m] = (T)sn;
                 S[(size_t)(nth + k) * SB + m] = (T)((double)(-n) * sn);
-                S[(size_t)(nth + k) * SB + 16 + m] = (T)((double)n * cn);
+                S[(size_t)(nth + k) * SB + SH + m] = (T)((double)n * cn);
                 S[(size_t)(2 * nth + k) * SB + m] = (T)((double)(-n * n) * cn);
-                S[(size_t)(2 * nth + k) * SB + 16 + m] = (T)((double)(-n * n) * sn);
+                S[(size_t)(2 * nth + k) * SB + SH + m] = (T)((double)(-n * n) * sn);
             }
         }
         std::vector<T> Sc((size_t)3 * nth * ldc, T(0));
@@ -470,7 +470,7 @@ struct Sim final : fcc_sim {
                 out[0] = row[0];
                 for (int m = 1; m <= K; m++) {
                     out[m] = row[m];
-                    out[K + m] = row[16 + m];
+                    out[K + m] = row[SH + m];
                 }
             }
         T *dS, *dcs, *dSc;
@@ -618,9 +618,13 @@ struct Sim final : fcc_sim {
             if (mixed) hipLaunchKernelGGL((k_init<T, 7, true>), g, 256, 0, stream, p, d);
             else hipLaunchKernelGGL((k_init<T, 7, false>), g, 256, 0, stream, p, d);
             break;
-        default:
+        case 15:
             if (mixed) hipLaunchKernelGGL((k_init<T, 15, true>), g, 256, 0, stream, p, d);
             else hipLaunchKernelGGL((k_init<T, 15, false>), g, 256, 0, stream, p, d);
+            break;
+        default:
+            if (mixed) hipLaunchKernelGGL((k_init<T, 31, true>), g, 256, 0, stream, p, d);
+            else hipLaunchKernelGGL((k_init<T, 31, false>), g, 256, 0, stream, p, d);
         }
         HIPCHK(hipGetLastError());
         return FCC_OK;
@@ -698,7 +702,8 @@ struct Sim final : fcc_sim {
             switch (KM) {
             case 5: FCC_WIDE(5) break;
             case 7: FCC_WIDE(7) break;
-            default: FCC_WIDE(15)
+            case 15: FCC_WIDE(15) break;
+            default: FCC_WIDE(31)
             }
 #undef FCC_WIDE
             HIPCHK(hipGetLastError());
@@ -710,7 +715,8 @@ struct Sim final : fcc_sim {
         switch (KM) {
         case 5: FCC_FORCES(5) break;
         case 7: FCC_FORCES(7) break;
-        default: FCC_FORCES(15)
+        case 15: FCC_FORCES(15) break;
+        default: FCC_FORCES(31)
         }
 #undef FCC_FORCES
         HIPCHK(hipGetLastError());
@@ -733,8 +739,10 @@ struct Sim final : fcc_sim {
             FCC_UPD(5, false)
         } else if (KM == 7) {
             FCC_UPD(7, false)
-        } else {
+        } else if (KM == 15) {
             FCC_UPD(15, false)
+        } else {
+            FCC_UPD(31, false)
         }
 #undef FCC_UPD
         HIPCHK(hipGetLastError());
@@ -993,7 +1001,8 @@ struct Sim final : fcc_sim {
             switch (KM) {
             case 5: hipLaunchKernelGGL((k_body_frame<T, 5>), g, 256, 0, stream, p, d); break;
             case 7: hipLaunchKernelGGL((k_body_frame<T, 7>), g, 256, 0, stream, p, d); break;
-            default: hipLaunchKernelGGL((k_body_frame<T, 15>), g, 256, 0, stream, p, d);
+            case 15: hipLaunchKernelGGL((k_body_frame<T, 15>), g, 256, 0, stream, p, d); break;
+            default: hipLaunchKernelGGL((k_body_frame<T, 31>), g, 256, 0, stream, p, d);
             }
             HIPCHK(hipGetLastError());
         }
@@ -1047,7 +1056,8 @@ struct Sim final : fcc_sim {
         switch (KM) {
         case 5: hipLaunchKernelGGL((k_coef_from_lab<T, 5>), g, 256, 0, stream, p, d); break;
         case 7: hipLaunchKernelGGL((k_coef_from_lab<T, 7>), g, 256, 0, stream, p, d); break;
-        default: hipLaunchKernelGGL((k_coef_from_lab<T, 15>), g, 256, 0, stream, p, d);
+        case 15: hipLaunchKernelGGL((k_coef_from_lab<T, 15>), g, 256, 0, stream, p, d); break;
+        default: hipLaunchKernelGGL((k_coef_from_lab<T, 31>), g, 256, 0, stream, p, d);
         }
         HIPCHK(hipGetLastError());
         return FCC_OK;
@@ -1126,7 +1136,8 @@ struct Sim final : fcc_sim {
         switch (KM) {
         case 5: hipLaunchKernelGGL((k_geometry<T, 5>), g, 256, 0, stream, p, d, R, dR, ddR, per, area); break;
         case 7: hipLaunchKernelGGL((k_geometry<T, 7>), g, 256, 0, stream, p, d, R, dR, ddR, per, area); break;
-        default: hipLaunchKernelGGL((k_geometry<T, 15>), g, 256, 0, stream, p, d, R, dR, ddR, per, area);
+        case 15: hipLaunchKernelGGL((k_geometry<T, 15>), g, 256, 0, stream, p, d, R, dR, ddR, per, area); break;
+        default: hipLaunchKernelGGL((k_geometry<T, 31>), g, 256, 0, stream, p, d, R, dR, ddR, per, area);
         }
         HIPCHK(hipGetLastError());
         return FCC_OK;
@@ -1399,7 +1410,8 @@ struct Sim final : fcc_sim {
         switch (KM) {
         case 5: hipLaunchKernelGGL((k_contours<T, 5>), g, 256, 0, stream, p, d, X, Y); break;
         case 7: hipLaunchKernelGGL((k_contours<T, 7>), g, 256, 0, stream, p, d, X, Y); break;
-        default: hipLaunchKernelGGL((k_contours<T, 15>), g, 256, 0, stream, p, d, X, Y);
+        case 15: hipLaunchKernelGGL((k_contours<T, 15>), g, 256, 0, stream, p, d, X, Y); break;
+        default: hipLaunchKernelGGL((k_contours<T, 31>), g, 256, 0, stream, p, d, X, Y);
         }
         HIPCHK(hipGetLastError());
         return FCC_OK;
@@ -1657,7 +1669,8 @@ struct Sim final : fcc_sim {
         switch (KM) {
         case 5: hipLaunchKernelGGL((k_observe<T, 5>), g, OBS_BLOCK, sh, stream, p, d, o); break;
         case 7: hipLaunchKernelGGL((k_observe<T, 7>), g, OBS_BLOCK, sh, stream, p, d, o); break;
-        default: hipLaunchKernelGGL((k_observe<T, 15>), g, OBS_BLOCK, sh, stream, p, d, o);
+        case 15: hipLaunchKernelGGL((k_observe<T, 15>), g, OBS_BLOCK, sh, stream, p, d, o); break;
+        default: hipLaunchKernelGGL((k_observe<T, 31>), g, OBS_BLOCK, sh, stream, p, d, o);
         }
         hipLaunchKernelGGL(k_observe_finish, 1, OBS_BLOCK, 0, stream, obs.part, obs.nblk, nt, obs.n, d.st, obs.msd,
                            obs.pm, obs.p2, obs.clk);

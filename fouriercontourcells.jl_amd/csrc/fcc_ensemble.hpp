@@ -218,7 +218,8 @@ struct Ensemble final : fcc_ensemble {
             switch (a.KM) {
             case 5: FCC_WE(5) break;
             case 7: FCC_WE(7) break;
-            default: FCC_WE(15)
+            case 15: FCC_WE(15) break;
+            default: FCC_WE(31)
             }
 #undef FCC_WE
             HIPCHK(hipGetLastError());
@@ -230,7 +231,8 @@ struct Ensemble final : fcc_ensemble {
         switch (a.KM) {
         case 5: FCC_GE(5) break;
         case 7: FCC_GE(7) break;
-        default: FCC_GE(15)
+        case 15: FCC_GE(15) break;
+        default: FCC_GE(31)
         }
 #undef FCC_GE
         HIPCHK(hipGetLastError());
@@ -254,8 +256,10 @@ struct Ensemble final : fcc_ensemble {
             FCC_UE(5, false)
         } else if (a.KM == 7) {
             FCC_UE(7, false)
-        } else {
+        } else if (a.KM == 15) {
             FCC_UE(15, false)
+        } else {
+            FCC_UE(31, false)
         }
 #undef FCC_UE
         HIPCHK(hipGetLastError());

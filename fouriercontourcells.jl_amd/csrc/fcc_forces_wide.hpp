@@ -32,9 +32,9 @@ __device__ __forceinline__ void synth_lane(const T *S, int nth, int k, const T (
 #pragma unroll
     for (int m = 1; m <= KM; m++) {
         if (m <= K) {
-            r += C[m] * l[m] + C[16 + m] * l[KM + m];
-            dr += D[m] * l[m] + D[16 + m] * l[KM + m];
-            if (need_dd) ddr += DD[m] * l[m] + DD[16 + m] * l[KM + m];
+            r += C[m] * l[m] + C[SH + m] * l[KM + m];
+            dr += D[m] * l[m] + D[SH + m] * l[KM + m];
+            if (need_dd) ddr += DD[m] * l[m] + DD[SH + m] * l[KM + m];
         }
     }
     R = r0 * r;
@@ -218,10 +218,10 @@ __device__ __forceinline__ void k_forces_wide_impl(const PP &p, const DP &d, int
         for (int m = 1; m <= KM; m++) {
             if (m <= K) {
                 T a = C[m] * GR + D[m] * GD;
-                T b = C[16 + m] * GR + D[16 + m] * GD;
+                T b = C[SH + m] * GR + D[SH + m] * GD;
                 if (need_dd) {
                     a += DD[m] * GDD;
-                    b += DD[16 + m] * GDD;
+                    b += DD[SH + m] * GDD;
                 }
                 acc[m] += a;
                 acc[KM + m] += b;

@@ -11,8 +11,9 @@
 
 namespace fcc {
 
-// Basis row stride on the device: [a0 | cos modes m = 1..15 | sin modes at 16+m].
-constexpr int SB = 32;
+// Basis row stride on the device: [a0 | cos modes m = 1..31 | sin modes at SH+m].
+constexpr int SB = 64;
+constexpr int SH = SB / 2;
 
 template <typename T, int KM>
 __device__ __forceinline__ void synth(const FCC_CONST T *S, int nth, int k, const T (&l)[2 * KM + 1],
@@ -24,9 +25,9 @@ __device__ __forceinline__ void synth(const FCC_CONST T *S, int nth, int k, cons
 #pragma unroll
     for (int m = 1; m <= KM; m++) {
         if (m <= K) {
-            r += C[m] * l[m] + C[16 + m] * l[KM + m];
-            dr += D[m] * l[m] + D[16 + m] * l[KM + m];
-            if (need_dd) ddr += DD[m] * l[m] + DD[16 + m] * l[KM + m];
+            r += C[m] * l[m] + C[SH + m] * l[KM + m];
+            dr += D[m] * l[m] + D[SH + m] * l[KM + m];
+            if (need_dd) ddr += DD[m] * l[m] + DD[SH + m] * l[KM + m];
         }
     }
     R = r0 * r;
@@ -198,10 +199,10 @@ __device__ __forceinline__ void k_forces_impl(const PP &p, const DP &d, int use_
         for (int m = 1; m <= KM; m++) {
             if (m <= K) {
                 T a = C[m] * GR + D[m] * GD;
-                T b = C[16 + m] * GR + D[16 + m] * GD;
+                T b = C[SH + m] * GR + D[SH + m] * GD;
                 if (need_dd) {
                     a += DD[m] * GDD;
-                    b += DD[16 + m] * GDD;
+                    b += DD[SH + m] * GDD;
                 }
                 gl[m] += a;
                 gl[KM + m] += b;

@@ -9,7 +9,7 @@ import os
 
 FCC_MAX_TYPES = 4
 FCC_MAX_TERMS = 8
-FCC_MAX_MODES = 15
+FCC_MAX_MODES = 31
 FCC_MAX_THETA = 8000
 
 FCC_OK = 0

@@ -17,7 +17,7 @@
 # between the kernels of every step, src/simulation.jl:181-191), an explicit
 # `layout` (a choice among the reference's kernels), and what the library
 # returns FCC_ERR_UNSUPPORTED for (more than 4 body types or 4 body terms,
-# nθ > 8000, more than 15 shape modes). `T = :mixed` / `:mixed_bf16` have no
+# nθ > 8000, more than 31 shape modes). `T = :mixed` / `:mixed_bf16` have no
 # reference counterpart; the reference path then runs in Float32.
 # `HIPEnsemble` runs independent replicas by one launch per kernel and step;
 # `observer_start!` / `run_observed!` / `observations` accumulate long-run

@@ -29,7 +29,7 @@ extern "C" {
 
 #define FCC_MAX_TYPES 4        /* body types per simulation */
 #define FCC_MAX_TERMS 8        /* energy terms per simulation */
-#define FCC_MAX_MODES 15       /* K = maxmode - 1 */
+#define FCC_MAX_MODES 31       /* K = maxmode - 1 */
 #define FCC_MAX_THETA 8000     /* contour nodes handled on the device (above 64: one workgroup per body) */
 
 /* Error codes (the reference throws ArgumentError / DimensionMismatch on the
