@@ -254,10 +254,19 @@ struct Sim final : fcc_sim {
     double t = 0.0;
     std::vector<void *> allocs;
     // graph mode (ext/FourierContourCellsCUDAExt.jl:52-71)
-    hipGraphExec_t gexec = nullptr, gexec_n = nullptr;  // one step; GRAPH_STEPS steps
+    // captured steps: gx[b] holds 2^b steps (b < NGRAPH, 2^(NGRAPH-1) = GRAPH_STEPS)
+    static constexpr int NGRAPH = 4;
+    hipGraphExec_t gx[NGRAPH] = {};
     int gen = 0, gexec_gen = -1;
     bool want_timed = false;
-    hipEvent_t tev[4] = {nullptr, nullptr, nullptr, nullptr};
+    // Pipelined chunks of fcc_run: status snapshots in pinned memory, one
+    // slot per queued chunk (and slot NQ for the status at the start of a
+    // run), each with its completion event, and the timing events of each
+    // chunk's eagerly launched last step (fcc_run_timed).
+    static constexpr int NQ = 2;
+    DevStatus *snap = nullptr;
+    hipEvent_t sev[NQ + 1] = {};
+    hipEvent_t tev[NQ][4] = {};
     double interval = 16.0;
     int64_t since_rebuild = 0, gbuilds = 0;
 
@@ -268,10 +277,14 @@ struct Sim final : fcc_sim {
         if (fr.done) (void)hipEventDestroy(fr.done);
         if (fr.hX) (void)hipHostFree(fr.hX);
         if (fr.copy) (void)hipStreamDestroy(fr.copy);
-        if (gexec) (void)hipGraphExecDestroy(gexec);
-        if (gexec_n) (void)hipGraphExecDestroy(gexec_n);
-        for (hipEvent_t e : tev)
+        for (hipGraphExec_t g : gx)
+            if (g) (void)hipGraphExecDestroy(g);
+        for (hipEvent_t e : sev)
             if (e) (void)hipEventDestroy(e);
+        for (int q = 0; q < NQ; q++)
+            for (hipEvent_t e : tev[q])
+                if (e) (void)hipEventDestroy(e);
+        if (snap) (void)hipHostFree(snap);
         for (void *q : allocs) (void)hipFree(q);
         if (stream && own_stream) (void)hipStreamDestroy(stream);
     }
@@ -855,9 +868,10 @@ struct Sim final : fcc_sim {
     }
 
     // run_steps! in graph mode (ext/FourierContourCellsCUDAExt.jl:65-71,133-180).
-    // Two graphs: one step, and GRAPH_STEPS steps in a row (a graph launch
+    // Graphs of 1, 2, 4 and GRAPH_STEPS = 8 steps in a row (a graph launch
     // costs several µs between replays; inside a graph the kernels follow
-    // each other without a gap).
+    // each other without a gap), so k steps take k / 8 + popcount(k mod 8)
+    // launches.
     int capture_steps(int steps, hipGraphExec_t *out) {
         hipGraph_t graph;
         HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
@@ -874,23 +888,25 @@ struct Sim final : fcc_sim {
 
     int capture() {
         std::lock_guard<std::mutex> lk(g_capture_lock);
-        for (hipGraphExec_t *g : {&gexec, &gexec_n})
-            if (*g) {
-                HIPCHK(hipGraphExecDestroy(*g));
-                *g = nullptr;
+        for (hipGraphExec_t &g : gx)
+            if (g) {
+                HIPCHK(hipGraphExecDestroy(g));
+                g = nullptr;
             }
         p.skin_check = (has_contact && skin > 0.0) ? 1 : 0;
-        int rc = capture_steps(1, &gexec);
-        if (!rc) rc = capture_steps(GRAPH_STEPS, &gexec_n);
-        if (rc) return rc;
+        for (int b = 0; b < NGRAPH; b++)
+            if (int rc = capture_steps(1 << b, &gx[b])) return rc;
         gexec_gen = gen;
         return FCC_OK;
     }
 
-    // k graph steps: k / GRAPH_STEPS launches of the long graph, the rest one by one.
+    // k graph steps: k / GRAPH_STEPS launches of the longest graph, then one
+    // launch per set bit of the rest.
     int replay(int64_t k) {
-        for (int64_t q = 0; q < k / GRAPH_STEPS; q++) HIPCHK(hipGraphLaunch(gexec_n, stream));
-        for (int64_t q = 0; q < k % GRAPH_STEPS; q++) HIPCHK(hipGraphLaunch(gexec, stream));
+        static_assert((1 << (NGRAPH - 1)) == GRAPH_STEPS, "graph sizes");
+        for (int64_t q = 0; q < k / GRAPH_STEPS; q++) HIPCHK(hipGraphLaunch(gx[NGRAPH - 1], stream));
+        for (int b = NGRAPH - 2; b >= 0; b--)
+            if ((k % GRAPH_STEPS) & (1 << b)) HIPCHK(hipGraphLaunch(gx[b], stream));
         return FCC_OK;
     }
 
@@ -933,54 +949,124 @@ struct Sim final : fcc_sim {
         return FCC_OK;
     }
 
+    int ensure_pipeline() {
+        if (snap) return FCC_OK;
+        HIPCHK(hipHostMalloc((void **)&snap, sizeof(DevStatus) * (NQ + 1), hipHostMallocDefault));
+        for (auto &e : sev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        for (int q = 0; q < NQ; q++)
+            for (auto &e : tev[q]) HIPCHK(hipEventCreate(&e));
+        return FCC_OK;
+    }
+
+    // k steps, then a snapshot of the status into pinned slot `slot`.
+    int issue(int64_t k, int slot) {
+        int rc;
+        if (!want_timed) {
+            if ((rc = replay(k))) return rc;
+        } else {
+            // the chunk's last step is launched eagerly between events:
+            // the same kernels and arguments as the graph
+            if ((rc = replay(k - 1))) return rc;
+            p.skin_check = (has_contact && skin > 0.0) ? 1 : 0;
+            if ((rc = launch_forces(1, tev[slot][0], tev[slot][1]))) return rc;
+            if ((rc = launch_update(tev[slot][2], tev[slot][3]))) return rc;
+        }
+        HIPCHK(hipMemcpyAsync(snap + slot, d.st, sizeof(DevStatus), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipEventRecord(sev[slot], stream));
+        return FCC_OK;
+    }
+
+    // Steps per chunk from the rebuild-interval estimate.
+    int64_t chunk_len() const {
+        if (!has_contact) return 256;
+        return std::min<int64_t>(256, std::max<int64_t>(4, (int64_t)std::llround(interval / 4)));
+    }
+
+    // run! in graph mode, pipelined: the host keeps up to NQ chunks queued
+    // and reads each chunk's status snapshot while the next one runs, so the
+    // device does not wait for the host between chunks. A raised rebuild flag
+    // freezes every later step, those of chunks already queued included,
+    // until the host has rebuilt the lists; chunks are queued only for steps
+    // still due if every queued step executes, so at most n steps run. The
+    // chunk length follows the rebuild-interval estimate of
+    // ext/FourierContourCellsCUDAExt.jl:65-71; it does not change results.
     int run(int64_t n) override {
         if (n <= 0) return FCC_OK;
         if (has_contact && skin <= 0.0) return step(n);  // graph_capable (src/simulation.jl:206-207)
         int rc;
-        if (has_contact && (rc = update_neighbors(valid ? 0 : 1))) return rc;
+        if ((rc = ensure_pipeline())) return rc;
+        if (has_contact && !valid && (rc = rebuild())) return rc;
+        // The status at the start of the run is one more snapshot (slot NQ),
+        // read while the first chunks run. A rebuild flag raised by the last
+        // step before the run freezes them, and the lists are rebuilt before
+        // the run goes on (update_neighbors!(sim), src/simulation.jl:132-142).
+        HIPCHK(hipMemcpyAsync(snap + NQ, d.st, sizeof(DevStatus), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipEventRecord(sev[NQ], stream));
+        bool started = false;
         DevStatus st;
-        HIPCHK(hipMemcpyAsync(&st, d.st, sizeof st, hipMemcpyDeviceToHost, stream));
-        HIPCHK(hipStreamSynchronize(stream));
-        int64_t clock = st.clock, done = 0;
-        while (done < n) {
-            if (!gexec || gexec_gen != gen) {
-                if ((rc = capture())) return rc;
-            }
-            if (builds != gbuilds) {
-                gbuilds = builds;
-                since_rebuild = 0;
-            }
-            const int64_t chunk = has_contact ? std::min<int64_t>(256, std::max<int64_t>(4, (int64_t)std::llround(interval / 4))) : 256;
-            const int64_t k = std::min(chunk, n - done);
-            if (!want_timed) {
-                if ((rc = replay(k))) return rc;
-            } else {
-                // the chunk's last step is launched eagerly between events:
-                // the same kernels and arguments as the graph
-                if (!tev[0])
-                    for (auto &ev : tev) HIPCHK(hipEventCreate(&ev));
-                if ((rc = replay(k - 1))) return rc;
-                p.skin_check = (has_contact && skin > 0.0) ? 1 : 0;
-                if ((rc = launch_forces(1, tev[0], tev[1]))) return rc;
-                if ((rc = launch_update(tev[2], tev[3]))) return rc;
-            }
-            HIPCHK(hipMemcpyAsync(&st, d.st, sizeof st, hipMemcpyDeviceToHost, stream));
-            HIPCHK(hipStreamSynchronize(stream));
-            const int64_t executed = st.clock - clock;
-            clock = st.clock;
-            if (want_timed && executed == k) {  // the chunk's last step ran (not frozen)
-                float a = 0.f, b = 0.f;
-                HIPCHK(hipEventElapsedTime(&a, tev[0], tev[1]));
-                HIPCHK(hipEventElapsedTime(&b, tev[2], tev[3]));
-                acc_f += a;
-                acc_u += b;
-                acc_n++;
-            }
+        int64_t clock = 0, done = 0, planned = 0, issued = 0, read = 0;
+        int64_t kq[NQ] = {};
+        auto account = [&](int64_t executed) {
             for (int64_t q = 0; q < executed; q++) t += cfg.dt;
             nstep += executed;
             done += executed;
             since_rebuild += executed;
-            if (st.rebuild != 0 && done < n) {
+        };
+        // every queued step after a raised rebuild flag is frozen: let them
+        // drain and take the clock they leave (no step of theirs ran)
+        auto drain = [&]() -> int {
+            HIPCHK(hipMemcpyAsync(&st, d.st, sizeof st, hipMemcpyDeviceToHost, stream));
+            HIPCHK(hipStreamSynchronize(stream));
+            account(st.clock - clock);
+            clock = st.clock;
+            read = issued;
+            planned = done;
+            return FCC_OK;
+        };
+        while (done < n) {
+            while (issued - read < NQ && planned < n) {
+                if (!gx[0] || gexec_gen != gen) {
+                    if (issued > read) break;  // capture with nothing queued
+                    if ((rc = capture())) return rc;
+                }
+                if (builds != gbuilds) {
+                    gbuilds = builds;
+                    since_rebuild = 0;
+                }
+                const int64_t k = std::min(chunk_len(), n - planned);
+                const int slot = (int)(issued % NQ);
+                if ((rc = issue(k, slot))) return rc;
+                kq[slot] = k;
+                planned += k;
+                issued++;
+            }
+            if (!started) {
+                HIPCHK(hipEventSynchronize(sev[NQ]));
+                clock = snap[NQ].clock;
+                started = true;
+                if (has_contact && snap[NQ].rebuild != 0) {
+                    if ((rc = drain()) || (rc = rebuild())) return rc;
+                    continue;
+                }
+            }
+            if (read == issued) return fail(FCC_ERR_STATE, "run: no chunk in flight with %lld steps due", (long long)(n - done));
+            const int slot = (int)(read % NQ);
+            HIPCHK(hipEventSynchronize(sev[slot]));
+            const DevStatus s = snap[slot];
+            read++;
+            const int64_t executed = s.clock - clock;
+            clock = s.clock;
+            if (want_timed && executed == kq[slot]) {  // the chunk's last step ran (not frozen)
+                float a = 0.f, b = 0.f;
+                HIPCHK(hipEventElapsedTime(&a, tev[slot][0], tev[slot][1]));
+                HIPCHK(hipEventElapsedTime(&b, tev[slot][2], tev[slot][3]));
+                acc_f += a;
+                acc_u += b;
+                acc_n++;
+            }
+            account(executed);
+            if (s.rebuild != 0 && (rc = drain())) return rc;
+            if (s.rebuild != 0 && done < n) {
                 if ((rc = update_neighbors(0))) return rc;
                 interval = (interval + (double)since_rebuild) / 2;
                 since_rebuild = 0;
