@@ -235,6 +235,13 @@ struct Sim final : fcc_sim {
     // nθ = 40 (FOUR_LANES_BELOW; 0 with FCC_FLAG_TWO_LANES).
     int64_t four_below = FOUR_LANES_BELOW;
     bool has_bonds = false;  // bond terms and at least one bond
+    // One-kernel steps (k_step_mfma, k_step_fast2): a step writes positions,
+    // lab rows and radius bounds into the other buffer of each pair, so d.x,
+    // d.y, d.lab, d.rmax, d.rmin name the current state and d.xn, ... the
+    // other buffers; `par` is the parity of the current buffers (flip()).
+    bool member = false;       // replica of an ensemble (the ensemble launches its kernels)
+    bool two_kernels = false;  // FCC_FLAG_TWO_KERNELS
+    int par = 0;
     int64_t nbonds = 0;
     double skin = 0.0, range = 0.0;
     int32_t ncx = 1, ncy = 1;
@@ -254,9 +261,12 @@ struct Sim final : fcc_sim {
     double t = 0.0;
     std::vector<void *> allocs;
     // graph mode (ext/FourierContourCellsCUDAExt.jl:52-71)
-    // captured steps: gx[b] holds 2^b steps (b < NGRAPH, 2^(NGRAPH-1) = GRAPH_STEPS)
+    // captured steps: gx[q][b] holds 2^b steps from the buffers of parity q
+    // (b < NGRAPH, 2^(NGRAPH-1) = GRAPH_STEPS; set 1 only for one-kernel
+    // steps, gone)
     static constexpr int NGRAPH = 4;
-    hipGraphExec_t gx[NGRAPH] = {};
+    hipGraphExec_t gx[2][NGRAPH] = {};
+    bool gone = false;
     int gen = 0, gexec_gen = -1;
     bool want_timed = false;
     // Pipelined chunks of fcc_run: status snapshots in pinned memory, one
@@ -277,8 +287,9 @@ struct Sim final : fcc_sim {
         if (fr.done) (void)hipEventDestroy(fr.done);
         if (fr.hX) (void)hipHostFree(fr.hX);
         if (fr.copy) (void)hipStreamDestroy(fr.copy);
-        for (hipGraphExec_t g : gx)
-            if (g) (void)hipGraphExecDestroy(g);
+        for (auto &set : gx)
+            for (hipGraphExec_t g : set)
+                if (g) (void)hipGraphExecDestroy(g);
         for (hipEvent_t e : sev)
             if (e) (void)hipEventDestroy(e);
         for (int q = 0; q < NQ; q++)
@@ -318,6 +329,8 @@ struct Sim final : fcc_sim {
         mixed = c->precision == FCC_MIXED_FP16 || c->precision == FCC_MIXED_BF16;
         debug = (c->flags & FCC_FLAG_DEBUG_BUFFERS) != 0;
         four_below = (c->flags & FCC_FLAG_TWO_LANES) ? 0 : FOUR_LANES_BELOW;
+        two_kernels = (c->flags & FCC_FLAG_TWO_KERNELS) != 0;
+        member = ext_st != nullptr;
         skin = c->skin;
         if (ext_stream) {
             stream = ext_stream;
@@ -450,6 +463,7 @@ struct Sim final : fcc_sim {
         HIPCHK(hipMemsetAsync(d.gth, 0, sizeof(T) * N, stream));
         HIPCHK(hipMemsetAsync(d.Ebody, 0, sizeof(T) * N, stream));
         HIPCHK(hipMemsetAsync(d.st, 0, sizeof(DevStatus), stream));
+        hipLaunchKernelGGL(k_clear_status, 1, 1, 0, stream, d.st);
         hipLaunchKernelGGL(k_set_target, 1, 1, 0, stream, d.st, INT64_MAX);
         if (debug) {
             if ((rc = alloc(&d.P, N * nth)) || (rc = alloc(&d.GR, N * nth)) || (rc = alloc(&d.GD, N * nth)) ||
@@ -762,12 +776,110 @@ struct Sim final : fcc_sim {
         return FCC_OK;
     }
 
-    // advance! (src/simulation.jl:181-191) as device work on the stream.
+    // Whether a step runs as one kernel: single simulations on the two-lane
+    // FP32 / mixed kernels (the mixed one without Bending) with contact and a
+    // Verlet skin, without parent drag (a child reads its parent's θ before
+    // the update), bonds (their kernel adds to the centre forces between the
+    // forces and the update) or debug buffers.
+    bool one_kernel() const {
+        if constexpr (sizeof(T) != 4) return false;
+        return !member && !two_kernels && fast_ok() && nth == 40 && N >= four_below && has_contact &&
+               skin > 0.0 && !has_pdrag && !has_bonds && !debug && (!mfma_T || mfma_T == 3);
+    }
+
+    // The other buffer of each state pair, allocated at the first one-kernel step.
+    int ensure_next() {
+        if (d.xn) return FCC_OK;
+        int rc;
+        if ((rc = alloc(&d.xn, N)) || (rc = alloc(&d.yn, N)) || (rc = alloc(&d.labn, N * ldc)) ||
+            (rc = alloc(&d.rmaxn, N)) || (rc = alloc(&d.rminn, N)))
+            return rc;
+        HIPCHK(hipMemsetAsync(d.labn, 0, sizeof(T) * N * ldc, stream));
+        return FCC_OK;
+    }
+
+    // Before one-kernel steps from the current buffers (and after eager ones):
+    // a skin trip of the last one-kernel step becomes the rebuild flag, and
+    // the step clock slot of parity `par` takes the clock (k_step_slots).
+    int sync_slots() {
+        hipLaunchKernelGGL(k_step_slots, 1, 1, 0, stream, d.st, par);
+        HIPCHK(hipGetLastError());
+        return FCC_OK;
+    }
+
+    // The state of an executed one-kernel step is in the other buffers.
+    void flip() {
+        std::swap(d.x, d.xn);
+        std::swap(d.y, d.yn);
+        std::swap(d.lab, d.labn);
+        std::swap(d.rmax, d.rmaxn);
+        std::swap(d.rmin, d.rminn);
+        par ^= 1;
+    }
+    void set_parity(int q) {
+        if (par != q) flip();
+    }
+
+    // use_status 3: the step's status protocol (step_open, reading the
+    // buffers of parity `par`); 4: the same without the freeze check (timing
+    // series, whose state is restored afterwards).
+    int launch_one_kernel(int use_status, hipEvent_t eb, hipEvent_t ee) {
+        const unsigned g = blocks(N, 64);
+        if constexpr (sizeof(T) == 4) {
+            if (mfma_T) {
+                const bool bf = cfg.precision == FCC_MIXED_BF16;
+#define FCC_STEP_MF(KV, NCV)                                                                              \
+    if (bf) FCC_LAUNCH((k_step_mfma<KV, 3, NCV, true>), g, 128, 0, stream, eb, ee, p, d, asyn, bpull, use_status, par); \
+    else FCC_LAUNCH((k_step_mfma<KV, 3, NCV, false>), g, 128, 0, stream, eb, ee, p, d, asyn, bpull, use_status, par);
+                if (K == 5) {
+                    if (p.ncls <= 1) { FCC_STEP_MF(5, 1) } else { FCC_STEP_MF(5, 2) }
+                } else {
+                    if (p.ncls <= 1) { FCC_STEP_MF(7, 1) } else { FCC_STEP_MF(7, 2) }
+                }
+#undef FCC_STEP_MF
+            } else {
+#define FCC_STEP_F(KV, NCV) FCC_LAUNCH((k_step_fast2<T, KV, NCV>), g, 128, 0, stream, eb, ee, p, d, use_status, par);
+                if (K == 5) {
+                    if (p.ncls <= 1) { FCC_STEP_F(5, 1) } else { FCC_STEP_F(5, 2) }
+                } else {
+                    if (p.ncls <= 1) { FCC_STEP_F(7, 1) } else { FCC_STEP_F(7, 2) }
+                }
+#undef FCC_STEP_F
+            }
+            HIPCHK(hipGetLastError());
+            return FCC_OK;
+        }
+        return fail(FCC_ERR_STATE, "one-kernel step in double precision");
+    }
+
+    // One step on the stream: one kernel where one_kernel() holds (the host
+    // then flips to the buffers it writes), else the forces and the update
+    // kernels. eb/ee time the forces kernel (the step's kernel in a
+    // one-kernel step), ub/ue the update kernel (recorded back to back in a
+    // one-kernel step). The other buffers must exist (ensure_next) before a
+    // one-kernel step is captured.
+    int launch_step(hipEvent_t eb = nullptr, hipEvent_t ee = nullptr, hipEvent_t ub = nullptr,
+                    hipEvent_t ue = nullptr) {
+        int rc;
+        if (one_kernel()) {
+            if ((rc = launch_one_kernel(3, eb, ee))) return rc;
+            if (ub) HIPCHK(hipEventRecord(ub, stream));
+            if (ue) HIPCHK(hipEventRecord(ue, stream));
+            flip();
+            return FCC_OK;
+        }
+        if ((rc = launch_forces(1, eb, ee))) return rc;
+        return launch_update(ub, ue);
+    }
+
+    // advance! (src/simulation.jl:181-191) as device work on the stream. An
+    // eager one-kernel step leaves a skin trip in the rebuild flag.
     int device_step() {
         p.skin_check = (has_contact && valid && skin > 0.0) ? 1 : 0;
-        int rc = launch_forces(1);
-        if (rc) return rc;
-        return launch_update();
+        if (!one_kernel()) return launch_step();
+        int rc;
+        if ((rc = ensure_next()) || (rc = sync_slots()) || (rc = launch_step())) return rc;
+        return sync_slots();
     }
 
     // update_celllist! + update_neighbors!(nl) (src/neighbors.jl:81-92,223-242).
@@ -872,13 +984,16 @@ struct Sim final : fcc_sim {
     // costs several µs between replays; inside a graph the kernels follow
     // each other without a gap), so k steps take k / 8 + popcount(k mod 8)
     // launches.
+    // A captured one-kernel step flips the host's buffers as an executed one
+    // would; the parity the capture started from is restored.
     int capture_steps(int steps, hipGraphExec_t *out) {
         hipGraph_t graph;
+        const int par0 = par;
         HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
         int rc = FCC_OK;
-        for (int q = 0; q < steps && !rc; q++)
-            if (!(rc = launch_forces(1))) rc = launch_update();
+        for (int q = 0; q < steps && !rc; q++) rc = launch_step();
         hipError_t e = hipStreamEndCapture(stream, &graph);
+        set_parity(par0);
         if (rc) return rc;
         HIPCHK(e);
         HIPCHK(hipGraphInstantiate(out, graph, nullptr, nullptr, 0));
@@ -888,25 +1003,46 @@ struct Sim final : fcc_sim {
 
     int capture() {
         std::lock_guard<std::mutex> lk(g_capture_lock);
-        for (hipGraphExec_t &g : gx)
-            if (g) {
-                HIPCHK(hipGraphExecDestroy(g));
-                g = nullptr;
-            }
+        for (auto &set : gx)
+            for (hipGraphExec_t &g : set)
+                if (g) {
+                    HIPCHK(hipGraphExecDestroy(g));
+                    g = nullptr;
+                }
         p.skin_check = (has_contact && skin > 0.0) ? 1 : 0;
-        for (int b = 0; b < NGRAPH; b++)
-            if (int rc = capture_steps(1 << b, &gx[b])) return rc;
+        gone = one_kernel();
+        if (gone) {
+            int rc = ensure_next();
+            if (rc) return rc;
+        }
+        // one-kernel steps: a set of graphs for each parity of the buffers
+        const int par0 = par;
+        for (int q = 0; q < (gone ? 2 : 1); q++) {
+            if (gone) set_parity(q);
+            for (int b = 0; b < NGRAPH; b++)
+                if (int rc = capture_steps(1 << b, &gx[q][b])) {
+                    set_parity(par0);
+                    return rc;
+                }
+        }
+        set_parity(par0);
         gexec_gen = gen;
+        if (gone) return sync_slots();  // the steps before may have been two-kernel ones
         return FCC_OK;
     }
 
     // k graph steps: k / GRAPH_STEPS launches of the longest graph, then one
     // launch per set bit of the rest.
+    // With one-kernel steps the host flips its buffers as if every step
+    // executes (an odd graph, the one-step one, flips them).
     int replay(int64_t k) {
         static_assert((1 << (NGRAPH - 1)) == GRAPH_STEPS, "graph sizes");
-        for (int64_t q = 0; q < k / GRAPH_STEPS; q++) HIPCHK(hipGraphLaunch(gx[NGRAPH - 1], stream));
+        for (int64_t q = 0; q < k / GRAPH_STEPS; q++) HIPCHK(hipGraphLaunch(gx[gone ? par : 0][NGRAPH - 1], stream));
         for (int b = NGRAPH - 2; b >= 0; b--)
-            if ((k % GRAPH_STEPS) & (1 << b)) HIPCHK(hipGraphLaunch(gx[b], stream));
+            if ((k % GRAPH_STEPS) & (1 << b)) {
+                HIPCHK(hipGraphLaunch(gx[gone ? par : 0][b], stream));
+                if (gone && b == 0) flip();
+            }
         return FCC_OK;
     }
 
@@ -968,8 +1104,7 @@ struct Sim final : fcc_sim {
             // the same kernels and arguments as the graph
             if ((rc = replay(k - 1))) return rc;
             p.skin_check = (has_contact && skin > 0.0) ? 1 : 0;
-            if ((rc = launch_forces(1, tev[slot][0], tev[slot][1]))) return rc;
-            if ((rc = launch_update(tev[slot][2], tev[slot][3]))) return rc;
+            if ((rc = launch_step(tev[slot][0], tev[slot][1], tev[slot][2], tev[slot][3]))) return rc;
         }
         HIPCHK(hipMemcpyAsync(snap + slot, d.st, sizeof(DevStatus), hipMemcpyDeviceToHost, stream));
         HIPCHK(hipEventRecord(sev[slot], stream));
@@ -996,6 +1131,7 @@ struct Sim final : fcc_sim {
         int rc;
         if ((rc = ensure_pipeline())) return rc;
         if (has_contact && !valid && (rc = rebuild())) return rc;
+        if (one_kernel() && (rc = sync_slots())) return rc;
         // The status at the start of the run is one more snapshot (slot NQ),
         // read while the first chunks run. A rebuild flag raised by the last
         // step before the run freezes them, and the lists are rebuilt before
@@ -1006,6 +1142,9 @@ struct Sim final : fcc_sim {
         DevStatus st;
         int64_t clock = 0, done = 0, planned = 0, issued = 0, read = 0;
         int64_t kq[NQ] = {};
+        // parity of the buffers holding the state the device has reached
+        // (the host's buffers run ahead by the queued one-kernel steps)
+        int pact = par;
         auto account = [&](int64_t executed) {
             for (int64_t q = 0; q < executed; q++) t += cfg.dt;
             nstep += executed;
@@ -1017,15 +1156,17 @@ struct Sim final : fcc_sim {
         auto drain = [&]() -> int {
             HIPCHK(hipMemcpyAsync(&st, d.st, sizeof st, hipMemcpyDeviceToHost, stream));
             HIPCHK(hipStreamSynchronize(stream));
+            if (gone) pact ^= (int)((st.clock - clock) & 1);
             account(st.clock - clock);
             clock = st.clock;
             read = issued;
             planned = done;
+            set_parity(pact);
             return FCC_OK;
         };
         while (done < n) {
             while (issued - read < NQ && planned < n) {
-                if (!gx[0] || gexec_gen != gen) {
+                if (!gx[0][0] || gexec_gen != gen) {
                     if (issued > read) break;  // capture with nothing queued
                     if ((rc = capture())) return rc;
                 }
@@ -1056,6 +1197,9 @@ struct Sim final : fcc_sim {
             read++;
             const int64_t executed = s.clock - clock;
             clock = s.clock;
+            if (gone) pact ^= (int)(executed & 1);
+            // a rebuild is due: the flag, or a skin trip of the last one-kernel step
+            const bool due = s.rebuild != 0 || (gone && s.tstep[pact] == s.clock - 1);
             if (want_timed && executed == kq[slot]) {  // the chunk's last step ran (not frozen)
                 float a = 0.f, b = 0.f;
                 HIPCHK(hipEventElapsedTime(&a, tev[slot][0], tev[slot][1]));
@@ -1065,9 +1209,9 @@ struct Sim final : fcc_sim {
                 acc_n++;
             }
             account(executed);
-            if (s.rebuild != 0 && (rc = drain())) return rc;
-            if (s.rebuild != 0 && done < n) {
-                if ((rc = update_neighbors(0))) return rc;
+            if (due && (rc = drain())) return rc;
+            if (due && done < n) {
+                if ((rc = rebuild()) || (gone && (rc = sync_slots()))) return rc;
                 interval = (interval + (double)since_rebuild) / 2;
                 since_rebuild = 0;
                 gbuilds = builds;
@@ -1075,6 +1219,8 @@ struct Sim final : fcc_sim {
                 interval = std::min((double)since_rebuild, 4.0 * 256);
             }
         }
+        set_parity(pact);
+        if (gone) return sync_slots();  // a trip of the last step becomes the rebuild flag
         return FCC_OK;
     }
 
@@ -1337,6 +1483,11 @@ struct Sim final : fcc_sim {
     // unchanged.
     int time_kernels(int64_t reps, double *fms, double *ums) override {
         if (has_contact && !valid) return fail(FCC_ERR_STATE, "neighbour list not built (call fcc_update_neighbors)");
+        const bool one = one_kernel();
+        if (one) {
+            int rc = ensure_next();
+            if (rc) return rc;
+        }
         if (reps < 1) return fail(FCC_ERR_ARGUMENT, "reps must be positive");
         const auto regions = step_regions();
         size_t bytes = 0;
@@ -1358,14 +1509,23 @@ struct Sim final : fcc_sim {
             return FCC_OK;
         };
         // series s: reps forces launches (s = 0) or reps steps (s = 1)
+        // (one-kernel steps: without the freeze check, the host's buffers
+        // restored after the capture)
         auto capture_series = [&](int s, hipGraphExec_t *out) -> int {
             std::lock_guard<std::mutex> lk(g_capture_lock);
             hipGraph_t graph;
+            const int par0 = par;
             HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
             int r = FCC_OK;
-            for (int64_t q = 0; q < reps && !r; q++)
-                if (!(r = launch_forces(0)) && s == 1) r = launch_update();
+            for (int64_t q = 0; q < reps && !r; q++) {
+                if (s == 1 && one) {
+                    if (!(r = launch_one_kernel(4, nullptr, nullptr))) flip();
+                } else if (!(r = launch_forces(0)) && s == 1) {
+                    r = launch_update();
+                }
+            }
             hipError_t e = hipStreamEndCapture(stream, &graph);
+            set_parity(par0);
             if (r) return r;
             HIPCHK(e);
             e = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
@@ -1423,7 +1583,10 @@ struct Sim final : fcc_sim {
             if (has_contact && (rc = update_neighbors(0))) break;
             p.skin_check = (has_contact && valid && skin > 0.0) ? 1 : 0;
             hipError_t e = hipSuccess;
-            if (!(rc = launch_forces(1, e0, e1))) rc = launch_update(e2, e3);
+            const bool one = one_kernel();
+            if (one && ((rc = ensure_next()) || (rc = sync_slots()))) break;
+            rc = launch_step(e0, e1, e2, e3);
+            if (!rc && one) rc = sync_slots();
             if (!rc) e = hipEventSynchronize(e3);
             float a = 0.f, b = 0.f;
             if (!rc && e == hipSuccess) e = hipEventElapsedTime(&a, e0, e1);
@@ -1687,6 +1850,7 @@ struct Sim final : fcc_sim {
         st.stop = 0;
         st.clock = h.clock;
         st.target = INT64_MAX;
+        st.tstep[0] = st.tstep[1] = INT64_MIN;  // a pending skin trip is in the saved rebuild flag
         HIPCHK(hipMemcpyAsync(d.st, &st, sizeof st, hipMemcpyHostToDevice, stream));
         HIPCHK(hipStreamSynchronize(stream));
         nstep = h.step;

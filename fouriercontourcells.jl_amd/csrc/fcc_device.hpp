@@ -29,6 +29,11 @@ struct DevStatus {
     int64_t clock;
     int64_t target;
     int32_t builds, pad;
+    // One-kernel steps, by the parity q of the buffers a step reads: the
+    // step's clock (cslot[q], passed on by the step before), and the clock
+    // of the last step before it whose bodies tripped the skin check
+    // (tstep[q] = clock - 1 freezes the step).
+    int64_t cslot[2], tstep[2];
 };
 
 // Status prologue of the first kernel of a step: block 0 mirrors the freeze
@@ -56,10 +61,39 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t n) {
 // Status check of the forces kernel: use_status 1 = the kernel opens the step
 // (step_frozen); 2 = a gate kernel already took the step's freeze decision
 // into `update` (ensemble steps, whose lists are rebuilt concurrently).
+// One-kernel steps open with step_open instead.
 __device__ __forceinline__ bool forces_frozen(int use_status, DevStatus *st, int64_t blk, int lane) {
     if (use_status == 1) return step_frozen(st, blk, lane);
     if (use_status == 2) return st->update != 0;
     return false;
+}
+
+// Status prologue of a one-kernel step reading the buffers of parity q. Every
+// workgroup reads only fields that no workgroup of the launch writes, so the
+// decision is uniform: the step is frozen by a raised rebuild flag, a reached
+// step budget or a skin trip of the step before (tstep[q] = clock - 1; that
+// step wrote the marker while this launch's trips go to tstep[1 - q]).
+// Workgroup 0 passes the clock on to the next step (cslot[1 - q]) and to
+// `clock`, or, in a step frozen by a trip, raises the rebuild flag so that
+// the freeze holds for the steps after it. `timing` (state-restoring timing
+// series) never freezes. Returns whether the step is frozen; clk is its clock
+// (the Philox draw index).
+__device__ __forceinline__ bool step_open(DevStatus *st, int q, bool timing, int64_t blk, int lane,
+                                          int64_t &clk) {
+    const int64_t c = st->cslot[q];
+    const bool trip = st->tstep[q] == c - 1;
+    const bool fr = !timing && (st->rebuild != 0 || st->stop != 0 || c >= st->target || trip);
+    if (blk == 0 && lane == 0) {
+        if (!fr) {
+            st->cslot[q ^ 1] = c + 1;
+            st->clock = c + 1;
+            st->update = 0;
+        } else if (trip) {
+            st->rebuild = 1;
+        }
+    }
+    clk = c;
+    return fr;
 }
 
 // Scalar parameters of one simulation, passed by value to every kernel.
@@ -119,6 +153,12 @@ struct Ptrs {
     int32_t cap;
     double *x0, *y0;
     T *rmax0;
+    // one-kernel steps: the buffers the step writes the new state into
+    // (positions, lab rows, radius bounds); the kernels read x, y, lab, rmax,
+    // rmin of the current state, which neighbours in other workgroups still
+    // read while a body is updated
+    double *xn, *yn;
+    T *labn, *rmaxn, *rminn;
     int32_t *cidx, *ccount;
     int32_t ccap;
     T *disp;

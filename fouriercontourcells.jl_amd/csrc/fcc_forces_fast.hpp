@@ -433,24 +433,38 @@ __device__ __forceinline__ void quarter_turn(T &c, T &s, int r, bool inv) {
 // once, so all groups read the same wave-uniform basis rows through the scalar
 // cache, and its pullback sums take the inverse turn before the groups are
 // added.
-template <typename T, int K, int NCLS, int LPB, typename PP, typename DP>
+//
+// NW waves per workgroup (NW = 2 only with two lanes per body), wave w on the
+// bodies of tile wg·NW + w; with NW = 2 the workgroup is a one-kernel step
+// (the pullback rows and centre force / torque go to the waves' LDS slices,
+// then wave 0 updates the 64 bodies, fused_update).
+template <typename T, int K, int NCLS, int LPB, int NW, typename PP, typename DP>
 __device__ __forceinline__ void k_forces_fast2_impl(const PP &p, const DP &d, int use_status, int debug,
-                                                     int64_t blk) {
+                                                     int64_t wg, int q = 0) {
     static_assert(LPB == 2 || LPB == 4, "two or four lanes per body");
+    static_assert(NW == 1 || (NW == 2 && LPB == 2), "two waves per workgroup with two lanes per body");
     constexpr int NTH = 40, NB = 64 / LPB, NH2 = NTH / LPB;
     constexpr int NC = 2 * K + 1;
     constexpr int NCL = NCLS > 0 ? NCLS : 1;
     constexpr int W = row_width<NC, T>();
-    __shared__ T sR[NTH][NB];
+    static_assert(NB * W + 3 * NB <= NTH * NB, "pullback rows fit the wave's LDS slice");
+    __shared__ T sRw[NW][NTH][NB];
     __shared__ T scs[2][NTH];
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63, w = NW == 1 ? 0 : (int)(threadIdx.x >> 6);
+    T(*sR)[NB] = sRw[w];
     const int b = lane & (NB - 1), h = lane / NB;
-    for (int k = lane; k < NTH; k += 64) {
+    for (int k = threadIdx.x; k < NTH; k += 64 * NW) {
         scs[0][k] = d.cs[2 * k];
         scs[1][k] = d.cs[2 * k + 1];
     }
+    const int64_t blk = wg * NW + w;
     const int64_t i = (int64_t)blk * NB + b;
-    if (forces_frozen(use_status, d.st, blk, lane)) return;  // uniform over the grid
+    int64_t clk = 0;  // one-kernel step: its clock
+    if constexpr (NW == 2) {
+        if (step_open(d.st, q, use_status == 4, blk, lane, clk)) return;  // uniform over the grid
+    } else {
+        if (forces_frozen(use_status, d.st, blk, lane)) return;  // uniform over the grid
+    }
     const bool live = i < p.N;
     const int64_t ii = live ? i : 0;
     const FCC_CONST T *S = as_const(d.Sc);
@@ -714,11 +728,35 @@ __device__ __forceinline__ void k_forces_fast2_impl(const PP &p, const DP &d, in
         d.gth[ii] = tt;
         d.Ebody[ii] = E;
     }
+    if constexpr (NW == 2) {
+        // the same row, force and torque into the wave's LDS slice (pass 2
+        // has read its contour slots): body b at rows[b·W], the force and
+        // torque after the NB rows
+        T *rows = &sR[0][0];
+        if (h == 0) {
+#pragma unroll
+            for (int c = 0; c < NC; c++) rows[b * W + c] = gl[c];
+            rows[NB * W + 3 * b] = fx;
+            rows[NB * W + 3 * b + 1] = fy;
+            rows[NB * W + 3 * b + 2] = tt;
+        }
+        __syncthreads();
+        if (w != 0) return;
+        fused_update<T, K, false>(p, d, &sRw[0][0][0], NTH * NB, NB, wg * 64, lane, q, clk);
+    }
 }
 
 template <typename T, int K, int NCLS, int LPB>
 __global__ __launch_bounds__(64) void k_forces_fast2(const Params<T> p, const Ptrs<T> d, int use_status, int debug) {
-    k_forces_fast2_impl<T, K, NCLS, LPB>(p, d, use_status, debug, xcd_tile(blockIdx.x, gridDim.x));
+    k_forces_fast2_impl<T, K, NCLS, LPB, 1>(p, d, use_status, debug, xcd_tile(blockIdx.x, gridDim.x));
+}
+
+// One-kernel step of the plain path (two lanes per body, two waves and 64
+// bodies per workgroup) reading the buffers of parity q; use_status 3, or 4
+// for timing series.
+template <typename T, int K, int NCLS>
+__global__ __launch_bounds__(128) void k_step_fast2(const Params<T> p, const Ptrs<T> d, int use_status, int q) {
+    k_forces_fast2_impl<T, K, NCLS, 2, 2>(p, d, use_status, 0, xcd_tile(blockIdx.x, gridDim.x), q);
 }
 
 template <typename T, int K, int NCLS, int LPB>
@@ -728,7 +766,7 @@ __global__ __launch_bounds__(64) void k_forces_fast2_ens(const Params<T> *ps, co
     if (tl.rep < 0) return;
     const FCC_CONST Params<T> &p = as_const(ps)[tl.rep];
     const FCC_CONST Ptrs<T> &d = as_const(ds)[tl.rep];
-    k_forces_fast2_impl<T, K, NCLS, LPB>(p, d, use_status, debug, tl.blk);
+    k_forces_fast2_impl<T, K, NCLS, LPB, 1>(p, d, use_status, debug, tl.blk);
 }
 
 }  // namespace fcc

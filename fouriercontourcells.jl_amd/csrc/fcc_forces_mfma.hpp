@@ -55,15 +55,23 @@ struct MfmaType<true> {
     }
 };
 
+// Each value is pinned in a register as the rounded FP32 result of the
+// operation that produced it (an empty asm with a read-write operand) before
+// it is split: otherwise the compiler may fold the producing multiply into
+// the conversion (v_fma_mix, one rounding to 16 bits instead of two) in some
+// kernels and not in others, and kernels that share this code would round
+// differently.
 template <bool BF>
 __device__ __forceinline__ void split8(const float (&x)[8], typename MfmaType<BF>::v_t &hi,
                                        typename MfmaType<BF>::v_t &lo) {
     typedef typename MfmaType<BF>::s_t s_t;
 #pragma unroll
     for (int s = 0; s < 8; s++) {
-        const s_t h = (s_t)x[s];
+        float v = x[s];
+        asm volatile("" : "+v"(v));
+        const s_t h = (s_t)v;
         hi[s] = h;
-        lo[s] = (s_t)(x[s] - (float)h);
+        lo[s] = (s_t)(v - (float)h);
     }
 }
 
@@ -109,24 +117,39 @@ __device__ __forceinline__ void synth_tiles(const float *lab_row, bool live, int
 // Operand tables (built on the host, 16-bit values):
 //  asyn[(t * 2 + part) * 64 + lane][8]   synthesis A (S rows), part 0 hi / 1 lo
 //  bpull[(c * 2 + v) * 64 + lane][8]     pullback B, v 0 = [Sh | Sl], 1 = [Sh | 0]
-template <int K, int T, int NCLS, bool BF, typename PP, typename DP>
+//
+// NW waves per workgroup, wave w on the 32 bodies of tile wg·NW + w. With
+// NW = 2 the workgroup is a one-kernel step: the waves leave their bodies'
+// pullback rows and centre force / torque in their LDS slices (the contour
+// slots are dead after the contact pass), and wave 0 updates the 64 bodies
+// (fused_update).
+template <int K, int T, int NCLS, bool BF, int NW, typename PP, typename DP>
 __device__ __forceinline__ void k_forces_mfma_impl(const PP &p, const DP &d,
                                                     const uint16_t *asyn, const uint16_t *bpull, int use_status,
-                                                    int debug, int64_t blk) {
+                                                    int debug, int64_t wg, int q = 0) {
     typedef MfmaType<BF> MT;
     typedef typename MT::v_t v_t;
     constexpr int NC = 2 * K + 1;
     constexpr int W = row_width<NC, float>();
-    __shared__ float sR[NTH_M][32];  // R of the wave's 32 bodies for the contact pass
+    static_assert(NW == 1 || NW == 2, "one or two waves per workgroup");
+    static_assert(32 * W + 3 * 32 <= NTH_M * 32, "pullback rows fit the wave's LDS slice");
+    __shared__ float sRw[NW][NTH_M][32];  // R of each wave's 32 bodies for the contact pass
     __shared__ float scs[2][NTH_M];
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63, w = NW == 1 ? 0 : (int)(threadIdx.x >> 6);
+    float(*sR)[32] = sRw[w];
     const int b = lane & 31, h = lane >> 5;
-    for (int k = lane; k < NTH_M; k += 64) {
+    for (int k = threadIdx.x; k < NTH_M; k += 64 * NW) {
         scs[0][k] = d.cs[2 * k];
         scs[1][k] = d.cs[2 * k + 1];
     }
+    const int64_t blk = wg * NW + w;
     const int64_t i = (int64_t)blk * 32 + b;
-    if (forces_frozen(use_status, d.st, blk, lane)) return;  // uniform over the grid
+    int64_t clk = 0;  // one-kernel step: its clock
+    if constexpr (NW == 2) {
+        if (step_open(d.st, q, use_status == 4, blk, lane, clk)) return;  // uniform over the grid
+    } else {
+        if (forces_frozen(use_status, d.st, blk, lane)) return;  // uniform over the grid
+    }
     const bool live = i < p.N;
     const int64_t ii = live ? i : 0;
     const float r0 = live ? d.R0[ii] : 1.f;
@@ -421,13 +444,39 @@ __device__ __forceinline__ void k_forces_mfma_impl(const PP &p, const DP &d,
         d.gth[ii] = tt;
         d.Ebody[ii] = E;
     }
+    if constexpr (NW == 2) {
+        // the same rows into the wave's LDS slice: body rr of the wave at
+        // rows[rr·W], its force and torque after the 32 rows
+        float *rows = &sR[0][0];
+        if (n < NC) {
+#pragma unroll
+            for (int r = 0; r < 16; r++) rows[(4 * h + 8 * (r >> 2) + (r & 3)) * W + n] = gacc[r] * (1.f / PULL_SCALE);
+        }
+        if (h == 0) {
+            rows[32 * W + 3 * b] = fx;
+            rows[32 * W + 3 * b + 1] = fy;
+            rows[32 * W + 3 * b + 2] = tt;
+        }
+        __syncthreads();
+        if (w != 0) return;
+        fused_update<float, K, true>(p, d, &sRw[0][0][0], NTH_M * 32, 32, wg * 64, lane, q, clk);
+    }
 }
 
 template <int K, int T, int NCLS, bool BF>
 __global__ __launch_bounds__(64) void k_forces_mfma(const Params<float> p, const Ptrs<float> d,
                                                     const uint16_t *asyn, const uint16_t *bpull, int use_status,
                                                     int debug) {
-    k_forces_mfma_impl<K, T, NCLS, BF>(p, d, asyn, bpull, use_status, debug, xcd_tile(blockIdx.x, gridDim.x));
+    k_forces_mfma_impl<K, T, NCLS, BF, 1>(p, d, asyn, bpull, use_status, debug, xcd_tile(blockIdx.x, gridDim.x));
+}
+
+// One-kernel step of the mixed path (two waves, 64 bodies per workgroup)
+// reading the buffers of parity q; use_status 3, or 4 for timing series.
+template <int K, int T, int NCLS, bool BF>
+__global__ __launch_bounds__(128) void k_step_mfma(const Params<float> p, const Ptrs<float> d,
+                                                   const uint16_t *asyn, const uint16_t *bpull, int use_status,
+                                                   int q) {
+    k_forces_mfma_impl<K, T, NCLS, BF, 2>(p, d, asyn, bpull, use_status, 0, xcd_tile(blockIdx.x, gridDim.x), q);
 }
 
 template <int K, int T, int NCLS, bool BF>
@@ -438,7 +487,7 @@ __global__ __launch_bounds__(64) void k_forces_mfma_ens(const Params<float> *ps,
     if (tl.rep < 0) return;
     const FCC_CONST Params<float> &p = as_const(ps)[tl.rep];
     const FCC_CONST Ptrs<float> &d = as_const(ds)[tl.rep];
-    k_forces_mfma_impl<K, T, NCLS, BF>(p, d, asyn, bpull, use_status, debug, tl.blk);
+    k_forces_mfma_impl<K, T, NCLS, BF, 1>(p, d, asyn, bpull, use_status, debug, tl.blk);
 }
 
 }  // namespace fcc

@@ -37,9 +37,11 @@ struct Mfma16<false> {
     static __device__ __forceinline__ void split(const float (&x)[4], v_t &hi, v_t &lo) {
 #pragma unroll
         for (int s = 0; s < 4; s++) {
-            const _Float16 h = (_Float16)x[s];
+            float v = x[s];
+            asm volatile("" : "+v"(v));  // as in split8: the FP32 value, then its split
+            const _Float16 h = (_Float16)v;
             hi[s] = h;
-            lo[s] = (_Float16)(x[s] - (float)h);
+            lo[s] = (_Float16)(v - (float)h);
         }
     }
 };
@@ -52,8 +54,10 @@ struct Mfma16<true> {
     static __device__ __forceinline__ void split(const float (&x)[4], v_t &hi, v_t &lo) {
 #pragma unroll
         for (int s = 0; s < 4; s++) {
-            const __bf16 h = (__bf16)x[s];
-            const __bf16 l = (__bf16)(x[s] - (float)h);
+            float v = x[s];
+            asm volatile("" : "+v"(v));  // as in split8: the FP32 value, then its split
+            const __bf16 h = (__bf16)v;
+            const __bf16 l = (__bf16)(v - (float)h);
             hi[s] = __builtin_bit_cast(short, h);
             lo[s] = __builtin_bit_cast(short, l);
         }

@@ -613,11 +613,12 @@ __device__ __forceinline__ T shape_step(const PP &p, const TypeParams<T> &tp, in
 // frame of the new state (src/geometry.jl:73-92) and the Verlet skin check
 // (src/neighbors.jl:198-205) of body i, given its position (x, y), its new
 // coefficients c and orientation thn, its displacement (d0, d1) and the
-// skin reference (x0, y0, rmax0).
+// skin reference (x0, y0, rmax0). Returns whether a body of the wave tripped
+// the skin check; `trip` (if not null) is then set to 1.
 template <typename T, int KM, bool EXACT, typename PP, typename DP>
-__device__ __forceinline__ void place_body(const PP &p, const DP &d, int64_t i, int K, const T (&c)[2 * KM + 1],
+__device__ __forceinline__ bool place_body(const PP &p, const DP &d, int64_t i, int K, const T (&c)[2 * KM + 1],
                                            T thn, T r0, T d0, T d1, double x, double y, double x0, double y0,
-                                           T rmax0) {
+                                           T rmax0, int32_t *trip) {
 #pragma clang fp contract(off)
     constexpr int NCM = 2 * KM + 1;
     constexpr int W = row_width<NCM, T>();
@@ -648,9 +649,11 @@ __device__ __forceinline__ void place_body(const PP &p, const DP &d, int64_t i, 
         const double delta = sqrt(dx * dx + dy * dy) + (dr > 0.0 ? dr : 0.0);
         // one store per wave whose bodies tripped (not one per body)
         const uint64_t trips = __ballot(2.0 * delta > (double)p.skin);
-        if (trips != 0 && (int)(threadIdx.x & 63) == __ffsll((long long)trips) - 1)
-            __hip_atomic_store(&d.st->rebuild, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (trip && trips != 0 && (int)(threadIdx.x & 63) == __ffsll((long long)trips) - 1)
+            __hip_atomic_store(trip, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return trips != 0;
     }
+    return false;
 }
 
 template <typename T, int KM, bool EXACT>
@@ -720,10 +723,12 @@ __device__ __forceinline__ void load_body(const PP &p, const DP &d, int64_t i, i
 // The update of body i from its loaded inputs: displacement and noise, θ and
 // shape step, wrap, lab frame, skin check and stores. sin θ, cos θ and the
 // harmonics of θ are computed once for the displacement, the body-frame
-// coefficients and the body-frame gradient.
+// coefficients and the body-frame gradient. Returns whether a body of the
+// wave tripped the skin check (which also sets *trip unless it is null);
+// with `budget` body 0 takes the step budget into `stop`.
 template <typename T, int KM, bool EXACT, bool F64S, typename PP, typename DP>
-__device__ __forceinline__ void finish_body(const PP &p, const DP &d, int64_t i, int K, int use_disp, int debug,
-                                            uint64_t draw, BodyIn<T, KM> &b) {
+__device__ __forceinline__ bool finish_body(const PP &p, const DP &d, int64_t i, int K, int use_disp, int debug,
+                                            uint64_t draw, BodyIn<T, KM> &b, int32_t *trip, bool budget) {
     constexpr int NCM = 2 * KM + 1;
     const TypeParams<T> tp = type_params<T>(p, b.ti);
     T cn[KM + 1], sn[KM + 1];
@@ -749,8 +754,8 @@ __device__ __forceinline__ void finish_body(const PP &p, const DP &d, int64_t i,
     const T thn = shape_step<T, KM, F64S>(p, tp, K, b.gl, b.c, cn, sn, b.th, b.r0, b.gth, d2, gc);
     d.th[i] = thn;
     if (debug) store_coef<T, KM, EXACT>(d.gcoef + i * p.ldc, gc, K);
-    if (i == 0) d.st->stop = d.st->clock >= d.st->target ? 1 : 0;
-    place_body<T, KM, EXACT>(p, d, i, K, b.c, thn, b.r0, d0, d1, b.x, b.y, b.x0, b.y0, b.rmax0);
+    if (budget && i == 0) d.st->stop = d.st->clock >= d.st->target ? 1 : 0;
+    return place_body<T, KM, EXACT>(p, d, i, K, b.c, thn, b.r0, d0, d1, b.x, b.y, b.x0, b.y0, b.rmax0, trip);
 }
 
 // Fused Euler–Maruyama update (src/dynamics.jl:180-203) with the
@@ -767,7 +772,7 @@ __device__ __forceinline__ void k_update_impl(const PP &p, const DP &d, int use_
     load_body<T, KM, EXACT>(p, d, i, K, use_disp, b);
     if (d.st->update != 0) return;
     const uint64_t draw = (uint64_t)(d.st->clock - 1);
-    finish_body<T, KM, EXACT, F64S>(p, d, i, K, use_disp, debug, draw, b);
+    (void)finish_body<T, KM, EXACT, F64S>(p, d, i, K, use_disp, debug, draw, b, &d.st->rebuild, true);
 }
 
 template <typename T, int KM, bool EXACT, bool F64S>
@@ -784,6 +789,57 @@ __global__ __launch_bounds__(256) void k_update_ens(const Params<T> *ps, const P
     const FCC_CONST Params<T> &p = as_const(ps)[tl.rep];
     const FCC_CONST Ptrs<T> &d = as_const(ds)[tl.rep];
     k_update_impl<T, KM, EXACT, F64S, 0>(p, d, use_disp, debug, tl.blk);
+}
+
+// Update part of a one-kernel step (single simulations without parent drag,
+// bonds or debug buffers) reading the buffers of parity q at clock clk (the
+// status prologue, step_open): the forces part of the workgroup left, for each
+// of its 64 bodies, the pullback row (W values, at rows + (l / nb)·stride +
+// (l % nb)·W for body l of the workgroup) and the centre force and torque
+// (after the nb rows of each slice). One wave updates the 64 bodies, one
+// body per lane, with the arithmetic of k_update: it reads this step's state
+// and writes positions, lab rows and radius bounds into the next-state
+// buffers (d.xn, ...), since other workgroups may still read this step's
+// state as neighbours; θ and the image counters stay in place (only the body
+// itself reads them). The draw index is the step's clock. A skin trip marks
+// tstep[1 - q] with the clock, which freezes the next step. The rows are LDS
+// written before the workgroup barrier that precedes this call.
+template <typename T, int KM, bool F64S, typename PP, typename DP>
+__device__ __forceinline__ void fused_update(const PP &p, const DP &d, const T *rows, int stride, int nb,
+                                             int64_t body0, int lane, int q, int64_t clk) {
+    constexpr int NCM = 2 * KM + 1;
+    constexpr int W = row_width<NCM, T>();
+    const int64_t i = body0 + lane;
+    bool tripped = false;
+    if (i < p.N) {
+        const T *row = rows + (lane / nb) * stride + (lane % nb) * W;
+        const T *f = rows + (lane / nb) * stride + nb * W + 3 * (lane % nb);
+        BodyIn<T, KM> b;
+        b.ti = d.type[i];
+        b.th = d.th[i];
+        b.r0 = d.R0[i];
+        b.x = d.x[i];
+        b.y = d.y[i];
+        load_coef<T, KM, true>(b.c, d.lab + i * p.ldc, KM);
+        b.x0 = d.x0[i];
+        b.y0 = d.y0[i];
+        b.rmax0 = d.rmax0[i];
+#pragma unroll
+        for (int c = 0; c < NCM; c++) b.gl[c] = row[c];
+        b.f0 = f[0];
+        b.f1 = f[1];
+        b.gth = f[2];
+        b.f2 = b.f0;
+        DP dn = d;
+        dn.x = d.xn;
+        dn.y = d.yn;
+        dn.lab = d.labn;
+        dn.rmax = d.rmaxn;
+        dn.rmin = d.rminn;
+        tripped = finish_body<T, KM, true, F64S>(p, dn, i, KM, 0, 0, (uint64_t)clk, b, nullptr, false);
+    }
+    if (__ballot(tripped) != 0 && lane == 0)
+        __hip_atomic_store(&d.st->tstep[q ^ 1], clk, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ------------------------------------------------------- cell / neighbours
@@ -1030,6 +1086,17 @@ __global__ void k_clear_status(DevStatus *st) {
     st->rebuild = 0;
     st->update = 0;
     st->overflow = 0;
+    st->tstep[0] = st->tstep[1] = INT64_MIN;
+}
+
+// Before one-kernel steps reading the buffers of parity q: a skin trip of the
+// last one-kernel step (marked at tstep[q] with clock - 1) raises the rebuild
+// flag, the markers are cleared and the step clock slot takes the clock.
+__global__ void k_step_slots(DevStatus *st, int q) {
+    const int64_t c = st->clock;
+    if (st->tstep[q] == c - 1) st->rebuild = 1;
+    st->tstep[0] = st->tstep[1] = INT64_MIN;
+    st->cslot[q] = c;
 }
 
 // Step budget of a simulation: freeze once `clock` reaches `target`.

@@ -33,6 +33,7 @@ FCC_TERM_HARMONIC_BOND = 32
 
 FCC_FLAG_DEBUG_BUFFERS = 1
 FCC_FLAG_TWO_LANES = 2
+FCC_FLAG_TWO_KERNELS = 4
 
 FCC_NOALLOC_MAX_NOISE = 262144
 

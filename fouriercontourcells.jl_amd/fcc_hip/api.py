@@ -360,12 +360,15 @@ class Simulation:
     `engine` is the C-ABI implementation (default: libfcc_hip.so).
     `two_lanes` launches the two-lane forces kernels at every size (the
     library picks four lanes per body below 131072 bodies), for tests.
+    `two_kernels` keeps a step as a forces and an update kernel where the
+    library would run it as one kernel (bitwise the same results), for tests
+    and kernel profiles.
     """
 
     def __init__(self, bodies, box, types, energies=(), dt=0.005, ntheta=40, precision="fp64",
-                 seed=1, skin=0.3, debug=False, engine=None, bonds=(), two_lanes=False):
+                 seed=1, skin=0.3, debug=False, engine=None, bonds=(), two_lanes=False, two_kernels=False):
         self._setup(engine if engine is not None else _abi.hip_engine(), bodies, box, types, energies,
-                    dt, ntheta, precision, seed, skin, debug, two_lanes)
+                    dt, ntheta, precision, seed, skin, debug, two_lanes, two_kernels)
         h = ctypes.c_void_p()
         _check(self.engine, self.engine.create(ctypes.byref(self._cfg),
                                                ctypes.byref(self._bodies.struct()), ctypes.byref(h)))
@@ -397,7 +400,7 @@ class Simulation:
         self.bonds = [tuple(int(v) for v in b) for b in pairs]
 
     def _setup(self, engine, bodies, box, types, energies, dt, ntheta, precision, seed, skin, debug,
-               two_lanes=False):
+               two_lanes=False, two_kernels=False):
         self.engine = engine
         self.box = box
         self.types = tuple(types)
@@ -408,7 +411,8 @@ class Simulation:
         self.precision = precision
         self._bodies = bodies.copy()
         self._owner = None
-        flags = (_abi.FCC_FLAG_DEBUG_BUFFERS if debug else 0) | (_abi.FCC_FLAG_TWO_LANES if two_lanes else 0)
+        flags = ((_abi.FCC_FLAG_DEBUG_BUFFERS if debug else 0) | (_abi.FCC_FLAG_TWO_LANES if two_lanes else 0)
+                 | (_abi.FCC_FLAG_TWO_KERNELS if two_kernels else 0))
         self._cfg = _config(bodies, box, types, energies, dt, ntheta, precision, seed, skin, flags)
 
     def close(self):

@@ -68,10 +68,15 @@ enum fcc_term_kind {
 /* Simulation flags. */
 enum fcc_flags {
     FCC_FLAG_DEBUG_BUFFERS = 1, /* keep P, GR, GD, GDD and gcoef for inspection */
-    FCC_FLAG_TWO_LANES = 2      /* forces kernels with two lanes per body at every N
+    FCC_FLAG_TWO_LANES = 2,     /* forces kernels with two lanes per body at every N
                                    (by default four lanes below 131072 bodies at
                                    n_theta = 40); same results up to rounding, for
                                    tests of the two-lane kernels on small systems */
+    FCC_FLAG_TWO_KERNELS = 4    /* steps as a forces and an update kernel where the
+                                   one-kernel step would apply (single FP32 / mixed
+                                   simulations on the two-lane kernels without parent
+                                   drag, bonds or debug buffers); bitwise the same
+                                   results, for tests and kernel profiles */
 };
 
 /* BodyType (src/bodies.jl:30-45). */

@@ -1,0 +1,164 @@
+"""One-kernel steps (k_step_mfma, k_step_fast2): the forces and the update of
+a step in one kernel, whose workgroups write the new positions, lab rows and
+radius bounds into the other buffer of each pair while other workgroups still
+read the current state as neighbours. Every result must equal bitwise that of
+the two-kernel step (FCC_FLAG_TWO_KERNELS: the forces kernel, then the update
+kernel), whose arithmetic it shares: in graph mode and eagerly, over rebuilds,
+frozen chunks, odd step counts, checkpoints, observers and state reads. The
+two-kernel step itself is pinned to the FP64 oracle by test_gpu_parity.py and
+test_gpu_precision.py; ensembles (two-kernel) against one-kernel single runs
+are in test_gpu_ensemble.py and test_gpu_config4.py."""
+import sys
+
+import numpy as np
+import pytest
+
+import fcc_hip as F
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def monolayer(precision, two_kernels, n=16, maxmode=8, seed=3, v0=0.6, skin=0.08, types=None, energies=None,
+              two_lanes=True):
+    b, box = F.hexagonal_monolayer(n, n, packing=0.95, maxmode=maxmode, seed=seed)
+    rng = np.random.default_rng(seed)
+    K = (b.coef.shape[0] - 1) // 2
+    b.coef[1:, :] = rng.uniform(-0.03, 0.03, size=(2 * K, len(b)))
+    if types is None:
+        types = (F.BodyType(v0=v0, Dr=0.3, Dt=0.002),)
+    if energies is None:
+        energies = (F.Tension(0.01), F.Overlap(1.0))
+    return F.Simulation(b, box, types=types, energies=energies, dt=0.005, skin=skin, seed=seed,
+                        precision=precision, two_lanes=two_lanes, two_kernels=two_kernels)
+
+
+def state(s):
+    b = s.bodies
+    g = s.grad
+    c = s.counters()
+    return (b.x, b.y, b.theta, b.coef, b.imgx, b.imgy, g.gx, g.gy, g.gth, g.glab, g.Ebody), \
+        (c.step, c.t, c.clock, c.builds)
+
+
+def assert_same(a, b, what):
+    (sa, ca), (sb, cb) = state(a), state(b)
+    names = ("x", "y", "theta", "coef", "imgx", "imgy", "gx", "gy", "gth", "glab", "Ebody")
+    for name, u, v in zip(names, sa, sb):
+        assert np.array_equal(u, v), f"{what}: {name} differs (max {np.max(np.abs(u - v))})"
+    assert ca == cb, f"{what}: counters {ca} != {cb}"
+
+
+@pytest.mark.parametrize("precision", ["fp32", "mixed_fp16", "mixed_bf16"])
+@pytest.mark.parametrize("maxmode", [6, 8])
+def test_one_kernel_equals_two_kernels(precision, maxmode):
+    say("one kernel == two kernels", precision, maxmode)
+    a, b = monolayer(precision, False, maxmode=maxmode), monolayer(precision, True, maxmode=maxmode)
+    # odd counts: the state ends in either buffer of the pairs
+    for n in (137, 1, 64):
+        F.run(a, n)
+        F.run(b, n)
+        assert_same(a, b, f"run {n}")
+    F.step(a, 3)
+    F.step(b, 3)
+    assert_same(a, b, "eager steps")
+    assert a.counters().builds > 5
+    # a checkpoint taken from either buffer parity resumes alike
+    c = monolayer(precision, False, maxmode=maxmode).restore(a.checkpoint())
+    for s in (a, b, c):
+        F.run(s, 41)
+    assert_same(a, b, "after checkpoint")
+    assert_same(a, c, "restored")
+    # state written between runs (bodies setter) and a rebuilt list
+    for s in (a, b):
+        bo = s.bodies
+        bo.x = bo.x + 0.01
+        s.bodies = bo
+        F.run(s, 25)
+    assert_same(a, b, "after set_bodies")
+
+
+@pytest.mark.parametrize("precision", ["fp32", "mixed_fp16"])
+def test_one_kernel_graph_equals_eager_with_callbacks(precision):
+    say("one kernel graph == eager", precision)
+    a, b = monolayer(precision, False), monolayer(precision, False)
+    F.step(a, 90)
+    seen = []
+    F.run(b, 90, every=7, callback=lambda s: seen.append(s.step_count))
+    assert seen == list(range(7, 89, 7)) + [90]
+    assert_same(a, b, "graph vs eager")
+    a.dt = b.dt = 0.004
+    F.step(a, 31)
+    F.run(b, 31)
+    assert_same(a, b, "after dt change")
+
+
+def test_one_kernel_two_coupling_classes_and_bending():
+    # two coupling classes (the NCLS = 2 kernels) and Bending in the plain
+    # kernel; types 1 and 2 alternate
+    say("one kernel, two classes, bending")
+    types = (F.BodyType(v0=0.5, Dr=0.2), F.BodyType(v0=0.2, Dr=0.4, Dt=0.001))
+    energies = (F.Tension(0.01), F.OnTypes(F.Bending(0.002), 2),
+                F.Overlap([[1.0, 0.5], [0.5, 1.0]]))
+
+    def mk(prec, tk):
+        s = monolayer(prec, tk, types=types, energies=energies)
+        bo = s.bodies
+        bo.type = (np.arange(len(bo.x)) % 2 + 1).astype(np.int32)
+        s.bodies = bo
+        return s
+
+    for prec in ("fp32", "mixed_fp16"):
+        a, b = mk(prec, False), mk(prec, True)
+        F.run(a, 120)
+        F.run(b, 120)
+        assert_same(a, b, prec)
+
+
+def test_one_kernel_observer_and_frames():
+    # device observations between runs, a frame requested before a run (its
+    # contours are those of the state at the request), contours after
+    say("one kernel observer, frames")
+    a, b = monolayer("mixed_fp16", False), monolayer("mixed_fp16", True)
+    for s in (a, b):
+        F.run(s, 11)
+        F.observer_start(s, nbins=64, prange=(3.4, 4.4), max_samples=16)
+        F.run_observed(s, 35, 5)
+        F.frame_request(s)
+        F.run(s, 7)
+    oa, ob = F.observations(a), F.observations(b)
+    assert len(oa.clock) == 7
+    for u, v in ((oa.clock, ob.clock), (oa.msd, ob.msd), (oa.p_mean, ob.p_mean), (oa.hist, ob.hist)):
+        assert np.array_equal(u, v)
+    fa, fb = F.frame_read(a), F.frame_read(b)
+    assert fa[2] == fb[2] == 46
+    assert np.array_equal(fa[0], fb[0]) and np.array_equal(fa[1], fb[1])
+    for u, v in zip(F.contours(a), F.contours(b)):
+        assert np.array_equal(u, v)
+    assert_same(a, b, "observer run")
+
+
+@pytest.mark.parametrize("precision", ["mixed_fp16", "fp32"])
+def test_one_kernel_at_full_size(precision):
+    # the bench workload (N = 262144, default lane choice: two lanes, one kernel)
+    say("one kernel full size", precision)
+    def mk(tk):
+        b, box = F.hexagonal_monolayer(512, 512, packing=1.0, maxmode=8, seed=1234)
+        return F.Simulation(b, box, types=(F.BodyType(v0=0.1, Dr=0.01, muab=0.1),),
+                            energies=(F.Tension(0.01), F.Overlap(1.0)), dt=0.005, ntheta=40,
+                            precision=precision, seed=1234, skin=0.3, two_kernels=tk)
+    a = mk(False)
+    F.run(a, 21)
+    sa = state(a)
+    a.close()
+    b = mk(True)
+    F.run(b, 21)
+    sb = state(b)
+    b.close()
+    for u, v in zip(sa[0], sb[0]):
+        assert np.array_equal(u, v)
+    assert sa[1] == sb[1]
