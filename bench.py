@@ -9,17 +9,20 @@ step over all bodies; steps run in graph mode (captured hipGraph replayed in
 chunks, one host sync per chunk) with the state resident in HBM.
 
 Prints one JSON line: the mixed-precision path's throughput (`value`), the
-plain FP32 and the mixed BF16 paths' for comparison, the roofline of the
-dominant kernel (forces: its FP32 vector work against the vector peak, its
-contractions against the matrix-core peak) and of the update kernel from HIP
-events over the timed run (fcc_run_timed; back-to-back figures at its end
-state from fcc_time_kernels beside them), the list-build stages, the
-FP64 CPU baseline (the oracle, OpenMP over bodies) on a bounded sample of the
-same workload, and the batched-ensemble workload (64 replicas of uneven size
-in one launch per kernel and step) next to the same replicas run as
-independent simulations. The timed run and the kernel timing are roctx
-ranges ("timed", "kernel_timing") for rocprofv3 --marker-trace. Fields read from committed profiles name their
-file (`traffic_source`, `issue_source`).
+plain FP32 and the mixed BF16 paths' and the same workload with a forces and
+an update kernel per step for comparison, the roofline of the dominant kernel
+of the timed run (the one-kernel step: its FP32 vector work against the
+vector peak, its contractions against the matrix-core peak, its compulsory
+bytes against HBM) from HIP events over the timed run (fcc_run_timed), the
+forces part as its own kernel (back-to-back at the end state,
+fcc_time_kernels) and the update kernel of a two-kernel step, the list-build
+stages, the FP64 CPU baseline (the oracle, OpenMP over bodies) on a bounded
+sample of the same workload, and the batched-ensemble workload (64 replicas
+of uneven size in one launch per kernel and step) next to the same replicas
+run as independent simulations. The timed run and the kernel timing are
+roctx ranges ("timed", "kernel_timing") for rocprofv3 --marker-trace. Fields
+read from committed profiles name their file (`traffic_source`,
+`issue_source`).
 """
 import argparse
 import json
@@ -60,11 +63,11 @@ def marker(name=None):
         pass
 
 
-def make_sim(n, precision, seed=1234):
+def make_sim(n, precision, seed=1234, two_kernels=False):
     b, box = F.hexagonal_monolayer(n, n, packing=1.0, maxmode=8, seed=seed)
     return F.Simulation(b, box, types=(F.BodyType(v0=0.1, Dr=0.01, muab=0.1),),
                         energies=(F.Tension(0.01), F.Overlap(1.0)), dt=0.005, ntheta=40,
-                        precision=precision, seed=seed, skin=0.3)
+                        precision=precision, seed=seed, skin=0.3, two_kernels=two_kernels)
 
 
 def polydisperse_sim(precision, n=256, seed=7):
@@ -295,6 +298,16 @@ def contact_pairs(sim):
     return tested, inr
 
 
+# Compulsory HBM bytes per body of a one-kernel step (k_step_mfma /
+# k_step_fast2, FP32 state): reads x, y 16, θ 4, lab 60, R0 4, type 4, rmax,
+# rmin 8, Verlet reference x0, y0, rmax0 20, neighbour count and 6 list
+# entries 28; writes the new x, y 16, θ 4, lab 60, rmax, rmin 8 and the
+# gradients glab 60, gx, gy, gθ 12, Ebody 4. Neighbour gathers are L2 reads.
+STEP_BYTES = 148 + 164
+# The update kernel of a two-kernel step: 188 B read + 92 B written per body.
+UPDATE_BYTES = 280
+
+
 def forces_flops(sim, tested, inr):
     """Algorithmic FLOPs of one forces launch as (contractions, vector work):
     contour contractions 2·[2·(3nθ)·ncoef] per body (synthesis + pullback,
@@ -421,6 +434,8 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=2000)
     ap.add_argument("--ensemble-only", action="store_true",
                     help="only the ensemble leg (its own JSON line)")
+    ap.add_argument("--no-two-kernels", action="store_true",
+                    help="without the leg that runs the workload with two kernels per step")
     ap.add_argument("--no-independent", action="store_true",
                     help="ensemble leg without the independent-simulations comparison")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -488,19 +503,40 @@ def main():
     plain = None if args.no_plain or args.precision == "fp32" else other_path("fp32")
     bf16 = other_path("mixed_bf16") if args.precision in ("mixed", "mixed_fp16") and not args.no_plain else None
 
+    # the same workload with a forces and an update kernel per step
+    # (FCC_FLAG_TWO_KERNELS): its throughput, and the update kernel's own
+    # duration (eager samples over its timed run)
+    two = None
+    if not args.no_two_kernels:
+        s3 = make_sim(args.n, args.precision, two_kernels=True)
+        F.run(s3, args.warmup)
+        clock_warmup(s3)
+        tsec, tfms, tums = timed_run(s3, args.steps)
+        two = {"value": s3.N * args.steps / tsec, "ms_per_step": tsec / args.steps * 1e3,
+               "forces_ms": tfms, "update_ms": tums}
+        print(f"[bench] {args.precision}, two kernels per step: {two['value'] / 1e6:.1f} M body-steps/s "
+              f"(forces {tfms * 1e3:.1f} us, update {tums * 1e3:.1f} us)", file=sys.stderr, flush=True)
+        s3.close()
+
     mixed = args.precision.startswith("mixed")
     # kernel durations sampled over the timed run: flops of its mean state
     fl_mfma, fl_valu = forces_flops(sim, tested, inr)
     if not mixed:  # the contractions run on the vector units too
         fl_valu, fl_mfma = fl_valu + fl_mfma, 0.0
+    one = run_ums == 0.0  # the timed run's steps were one kernel each
     fkernel = "k_forces_mfma" if mixed else "k_forces_fast2"
+    skernel = ("k_step_mfma" if mixed else "k_step_fast2") if one else fkernel
     t_f = run_fms * 1e-3
     a_valu, a_mfma = fl_valu / t_f / 1e12, fl_mfma / t_f / 1e12
-    valu_frac, mfma_frac = measured_issue(fkernel) if sim.N == 262144 else (None, None)
-    upd_bytes = 280 * sim.N  # per body-step, 188 B read + 92 B written
-    print(f"[bench] forces {run_fms * 1e3:.1f} us, update {run_ums * 1e3:.1f} us (samples over the timed run; "
-          f"back-to-back at its end state: {end_fms * 1e3:.1f} / {end_ums * 1e3:.1f} us), "
-          f"pairs tested {tested}, in range {inr}", file=sys.stderr, flush=True)
+    on_profile = sim.N == 262144
+    valu_frac, mfma_frac = measured_issue(skernel) if on_profile else (None, None)
+    fvalu_frac, fmfma_frac = measured_issue(fkernel) if on_profile else (None, None)
+    step_bytes = STEP_BYTES * sim.N
+    upd_bytes = UPDATE_BYTES * sim.N
+    upd_ms = two["update_ms"] if two else (run_ums if not one else None)
+    print(f"[bench] step kernel {run_fms * 1e3:.1f} us (samples over the timed run); forces kernel alone, "
+          f"back-to-back at its end state: {end_fms * 1e3:.1f} us; pairs tested {tested}, in range {inr}",
+          file=sys.stderr, flush=True)
 
     ensemble = None
     if not args.no_ensemble:
@@ -508,13 +544,15 @@ def main():
                                   independent=not args.no_independent)
 
     cpu = None if args.no_cpu else cpu_baseline(args.n, args.cpu_seconds)
-    on_profile = sim.N == 262144
     timing_note = ("HIP events on the simulation's stream around the last step of each graph chunk of the "
                    "timed run, launched eagerly with the graph's kernels and arguments (fcc_run_timed; roctx "
                    "range 'timed'); flops of the mean of its first and last state")
     end_note = ("captured graph of 50 back-to-back launches at the end state of the timed run, right after it, "
                 "with ~400 untimed launches first (fcc_time_kernels; roctx range 'kernel_timing'); the forces "
                 "series re-evaluates one state, whose data stay in L2 between launches")
+    two_note = ("HIP events around the eager last step of each chunk of the same workload run with a forces and "
+                "an update kernel per step (FCC_FLAG_TWO_KERNELS, fcc_run_timed)")
+    fa_valu, fa_mfma = fl_valu / (end_fms * 1e-3) / 1e12, fl_mfma / (end_fms * 1e-3) / 1e12
 
     line = {
         "metric": "body-steps/sec at N=262144 (mixed vs plain FP32, 1 GPU) @ force err ≤1e-4",
@@ -536,14 +574,18 @@ def main():
         "plain_fp32_value": plain,
         "mixed_over_plain": (value / plain) if plain else None,
         "mixed_bf16_value": bf16,
-        "roofline": {"kernel": fkernel, "bound": "valu", "achieved": a_valu,
+        "two_kernels_value": two["value"] if two else None,
+        "one_over_two_kernels": (value / two["value"]) if two else None,
+        "roofline": {"kernel": skernel,
+                     "what": ("one kernel per step: forces and update of 64 bodies per workgroup" if one
+                              else "forces kernel of a two-kernel step"),
+                     "bound": "valu", "achieved": a_valu,
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": a_valu / PEAK_FP32_TFLOPS,
                      "peak_unpacked": PEAK_FP32_UNPACKED_TFLOPS,
                      "frac_unpacked": a_valu / PEAK_FP32_UNPACKED_TFLOPS,
-                     "traffic": measured_traffic(fkernel) if on_profile else None,
+                     "traffic": measured_traffic(skernel) if on_profile else None,
                      "traffic_source": TRAFFIC_FILE if on_profile else None,
                      "launch_ms": run_fms, "launch_ms_source": timing_note,
-                     "launch_ms_end_state": end_fms, "launch_ms_end_state_source": end_note,
                      "pairs_tested": tested, "pairs_in_range": inr,
                      "flops_per_launch": fl_valu + fl_mfma,
                      "parts": {
@@ -554,17 +596,36 @@ def main():
                                            "algorithmic flops counted once)",
                                    "flops": fl_mfma, "achieved": a_mfma, "peak": PEAK_FP16_MFMA_TFLOPS,
                                    "unit": "TFLOP/s", "frac": a_mfma / PEAK_FP16_MFMA_TFLOPS}
-                                  if mixed else None)},
+                                  if mixed else None),
+                         "hbm": ({"what": f"compulsory state and gradient bytes of the step ({STEP_BYTES} B "
+                                          "per body)", "bytes": step_bytes,
+                                  "achieved": step_bytes / t_f / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                                  "frac": step_bytes / t_f / 1e9 / PEAK_HBM_GBS} if one else None)},
                      "valu_issue_frac": valu_frac, "mfma_busy_frac": mfma_frac,
                      "issue_source": ISSUE_FILE if on_profile else None},
-        "roofline_update": {"kernel": "k_update", "bound": "hbm",
-                            "traffic": measured_traffic("k_update") if on_profile else None,
+        "roofline_forces": {"kernel": fkernel,
+                            "what": "the forces part as its own kernel (fcc_compute_gradients, ensembles, "
+                                    "two-kernel steps), same flops",
+                            "bound": "valu", "achieved": fa_valu, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                            "frac": fa_valu / PEAK_FP32_TFLOPS,
+                            "frac_unpacked": fa_valu / PEAK_FP32_UNPACKED_TFLOPS,
+                            "mfma": ({"achieved": fa_mfma, "peak": PEAK_FP16_MFMA_TFLOPS,
+                                      "frac": fa_mfma / PEAK_FP16_MFMA_TFLOPS} if mixed else None),
+                            "traffic": measured_traffic(fkernel) if on_profile else None,
                             "traffic_source": TRAFFIC_FILE if on_profile else None,
-                            "achieved": upd_bytes / (run_ums * 1e-3) / 1e9, "peak": PEAK_HBM_GBS,
-                            "unit": "GB/s", "frac": upd_bytes / (run_ums * 1e-3) / 1e9 / PEAK_HBM_GBS,
-                            "bytes_per_launch": upd_bytes, "launch_ms": run_ums, "launch_ms_source": timing_note,
-                            "launch_ms_end_state": end_ums,
-                            "launch_ms_end_state_source": end_note + "; update = (steps - forces) / 50"},
+                            "launch_ms": end_fms, "launch_ms_source": end_note,
+                            "valu_issue_frac": fvalu_frac, "mfma_busy_frac": fmfma_frac,
+                            "issue_source": ISSUE_FILE if on_profile else None},
+        "roofline_update": ({"kernel": "k_update", "bound": "hbm",
+                             "what": "update kernel of a two-kernel step (ensembles, parent drag, bonds, "
+                                     "FCC_FLAG_TWO_KERNELS)",
+                             "traffic": measured_traffic("k_update") if on_profile else None,
+                             "traffic_source": TRAFFIC_FILE if on_profile else None,
+                             "achieved": upd_bytes / (upd_ms * 1e-3) / 1e9, "peak": PEAK_HBM_GBS,
+                             "unit": "GB/s", "frac": upd_bytes / (upd_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                             "bytes_per_launch": upd_bytes, "launch_ms": upd_ms,
+                             "launch_ms_source": two_note if two else timing_note}
+                            if upd_ms else None),
         "roofline_binning": binning,
         "cpu_baseline": cpu,
         "ensemble": ensemble,

@@ -262,9 +262,8 @@ struct Sim final : fcc_sim {
     std::vector<void *> allocs;
     // graph mode (ext/FourierContourCellsCUDAExt.jl:52-71)
     // captured steps: gx[q][b] holds 2^b steps from the buffers of parity q
-    // (b < NGRAPH, 2^(NGRAPH-1) = GRAPH_STEPS; set 1 only for one-kernel
-    // steps, gone)
-    static constexpr int NGRAPH = 4;
+    // (b < NGRAPH: 1 to 32 steps; set 1 only for one-kernel steps, gone)
+    static constexpr int NGRAPH = 6;
     hipGraphExec_t gx[2][NGRAPH] = {};
     bool gone = false;
     int gen = 0, gexec_gen = -1;
@@ -980,9 +979,9 @@ struct Sim final : fcc_sim {
     }
 
     // run_steps! in graph mode (ext/FourierContourCellsCUDAExt.jl:65-71,133-180).
-    // Graphs of 1, 2, 4 and GRAPH_STEPS = 8 steps in a row (a graph launch
-    // costs several µs between replays; inside a graph the kernels follow
-    // each other without a gap), so k steps take k / 8 + popcount(k mod 8)
+    // Graphs of 1, 2, 4, ..., 32 steps in a row (a graph launch costs
+    // several µs between replays; inside a graph the kernels follow each
+    // other without a gap), so k steps take k / 32 + popcount(k mod 32)
     // launches.
     // A captured one-kernel step flips the host's buffers as an executed one
     // would; the parity the capture started from is restored.
@@ -1031,15 +1030,15 @@ struct Sim final : fcc_sim {
         return FCC_OK;
     }
 
-    // k graph steps: k / GRAPH_STEPS launches of the longest graph, then one
-    // launch per set bit of the rest.
+    // k graph steps: k / 32 launches of the longest graph, then one launch
+    // per set bit of the rest.
     // With one-kernel steps the host flips its buffers as if every step
     // executes (an odd graph, the one-step one, flips them).
     int replay(int64_t k) {
-        static_assert((1 << (NGRAPH - 1)) == GRAPH_STEPS, "graph sizes");
-        for (int64_t q = 0; q < k / GRAPH_STEPS; q++) HIPCHK(hipGraphLaunch(gx[gone ? par : 0][NGRAPH - 1], stream));
+        constexpr int64_t LONG = 1 << (NGRAPH - 1);
+        for (int64_t q = 0; q < k / LONG; q++) HIPCHK(hipGraphLaunch(gx[gone ? par : 0][NGRAPH - 1], stream));
         for (int b = NGRAPH - 2; b >= 0; b--)
-            if ((k % GRAPH_STEPS) & (1 << b)) {
+            if ((k % LONG) & (1 << b)) {
                 HIPCHK(hipGraphLaunch(gx[gone ? par : 0][b], stream));
                 if (gone && b == 0) flip();
             }
@@ -1104,17 +1103,25 @@ struct Sim final : fcc_sim {
             // the same kernels and arguments as the graph
             if ((rc = replay(k - 1))) return rc;
             p.skin_check = (has_contact && skin > 0.0) ? 1 : 0;
-            if ((rc = launch_step(tev[slot][0], tev[slot][1], tev[slot][2], tev[slot][3]))) return rc;
+            // (a one-kernel step has no update kernel to time)
+            if ((rc = launch_step(tev[slot][0], tev[slot][1], gone ? nullptr : tev[slot][2],
+                                  gone ? nullptr : tev[slot][3])))
+                return rc;
         }
         HIPCHK(hipMemcpyAsync(snap + slot, d.st, sizeof(DevStatus), hipMemcpyDeviceToHost, stream));
         HIPCHK(hipEventRecord(sev[slot], stream));
         return FCC_OK;
     }
 
-    // Steps per chunk from the rebuild-interval estimate.
+    // Steps per chunk: half the rebuild-interval estimate, 16 to 256. With
+    // two chunks queued, a chunk boundary costs the device a status snapshot
+    // and a graph launch, while a step frozen by a raised rebuild flag costs
+    // a launch whose workgroups return at once; the chunk length does not
+    // change results. (The reference, which waits for every chunk, takes a
+    // quarter of the estimate, from 4 steps: ext/FourierContourCellsCUDAExt.jl:65-71.)
     int64_t chunk_len() const {
         if (!has_contact) return 256;
-        return std::min<int64_t>(256, std::max<int64_t>(4, (int64_t)std::llround(interval / 4)));
+        return std::min<int64_t>(256, std::max<int64_t>(16, (int64_t)std::llround(interval / 2)));
     }
 
     // run! in graph mode, pipelined: the host keeps up to NQ chunks queued
@@ -1145,6 +1152,7 @@ struct Sim final : fcc_sim {
         // parity of the buffers holding the state the device has reached
         // (the host's buffers run ahead by the queued one-kernel steps)
         int pact = par;
+        bool trip_left = false;  // the last snapshot: a one-kernel step's trip not yet in the flag
         auto account = [&](int64_t executed) {
             for (int64_t q = 0; q < executed; q++) t += cfg.dt;
             nstep += executed;
@@ -1200,10 +1208,11 @@ struct Sim final : fcc_sim {
             if (gone) pact ^= (int)(executed & 1);
             // a rebuild is due: the flag, or a skin trip of the last one-kernel step
             const bool due = s.rebuild != 0 || (gone && s.tstep[pact] == s.clock - 1);
+            trip_left = gone && s.rebuild == 0 && s.tstep[pact] == s.clock - 1;
             if (want_timed && executed == kq[slot]) {  // the chunk's last step ran (not frozen)
                 float a = 0.f, b = 0.f;
                 HIPCHK(hipEventElapsedTime(&a, tev[slot][0], tev[slot][1]));
-                HIPCHK(hipEventElapsedTime(&b, tev[slot][2], tev[slot][3]));
+                if (!gone) HIPCHK(hipEventElapsedTime(&b, tev[slot][2], tev[slot][3]));
                 acc_f += a;
                 acc_u += b;
                 acc_n++;
@@ -1220,7 +1229,7 @@ struct Sim final : fcc_sim {
             }
         }
         set_parity(pact);
-        if (gone) return sync_slots();  // a trip of the last step becomes the rebuild flag
+        if (trip_left) return sync_slots();  // a trip of the last step becomes the rebuild flag
         return FCC_OK;
     }
 
@@ -1585,12 +1594,12 @@ struct Sim final : fcc_sim {
             hipError_t e = hipSuccess;
             const bool one = one_kernel();
             if (one && ((rc = ensure_next()) || (rc = sync_slots()))) break;
-            rc = launch_step(e0, e1, e2, e3);
+            rc = launch_step(e0, e1, one ? nullptr : e2, one ? nullptr : e3);
             if (!rc && one) rc = sync_slots();
-            if (!rc) e = hipEventSynchronize(e3);
+            if (!rc) e = hipEventSynchronize(one ? e1 : e3);
             float a = 0.f, b = 0.f;
             if (!rc && e == hipSuccess) e = hipEventElapsedTime(&a, e0, e1);
-            if (!rc && e == hipSuccess) e = hipEventElapsedTime(&b, e2, e3);
+            if (!rc && !one && e == hipSuccess) e = hipEventElapsedTime(&b, e2, e3);
             if (!rc && e != hipSuccess) rc = fail(FCC_ERR_DEVICE, "event timing: %s", hipGetErrorString(e));
             sf += a;
             su += b;

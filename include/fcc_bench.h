@@ -17,14 +17,17 @@ extern "C" {
 /* Run `steps` eager steps (advancing the simulation like fcc_step) and
  * return the average device time of the forces kernel and of the update
  * kernel(s) of a step, in milliseconds, from hipEventElapsedTime. Steps that
- * rebuild the neighbour list are included; the rebuild itself is not timed. */
+ * rebuild the neighbour list are included; the rebuild itself is not timed.
+ * Where a step runs as one kernel (forces and update, see
+ * FCC_FLAG_TWO_KERNELS), forces_ms is that kernel's time and update_ms 0. */
 int fcc_time_step_kernels(fcc_sim *sim, int64_t steps, double *forces_ms,
                           double *update_ms);
 
 /* fcc_run (graph mode) where the last step of every chunk is launched eagerly
  * (same kernels and arguments as the captured step) between HIP events on the
  * simulation's stream; returns the average forces and update kernel
- * durations over those steps, in milliseconds. Results equal fcc_run's. */
+ * durations over those steps, in milliseconds (one-kernel steps: the step
+ * kernel's time and 0). Results equal fcc_run's. */
 int fcc_run_timed(fcc_sim *sim, int64_t n, double *forces_ms, double *update_ms);
 
 /* Average duration of the forces kernel and of the update kernel(s) of a
@@ -33,8 +36,9 @@ int fcc_run_timed(fcc_sim *sim, int64_t n, double *forces_ms, double *update_ms)
  * series is first replayed untimed, about 400 launches in all, for the clock
  * ramp of an idle GPU):
  * `reps` back-to-back forces launches (evaluating the current state), then
- * `reps` steps (forces + update); update = (steps - forces) / reps. The state is saved before and restored afterwards: the
- * simulation is left unchanged. */
+ * `reps` steps (forces + update, or one-kernel steps where they apply);
+ * update = (steps - forces) / reps. The state is saved before and restored
+ * afterwards: the simulation is left unchanged. */
 int fcc_time_kernels(fcc_sim *sim, int64_t reps, double *forces_ms, double *update_ms);
 
 /* `reps` forced rebuilds of the cell and neighbour lists of the current state
