@@ -997,6 +997,7 @@ struct Sim final : fcc_sim {
         HIPCHK(e);
         HIPCHK(hipGraphInstantiate(out, graph, nullptr, nullptr, 0));
         HIPCHK(hipGraphDestroy(graph));
+        HIPCHK(hipGraphUpload(*out, stream));  // its first replay in a run starts at once
         return FCC_OK;
     }
 

@@ -363,6 +363,7 @@ struct Ensemble final : fcc_ensemble {
         HIPCHK(e);
         HIPCHK(hipGraphInstantiate(out, graph, nullptr, nullptr, 0));
         HIPCHK(hipGraphDestroy(graph));
+        HIPCHK(hipGraphUpload(*out, stream));  // its first replay in a run starts at once
         return FCC_OK;
     }
 
