@@ -268,16 +268,36 @@ __device__ __forceinline__ void harmonics(T th, int K, T (&cn)[KM + 1], T (&sn)[
 }
 
 // enforce_shape! (src/dynamics.jl:44-64) on a register row, in U precision.
+// With U = double and T = float, the bound test first takes the sum of the
+// mode amplitudes in FP32: each term sqrt(a² + b²) is within 3 ulps of the
+// exact one (two squares and their sum, halved by the root, and the
+// hardware root's own ≤ 1 ulp; a subnormal argument may flush, an absolute
+// error below 2^-63), and summing 15 or fewer non-negative terms adds at
+// most 14 roundings, so the FP32 sum is within 17·2^-24 < 2^-19 (relative)
+// of the exact one, and the FP64 sum within 2^-50 of it. An FP32 sum below
+// bound·(1 − 2^-16) thus decides "no rescale" as the FP64 sum would, and the
+// FP64 sum (seven FP64 square roots at K = 7) is taken only near or above
+// the bound.
 template <typename U, typename T, int KM>
 __device__ __forceinline__ void enforce_shape(T (&c)[2 * KM + 1], int K) {
 #pragma clang fp contract(off)
-    U sm = U(0);
-#pragma unroll
-    for (int m = 1; m <= KM; m++)
-        if (m <= K) sm += sqrt((U)c[m] * (U)c[m] + (U)c[KM + m] * (U)c[KM + m]);
     const U bound = U(0.816496580927726);
+    bool near = true;
+    if constexpr (sizeof(U) == 8 && sizeof(T) == 4 && KM <= 15) {
+        float st = 0.f;
+#pragma unroll
+        for (int m = 1; m <= KM; m++)
+            if (m <= K) st += __builtin_amdgcn_sqrtf(c[m] * c[m] + c[KM + m] * c[KM + m]);
+        near = !(st < (float)(0.816496580927726 * (1.0 - 1.0 / 65536.0)));
+    }
+    U sm = U(0);
+    if (near) {
+#pragma unroll
+        for (int m = 1; m <= KM; m++)
+            if (m <= K) sm += sqrt((U)c[m] * (U)c[m] + (U)c[KM + m] * (U)c[KM + m]);
+    }
     U f = U(1);
-    const bool scale = sm > bound;
+    const bool scale = near && sm > bound;
     if (scale) f = bound / sm;
     U q = U(0);
 #pragma unroll
