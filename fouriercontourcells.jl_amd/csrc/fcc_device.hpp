@@ -28,7 +28,8 @@ struct DevStatus {
     int32_t rebuild, update, overflow, stop;
     int64_t clock;
     int64_t target;
-    int32_t builds, pad;
+    int32_t builds;
+    int32_t cur;  // ensembles of one-kernel steps: parity of the current state's buffers
     // One-kernel steps, by the parity q of the buffers a step reads: the
     // step's clock (cslot[q], passed on by the step before), and the clock
     // of the last step before it whose bodies tripped the skin check

@@ -15,6 +15,14 @@
 // flag until the host rebuilds its list between graph chunks, and a
 // per-replica step budget freezes replicas that have finished, so every
 // replica executes exactly the kernels fcc_run would give it alone.
+//
+// Where every replica would run one kernel per step alone (two lanes per
+// body, FP32 or mixed without Bending, no parent drag, bonds or debug
+// buffers), an ensemble step is k_gate and one step kernel over tiles of 64
+// bodies: k_gate also flips the replica's buffer parity (`cur`) for a step
+// that executes, and the step kernel reads the other parity's positions, lab
+// rows and radius bounds and writes `cur`'s. Between runs every replica's
+// current buffers are its first ones and `cur` is 0 (normalize()).
 
 struct fcc_ensemble {
     virtual ~fcc_ensemble() {}
@@ -34,6 +42,7 @@ static int check_launch(const fcc_launch *lc) {
     if (lc->xcd_order != 0 && lc->xcd_order != 1) return fail(FCC_ERR_ARGUMENT, "xcd_order must be 0 or 1");
     if (lc->chunk < 0) return fail(FCC_ERR_ARGUMENT, "chunk must be non-negative");
     if (lc->lanes != 0 && lc->lanes != 2) return fail(FCC_ERR_ARGUMENT, "lanes must be 0 or 2");
+    if (lc->kernels != 0 && lc->kernels != 2) return fail(FCC_ERR_ARGUMENT, "kernels must be 0 or 2");
     return FCC_OK;
 }
 
@@ -74,9 +83,10 @@ struct Ensemble final : fcc_ensemble {
     std::vector<Params<T>> hP, upP;
     std::vector<Ptrs<T>> hD, upD;
     bool uploaded = false;
-    fcc_launch lc{256, 1, 0, 0};
-    fcc_launch tiles_lc{0, 0, 0, 0};
+    fcc_launch lc{256, 1, 0, 0, 0};
+    fcc_launch tiles_lc{0, 0, 0, 0, 0};
     int var = -1, tiles_var = -1, ncls = 1;  // forces kernel: 3 wide, 2 MFMA, 1 fast, 0 generic
+    bool one = false, tiles_one = false;     // one kernel per step (k_step_*_ens)
     bool bonds = false;                      // a replica has bond terms on bonds: bond kernel after forces
     hipGraphExec_t gexec = nullptr, gexec_n = nullptr;  // one step; GRAPH_STEPS steps
     bool graph_ok = false;
@@ -92,6 +102,7 @@ struct Ensemble final : fcc_ensemble {
     DevStatus *snap = nullptr;
     hipEvent_t sev[NB] = {};
     hipEvent_t tev[NB][4] = {};
+    bool sone[NB] = {};  // the chunk in the slot was issued as one-kernel steps (no update events)
 
     ~Ensemble() override {
         if (stream) (void)hipStreamSynchronize(stream);
@@ -148,9 +159,72 @@ struct Ensemble final : fcc_ensemble {
 
     int launch_gate() {
         const int R = (int)m.size();
-        hipLaunchKernelGGL(k_gate, blocks(R, 64), 64, 0, stream, st, R);
+        hipLaunchKernelGGL(k_gate, blocks(R, 64), 64, 0, stream, st, R, one ? 1 : 0);
         HIPCHK(hipGetLastError());
         return FCC_OK;
+    }
+
+    // Whether every replica would run one kernel per step alone (var: the
+    // forces kernel variant of prepare()).
+    bool one_kernel(int v) const {
+        if constexpr (sizeof(T) != 4) return false;
+        const Sim<T> &a = *m[0];
+        if ((v != 1 && v != 2) || lc.kernels == 2) return false;
+        if (a.nth != 40 || a.has_pdrag || a.debug || (a.cfg.flags & FCC_FLAG_TWO_KERNELS)) return false;
+        if (v == 2 && a.mfma_T != 3) return false;
+        for (const Sim<T> *s : m)
+            if (s->N < s->four_below || !s->has_contact || s->skin <= 0.0 || s->has_bonds) return false;
+        return true;
+    }
+
+    // The step kernel of a one-kernel ensemble step (tiles of 64 bodies).
+    int launch_step_kernel(hipEvent_t eb = nullptr, hipEvent_t ee = nullptr) {
+        if constexpr (sizeof(T) == 4) {
+            const Sim<T> &a = *m[0];
+            if (var == 2) {
+                const bool bf = a.cfg.precision == FCC_MIXED_BF16;
+#define FCC_SME(KV, NCV)                                                                                      \
+    if (bf) FCC_LAUNCH((k_step_mfma_ens<KV, 3, NCV, true>), gf, 128, 0, stream, eb, ee, dP, dD, tf, a.asyn, a.bpull); \
+    else FCC_LAUNCH((k_step_mfma_ens<KV, 3, NCV, false>), gf, 128, 0, stream, eb, ee, dP, dD, tf, a.asyn, a.bpull);
+                if (a.K == 5) {
+                    if (ncls <= 1) { FCC_SME(5, 1) } else { FCC_SME(5, 2) }
+                } else {
+                    if (ncls <= 1) { FCC_SME(7, 1) } else { FCC_SME(7, 2) }
+                }
+#undef FCC_SME
+            } else {
+#define FCC_SFE(KV, NCV) FCC_LAUNCH((k_step_fast2_ens<T, KV, NCV>), gf, 128, 0, stream, eb, ee, dP, dD, tf);
+                if (a.K == 5) {
+                    if (ncls <= 1) { FCC_SFE(5, 1) } else { FCC_SFE(5, 2) }
+                } else {
+                    if (ncls <= 1) { FCC_SFE(7, 1) } else { FCC_SFE(7, 2) }
+                }
+#undef FCC_SFE
+            }
+            HIPCHK(hipGetLastError());
+            return FCC_OK;
+        }
+        return fail(FCC_ERR_STATE, "one-kernel ensemble step in double precision");
+    }
+
+    // Every replica's current buffers named as its first ones, parity 0 on
+    // the device, and the tables uploaded to match (the device is idle).
+    int normalize() {
+        if (!one && !tiles_one) return FCC_OK;
+        std::vector<DevStatus> hs;
+        int rc = read_status(hs);
+        if (rc) return rc;
+        bool any = false;
+        for (size_t r = 0; r < m.size(); r++)
+            if (hs[r].cur != 0) {
+                m[r]->flip();
+                any = true;
+            }
+        if (!any) return FCC_OK;
+        const int R = (int)m.size();
+        hipLaunchKernelGGL(k_clear_parity, blocks(R, 64), 64, 0, stream, st, R);
+        HIPCHK(hipGetLastError());
+        return prepare();
     }
 
     // Forces of one ensemble step, then the bond kernel over the update tiles
@@ -282,6 +356,12 @@ struct Ensemble final : fcc_ensemble {
         ncls = nc;
         bonds = bd;
         const size_t R = m.size();
+        const bool o = one_kernel(v);
+        if (o != one) graph_ok = false;
+        one = o;
+        if (one)
+            for (Sim<T> *s : m)
+                if (int rc = s->ensure_next()) return rc;
         for (size_t r = 0; r < R; r++) {
             Sim<T> &s = *m[r];
             s.p.skin_check = (s.has_contact && s.skin > 0.0) ? 1 : 0;
@@ -296,8 +376,8 @@ struct Ensemble final : fcc_ensemble {
             HIPCHK(hipMemcpyAsync(dD, upD.data(), sizeof(Ptrs<T>) * R, hipMemcpyHostToDevice, stream));
             uploaded = true;
         }
-        if (var != tiles_var || lc.update_block != tiles_lc.update_block || lc.xcd_order != tiles_lc.xcd_order ||
-            lc.lanes != tiles_lc.lanes) {
+        if (var != tiles_var || one != tiles_one || lc.update_block != tiles_lc.update_block ||
+            lc.xcd_order != tiles_lc.xcd_order || lc.lanes != tiles_lc.lanes) {
             std::vector<int64_t> sizes(R), small(R, 0), large(R, 0);
             for (size_t r = 0; r < R; r++) sizes[r] = m[r]->N;
             // Bodies per forces workgroup: 16 for the four-lane kernels (replicas
@@ -306,7 +386,7 @@ struct Ensemble final : fcc_ensemble {
             // contours of more than 64 nodes, 64 otherwise.
             const bool fast40 = (var == 1 && m[0]->nth == 40) || var == 2;
             for (size_t r = 0; r < R; r++) (fast40 && sizes[r] < m[r]->four_below ? small : large)[r] = sizes[r];
-            const int tile = var == 3 ? 1 : (var == 2 || fast40 ? 32 : 64);
+            const int tile = one ? 64 : (var == 3 ? 1 : (var == 2 || fast40 ? 32 : 64));
             std::vector<Tile> a = tile_table(small, 16, lc.xcd_order != 0);
             const size_t n4 = a.size();
             const std::vector<Tile> a2 = tile_table(large, tile, lc.xcd_order != 0);
@@ -331,6 +411,7 @@ struct Ensemble final : fcc_ensemble {
             gf = (unsigned)(a.size() - n4);
             gu = (unsigned)b.size();
             tiles_var = var;
+            tiles_one = one;
             tiles_lc = lc;
             graph_ok = false;
         }
@@ -355,6 +436,10 @@ struct Ensemble final : fcc_ensemble {
         int rc = FCC_OK;
         for (int q = 0; q < steps && !rc; q++) {
             rc = launch_gate();
+            if (one) {
+                if (!rc) rc = launch_step_kernel();
+                continue;
+            }
             if (!rc) rc = launch_forces();
             if (!rc) rc = launch_update();
         }
@@ -433,8 +518,13 @@ struct Ensemble final : fcc_ensemble {
             if (rc0) return rc0;
             int rc = launch_gate();
             if (rc) return rc;
-            if ((rc = launch_forces(tev[slot][0], tev[slot][1]))) return rc;
-            if ((rc = launch_update(tev[slot][2], tev[slot][3]))) return rc;
+            sone[slot] = one;
+            if (one) {
+                if ((rc = launch_step_kernel(tev[slot][0], tev[slot][1]))) return rc;
+            } else {
+                if ((rc = launch_forces(tev[slot][0], tev[slot][1]))) return rc;
+                if ((rc = launch_update(tev[slot][2], tev[slot][3]))) return rc;
+            }
         }
         HIPCHK(hipMemcpyAsync(snap + (size_t)slot * R, st, sizeof(DevStatus) * R, hipMemcpyDeviceToHost, stream));
         HIPCHK(hipEventRecord(sev[slot], stream));
@@ -479,7 +569,7 @@ struct Ensemble final : fcc_ensemble {
             if (want_timed) {
                 float a = 0.f, b = 0.f;
                 HIPCHK(hipEventElapsedTime(&a, tev[slot][0], tev[slot][1]));
-                HIPCHK(hipEventElapsedTime(&b, tev[slot][2], tev[slot][3]));
+                if (!sone[slot]) HIPCHK(hipEventElapsedTime(&b, tev[slot][2], tev[slot][3]));
                 acc_f += a;
                 acc_u += b;
                 acc_n++;
@@ -507,7 +597,11 @@ struct Ensemble final : fcc_ensemble {
                     if (s[r].overflow > sim.d.cap || (fast_var() && s[r].overflow >= NLIM)) special.push_back(r);
                     continue;
                 }
-                if ((rc = sim.rebuild_async(side, fast_var() ? NLIM : INT32_MAX))) return rc;
+                // the replica's current buffers: parity s[r].cur (one-kernel steps)
+                if (s[r].cur) sim.flip();
+                rc = sim.rebuild_async(side, fast_var() ? NLIM : INT32_MAX);
+                if (s[r].cur) sim.flip();
+                if (rc) return rc;
                 issued_builds[r]++;
                 trip[r] = clock[r];
                 interval[r] = (interval[r] + (double)since[r]) / 2;
@@ -516,6 +610,7 @@ struct Ensemble final : fcc_ensemble {
             if (!special.empty()) {
                 HIPCHK(hipStreamSynchronize(stream));
                 HIPCHK(hipStreamSynchronize(side));
+                if ((rc = normalize())) return rc;  // the host's rebuilds use the members' current buffers
                 for (size_t r : special) {
                     Sim<T> &sim = *m[r];
                     if (s[r].overflow > sim.d.cap) {
@@ -544,7 +639,7 @@ struct Ensemble final : fcc_ensemble {
         if (rc) return rc;
         if (rc2) return rc2;
         HIPCHK(e);
-        return FCC_OK;
+        return normalize();
     }
 
     int run_timed(int64_t n, double *fms, double *ums) override {
@@ -658,11 +753,16 @@ struct Ensemble final : fcc_ensemble {
         const int ubs[3] = {64, 128, 256};
         const int nl = lanes_matter() ? 2 : 1;
         for (int li = 0; li < nl && !rc; li++)
+        for (int ki = 0; ki < 2 && !rc; ki++)
         for (int xo = 0; xo < 2 && !rc; xo++) {
             for (int u = 0; u < 3 && !rc; u++) {
-                lc = fcc_launch{ubs[u], xo, keep.chunk, nl == 1 ? keep.lanes : 2 * li};
+                lc = fcc_launch{ubs[u], xo, keep.chunk, nl == 1 ? keep.lanes : 2 * li, 2 * ki};
                 apply_lanes();
-                if ((rc = prepare()) || (rc = copy_regions(regions, snap, false))) break;
+                if ((rc = prepare())) break;
+                // the same candidate as another: one kernel per step (no
+                // update workgroup size), or kernels = 0 where it does not apply
+                if ((one && u > 0) || (ki == 0 && !one)) continue;
+                if ((rc = copy_regions(regions, snap, false))) break;
                 hipError_t e = hipSuccess;
                 for (int q = 0; q < 2 && e == hipSuccess; q++) e = hipGraphLaunch(gexec, stream);
                 if (e == hipSuccess) e = hipEventRecord(e0, stream);

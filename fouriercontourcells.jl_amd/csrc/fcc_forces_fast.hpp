@@ -461,7 +461,12 @@ __device__ __forceinline__ void k_forces_fast2_impl(const PP &p, const DP &d, in
     const int64_t i = (int64_t)blk * NB + b;
     int64_t clk = 0;  // one-kernel step: its clock
     if constexpr (NW == 2) {
-        if (step_open(d.st, q, use_status == 4, blk, lane, clk)) return;  // uniform over the grid
+        if (use_status == 2) {  // ensemble step: k_gate decided and ticked the clock
+            if (d.st->update != 0) return;
+            clk = d.st->clock - 1;
+        } else if (step_open(d.st, q, use_status == 4, blk, lane, clk)) {
+            return;  // uniform over the grid
+        }
     } else {
         if (forces_frozen(use_status, d.st, blk, lane)) return;  // uniform over the grid
     }
@@ -742,7 +747,7 @@ __device__ __forceinline__ void k_forces_fast2_impl(const PP &p, const DP &d, in
         }
         __syncthreads();
         if (w != 0) return;
-        fused_update<T, K, false>(p, d, &sRw[0][0][0], NTH * NB, NB, wg * 64, lane, q, clk);
+        fused_update<T, K, false>(p, d, &sRw[0][0][0], NTH * NB, NB, wg * 64, lane, q, clk, use_status == 2);
     }
 }
 
@@ -757,6 +762,17 @@ __global__ __launch_bounds__(64) void k_forces_fast2(const Params<T> p, const Pt
 template <typename T, int K, int NCLS>
 __global__ __launch_bounds__(128) void k_step_fast2(const Params<T> p, const Ptrs<T> d, int use_status, int q) {
     k_forces_fast2_impl<T, K, NCLS, 2, 2>(p, d, use_status, 0, xcd_tile(blockIdx.x, gridDim.x), q);
+}
+
+// Ensemble form of the one-kernel step: tiles of 64 bodies, each replica's
+// buffers by the parity k_gate left in its status (step_buffers).
+template <typename T, int K, int NCLS>
+__global__ __launch_bounds__(128) void k_step_fast2_ens(const Params<T> *ps, const Ptrs<T> *ds, const Tile *tiles) {
+    const Tile tl = load_uniform(tiles, blockIdx.x);
+    if (tl.rep < 0) return;
+    const FCC_CONST Params<T> &p = as_const(ps)[tl.rep];
+    const Ptrs<T> d = step_buffers(load_uniform(ds, tl.rep));
+    k_forces_fast2_impl<T, K, NCLS, 2, 2>(p, d, 2, 0, tl.blk);
 }
 
 template <typename T, int K, int NCLS, int LPB>

@@ -146,7 +146,12 @@ __device__ __forceinline__ void k_forces_mfma_impl(const PP &p, const DP &d,
     const int64_t i = (int64_t)blk * 32 + b;
     int64_t clk = 0;  // one-kernel step: its clock
     if constexpr (NW == 2) {
-        if (step_open(d.st, q, use_status == 4, blk, lane, clk)) return;  // uniform over the grid
+        if (use_status == 2) {  // ensemble step: k_gate decided and ticked the clock
+            if (d.st->update != 0) return;
+            clk = d.st->clock - 1;
+        } else if (step_open(d.st, q, use_status == 4, blk, lane, clk)) {
+            return;  // uniform over the grid
+        }
     } else {
         if (forces_frozen(use_status, d.st, blk, lane)) return;  // uniform over the grid
     }
@@ -459,7 +464,7 @@ __device__ __forceinline__ void k_forces_mfma_impl(const PP &p, const DP &d,
         }
         __syncthreads();
         if (w != 0) return;
-        fused_update<float, K, true>(p, d, &sRw[0][0][0], NTH_M * 32, 32, wg * 64, lane, q, clk);
+        fused_update<float, K, true>(p, d, &sRw[0][0][0], NTH_M * 32, 32, wg * 64, lane, q, clk, use_status == 2);
     }
 }
 
@@ -477,6 +482,19 @@ __global__ __launch_bounds__(128) void k_step_mfma(const Params<float> p, const 
                                                    const uint16_t *asyn, const uint16_t *bpull, int use_status,
                                                    int q) {
     k_forces_mfma_impl<K, T, NCLS, BF, 2>(p, d, asyn, bpull, use_status, 0, xcd_tile(blockIdx.x, gridDim.x), q);
+}
+
+// Ensemble form of the one-kernel step: tiles of 64 bodies, each replica's
+// buffers by the parity k_gate left in its status (step_buffers).
+template <int K, int T, int NCLS, bool BF>
+__global__ __launch_bounds__(128) void k_step_mfma_ens(const Params<float> *ps, const Ptrs<float> *ds,
+                                                       const Tile *tiles, const uint16_t *asyn,
+                                                       const uint16_t *bpull) {
+    const Tile tl = load_uniform(tiles, blockIdx.x);
+    if (tl.rep < 0) return;
+    const FCC_CONST Params<float> &p = as_const(ps)[tl.rep];
+    const Ptrs<float> d = step_buffers(load_uniform(ds, tl.rep));
+    k_forces_mfma_impl<K, T, NCLS, BF, 2>(p, d, asyn, bpull, 2, 0, tl.blk);
 }
 
 template <int K, int T, int NCLS, bool BF>

@@ -822,11 +822,15 @@ __global__ __launch_bounds__(256) void k_update_ens(const Params<T> *ps, const P
 // buffers (d.xn, ...), since other workgroups may still read this step's
 // state as neighbours; θ and the image counters stay in place (only the body
 // itself reads them). The draw index is the step's clock. A skin trip marks
-// tstep[1 - q] with the clock, which freezes the next step. The rows are LDS
-// written before the workgroup barrier that precedes this call.
+// tstep[1 - q] with the clock, which freezes the next step; in an ensemble
+// step (`gated`: k_gate took the freeze decision into `update` and ticked
+// the clock before this launch) it raises the rebuild flag, which no
+// workgroup of the launch reads, and body 0 takes the step budget, as in
+// k_update. The rows are LDS written before the workgroup barrier that
+// precedes this call.
 template <typename T, int KM, bool F64S, typename PP, typename DP>
 __device__ __forceinline__ void fused_update(const PP &p, const DP &d, const T *rows, int stride, int nb,
-                                             int64_t body0, int lane, int q, int64_t clk) {
+                                             int64_t body0, int lane, int q, int64_t clk, bool gated) {
     constexpr int NCM = 2 * KM + 1;
     constexpr int W = row_width<NCM, T>();
     const int64_t i = body0 + lane;
@@ -856,9 +860,10 @@ __device__ __forceinline__ void fused_update(const PP &p, const DP &d, const T *
         dn.lab = d.labn;
         dn.rmax = d.rmaxn;
         dn.rmin = d.rminn;
-        tripped = finish_body<T, KM, true, F64S>(p, dn, i, KM, 0, 0, (uint64_t)clk, b, nullptr, false);
+        tripped = finish_body<T, KM, true, F64S>(p, dn, i, KM, 0, 0, (uint64_t)clk, b,
+                                                  gated ? &d.st->rebuild : nullptr, gated);
     }
-    if (__ballot(tripped) != 0 && lane == 0)
+    if (!gated && __ballot(tripped) != 0 && lane == 0)
         __hip_atomic_store(&d.st->tstep[q ^ 1], clk, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
@@ -1128,13 +1133,45 @@ __global__ void k_set_target(DevStatus *st, int64_t target) {
 // Step opening of an ensemble: the freeze decision of every replica (its
 // rebuild flag read with acquire order, so that lists completed by a build on
 // another stream are visible to the kernels that follow) and the clock tick.
-__global__ void k_gate(DevStatus *st, int n) {
+__global__ void k_gate(DevStatus *st, int n, int flip) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
     const int32_t rb = __hip_atomic_load(&st[r].rebuild, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
     const int32_t fr = rb | st[r].stop;
     st[r].update = fr;
-    if (fr == 0) st[r].clock += 1;
+    if (fr == 0) {
+        st[r].clock += 1;
+        if (flip) st[r].cur ^= 1;  // one-kernel steps: the step writes the other buffers
+    }
+}
+
+// Buffers of an ensemble's one-kernel step: the device table names a
+// replica's parity-0 buffers x, y, lab, rmax, rmin and its parity-1 ones xn,
+// ...; k_gate has set `cur` to the parity the step writes, so the step reads
+// the other one.
+template <typename DP>
+__device__ __forceinline__ DP step_buffers(const DP &t) {
+    DP d = t;
+    if (t.st->cur == 0) {
+        d.x = t.xn;
+        d.xn = t.x;
+        d.y = t.yn;
+        d.yn = t.y;
+        d.lab = t.labn;
+        d.labn = t.lab;
+        d.rmax = t.rmaxn;
+        d.rmaxn = t.rmax;
+        d.rmin = t.rminn;
+        d.rminn = t.rmin;
+    }
+    return d;
+}
+
+// After an ensemble run of one-kernel steps the host names each replica's
+// current buffers as its first ones: every parity back to 0.
+__global__ void k_clear_parity(DevStatus *st, int n) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n) st[r].cur = 0;
 }
 
 // Start of a list build on the side stream: the neighbour kernel's maximum.

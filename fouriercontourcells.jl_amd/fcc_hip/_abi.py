@@ -107,6 +107,7 @@ class FccLaunch(ctypes.Structure):
         ("xcd_order", ctypes.c_int32),
         ("chunk", ctypes.c_int32),
         ("lanes", ctypes.c_int32),
+        ("kernels", ctypes.c_int32),
     ]
 
 

@@ -538,10 +538,11 @@ class Ensemble:
     `members` are Simulation handles of the replicas (every function of
     this module applies to them); each replica ends bitwise where `run`
     would take it alone. `tune` runs the launch autotuner once.
+    `two_lanes` and `two_kernels` are the Simulation flags, for every replica.
     """
 
     def __init__(self, replicas, ntheta=40, precision="fp32", debug=False, tune=False, tune_steps=20,
-                 engine=None, two_lanes=False):
+                 engine=None, two_lanes=False, two_kernels=False):
         self.engine = engine if engine is not None else _abi.hip_engine()
         replicas = list(replicas)
         if not replicas:
@@ -553,7 +554,7 @@ class Ensemble:
         for r, rp in enumerate(replicas):
             sim = Simulation.__new__(Simulation)
             sim._setup(self.engine, rp.bodies, rp.box, rp.types, rp.energies, rp.dt, ntheta, precision,
-                       rp.seed, rp.skin, debug, two_lanes)
+                       rp.seed, rp.skin, debug, two_lanes, two_kernels)
             cfgs[r] = sim._cfg
             structs[r] = sim._bodies.struct()
             self.members.append(sim)
@@ -604,12 +605,12 @@ class Ensemble:
         lc = _abi.FccLaunch()
         _check(self.engine, self.engine.ensemble_get_launch(self._h, ctypes.byref(lc)))
         return {"update_block": lc.update_block, "xcd_order": lc.xcd_order, "chunk": lc.chunk,
-                "lanes": lc.lanes}
+                "lanes": lc.lanes, "kernels": lc.kernels}
 
     @launch.setter
     def launch(self, v):
         lc = _abi.FccLaunch(int(v.get("update_block", 256)), int(v.get("xcd_order", 1)),
-                            int(v.get("chunk", 0)), int(v.get("lanes", 0)))
+                            int(v.get("chunk", 0)), int(v.get("lanes", 0)), int(v.get("kernels", 0)))
         _check(self.engine, self.engine.ensemble_set_launch(self._h, ctypes.byref(lc)))
 
     def observer_start(self, nbins=256, prange=(3.5, 5.5), max_samples=1024):

@@ -430,7 +430,7 @@ end
 # and step (README.md:249-261 runs them as concurrent tasks instead)
 
 struct FccLaunch
-    update_block::Int32; xcd_order::Int32; chunk::Int32; lanes::Int32
+    update_block::Int32; xcd_order::Int32; chunk::Int32; lanes::Int32; kernels::Int32
 end
 
 mutable struct HIPEnsembleHandle
@@ -503,7 +503,7 @@ run_observed!(e::HIPEnsembleHandle, nsteps::Integer; every::Integer) =
         every)); e)
 
 function autotune!(e::HIPEnsembleHandle; steps::Integer = 20)
-    best = Ref(FccLaunch(0, 0, 0, 0))
+    best = Ref(FccLaunch(0, 0, 0, 0, 0))
     ms = Ref{Cdouble}(0.0)
     check(ccall((:fcc_ensemble_autotune, libfcc_hip), Cint, (Ptr{Cvoid}, Int64, Ref{FccLaunch}, Ref{Cdouble}),
         e.handle, steps, best, ms))

@@ -331,6 +331,10 @@ typedef struct {
                              replica's own (four below 131072 bodies, two with
                              FCC_FLAG_TWO_LANES), 2 = two for every replica from now on, as
                              if created with FCC_FLAG_TWO_LANES (member handles included) */
+    int32_t kernels;      /* 0: one kernel per step where every replica qualifies (two lanes,
+                             FP32 or mixed without Bending, no parent drag, bonds or debug
+                             buffers; see FCC_FLAG_TWO_KERNELS), 2: a forces and an update
+                             kernel per step; results are the same bit for bit */
 } fcc_launch;
 
 /* Create n replicas from cfgs[n] and bodies[n] (validated like fcc_create).
@@ -350,11 +354,12 @@ int fcc_ensemble_get_launch(fcc_ensemble *ens, fcc_launch *launch);
 int fcc_ensemble_set_launch(fcc_ensemble *ens, const fcc_launch *launch);
 
 /* Autotuner: replays `steps` ensemble steps for every candidate launch shape
- * (update_block x xcd_order, and lanes 0 / 2 when some replica would take
- * the four-lane kernels) from the same state, restored after each trial and
- * at the end (the run state is unchanged), keeps the fastest and returns it
- * with its device time per step in milliseconds. With lanes = 2 chosen, the
- * replicas continue as replicas created with FCC_FLAG_TWO_LANES would. */
+ * (update_block x xcd_order, lanes 0 / 2 when some replica would take the
+ * four-lane kernels, kernels 0 / 2 where one kernel per step applies) from
+ * the same state, restored after each trial and at the end (the run state is
+ * unchanged), keeps the fastest and returns it with its device time per step
+ * in milliseconds. With lanes = 2 chosen, the replicas continue as replicas
+ * created with FCC_FLAG_TWO_LANES would. */
 int fcc_ensemble_autotune(fcc_ensemble *ens, int64_t steps, fcc_launch *best,
                           double *ms_per_step);
 

@@ -52,7 +52,7 @@ def test_struct_layout():
     assert ctypes.sizeof(_abi.FccBodyType) == 56
     assert ctypes.sizeof(_abi.FccTerm) == 8 + 16 + 16 + 2 * 8 * 16
     assert ctypes.sizeof(_abi.FccBodies) == 9 * 8
-    assert ctypes.sizeof(_abi.FccLaunch) == 16
+    assert ctypes.sizeof(_abi.FccLaunch) == 20
 
 
 def test_host_validation_errors():

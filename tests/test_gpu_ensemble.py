@@ -75,15 +75,18 @@ def test_uneven_sizes_and_parameters(precision, maxmode):
         assert_same(a, b)
 
 
-@pytest.mark.parametrize("update_block,xcd_order,lanes", list(itertools.product([64, 128, 256], [0, 1], [0, 2])))
-def test_every_launch_shape_gives_the_same_run(update_block, xcd_order, lanes):
+@pytest.mark.parametrize("update_block,xcd_order,lanes,kernels",
+                         list(itertools.product([64, 128, 256], [0, 1], [0, 2], [0, 2])))
+def test_every_launch_shape_gives_the_same_run(update_block, xcd_order, lanes, kernels):
     # lanes = 2: the two-lane forces kernels for every replica, as replicas
-    # created with FCC_FLAG_TWO_LANES would run alone
-    say("launch shape", update_block, xcd_order, lanes)
+    # created with FCC_FLAG_TWO_LANES would run alone (and then, with
+    # kernels = 0, one kernel per step)
+    say("launch shape", update_block, xcd_order, lanes, kernels)
     reps = [fast_replica(seed=s, nx=nx, ny=ny) for s, (nx, ny) in enumerate([(6, 6), (20, 14), (8, 30)])]
     seq = [F.run(alone(rp, "fp32", two_lanes=lanes == 2), 80) for rp in reps]
     ens = F.Ensemble(reps, precision="fp32")
-    launch = {"update_block": update_block, "xcd_order": xcd_order, "chunk": 0, "lanes": lanes}
+    launch = {"update_block": update_block, "xcd_order": xcd_order, "chunk": 0, "lanes": lanes,
+              "kernels": kernels}
     ens.launch = launch
     assert ens.launch == launch
     ens.run(80)

@@ -162,3 +162,73 @@ def test_one_kernel_at_full_size(precision):
     for u, v in zip(sa[0], sb[0]):
         assert np.array_equal(u, v)
     assert sa[1] == sb[1]
+
+
+def fast_replica(seed, n, v0=2.0, skin=0.05, maxmode=8):
+    # fast cells and a thin skin: many rebuilds on the side stream, frozen chunks
+    b, box = F.hexagonal_monolayer(n, n, packing=0.95, maxmode=maxmode, seed=seed)
+    return F.Replica(b, box, types=(F.BodyType(v0=v0, Dr=0.5, Dt=0.002),),
+                     energies=(F.Tension(0.01), F.Overlap(1.0)), dt=0.005, seed=seed, skin=skin)
+
+
+def alone(rp, precision, two_kernels):
+    return F.Simulation(rp.bodies, rp.box, types=rp.types, energies=rp.energies, dt=rp.dt, seed=rp.seed,
+                        skin=rp.skin, precision=precision, two_lanes=True, two_kernels=two_kernels)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "mixed_fp16"])
+def test_one_kernel_ensemble_equals_two_kernel_replicas(precision):
+    # the ensemble's one-kernel step (k_gate + k_step_*_ens, per-replica
+    # buffer parity) against each replica run alone with two kernels per
+    # step; sizes that are not multiples of the 64-body tiles, odd step
+    # counts, and member state read between runs
+    say("one-kernel ensemble == two-kernel replicas", precision)
+    reps = [fast_replica(s, n) for s, n in ((1, 8), (2, 10), (3, 6), (4, 12))]
+    ens = F.Ensemble(reps, precision=precision, two_lanes=True)
+    solo = [alone(rp, precision, True) for rp in reps]
+    for n in (77, 1, 40):
+        ens.run(n)
+        for s in solo:
+            F.run(s, n)
+        for a, b in zip(ens.members, solo):
+            assert_same(a, b, f"ensemble run {n}")
+    assert min(m.counters().builds for m in ens.members) > 5
+    ens.run_timed(21)
+    for s in solo:
+        F.run(s, 21)
+    for a, b in zip(ens.members, solo):
+        assert_same(a, b, "timed run")
+    # the same ensemble forced to two kernels per step ends alike
+    ens2 = F.Ensemble(reps, precision=precision, two_lanes=True, two_kernels=True)
+    ens2.run(77 + 1 + 40 + 21)
+    for a, b in zip(ens.members, ens2.members):
+        assert_same(a, b, "two-kernel ensemble")
+    ens.close()
+    ens2.close()
+
+
+def test_one_kernel_ensemble_autotune_and_observer():
+    # the autotuner times one-kernel and two-kernel candidates (lanes 2 and
+    # each replica's own) from a snapshot and restores it; observations of the
+    # members between runs
+    say("one-kernel ensemble autotune, observer")
+    reps = [fast_replica(s, n, v0=0.5, skin=0.1) for s, n in ((5, 8), (6, 12), (7, 10))]
+    ens = F.Ensemble(reps, precision="fp32")
+    launch, _ = ens.autotune(steps=16)
+    ens.run(30)
+    solo = [alone(rp, "fp32", True) if launch["lanes"] == 2 else
+            F.Simulation(rp.bodies, rp.box, types=rp.types, energies=rp.energies, dt=rp.dt, seed=rp.seed,
+                         skin=rp.skin, precision="fp32") for rp in reps]
+    for s in solo:
+        F.run(s, 30)
+    ens.observer_start(nbins=32, prange=(3.4, 4.4), max_samples=8)
+    for s in solo:
+        F.observer_start(s, nbins=32, prange=(3.4, 4.4), max_samples=8)
+    ens.run_observed(45, 9)
+    for s in solo:
+        F.run_observed(s, 45, 9)
+    for a, b in zip(ens.members, solo):
+        assert_same(a, b, "after autotune")
+        oa, ob = F.observations(a), F.observations(b)
+        assert np.array_equal(oa.msd, ob.msd) and np.array_equal(oa.hist, ob.hist)
+    ens.close()
