@@ -70,7 +70,7 @@ def make_sim(n, precision, seed=1234, two_kernels=False):
                         precision=precision, seed=seed, skin=0.3, two_kernels=two_kernels)
 
 
-def polydisperse_sim(precision, n=256, seed=7):
+def polydisperse_sim(precision, n=256, seed=7, two_kernels=False):
     """Polydisperse monolayer: hexagonal_monolayer(256, 256; packing 0.95),
     N = 65536, lognormal R0 (σ_log 0.1, mean 1), modes 2..6 with aₙ, bₙ
     uniform in [-1, 1] rescaled to Σ√(aₙ² + bₙ²) = 0.3 (a₀ from the area
@@ -88,10 +88,10 @@ def polydisperse_sim(precision, n=256, seed=7):
     b.coef[1:, :] = ab * (0.3 / amp)
     return F.Simulation(b, box, types=(F.BodyType(v0=0.06, Dr=0.01, muab=0.1),),
                         energies=(F.Tension(0.01), F.Overlap(1.0)), dt=0.005, ntheta=40,
-                        precision=precision, seed=seed, skin=0.3)
+                        precision=precision, seed=seed, skin=0.3, two_kernels=two_kernels)
 
 
-def nuclei_sim(precision, nx=256, ny=128, seed=1):
+def nuclei_sim(precision, nx=256, ny=128, seed=1, two_kernels=False):
     """Typed bodies with nuclei: hexagonal_monolayer(256, 128; packing 0.95)
     cells + one nucleus each (with_nuclei, ratio 0.4), 65536 bodies, modes
     2..8; cells: Tension(0.01), PerimeterElasticity(0.1, 3.7), v0 = 0.06;
@@ -104,7 +104,7 @@ def nuclei_sim(precision, nx=256, ny=128, seed=1):
                 F.OnTypes(F.Bending(0.01), 2),
                 F.Overlap([[1.0, 1.0], [1.0, 1.0]], related=[[0.0, -2.0], [-2.0, 0.0]]))
     return F.Simulation(b, box, types=types, energies=energies, dt=0.002, ntheta=40,
-                        precision=precision, seed=seed, skin=0.3)
+                        precision=precision, seed=seed, skin=0.3, two_kernels=two_kernels)
 
 
 WORKLOADS = {
@@ -115,12 +115,17 @@ WORKLOADS = {
 
 def bench_workload(name, precision, steps, warmup):
     """Throughput of one of the other workloads (its own JSON line), with the
-    plain FP32 path beside a mixed one."""
+    plain FP32 path beside a mixed one, and the workload's precision with two
+    kernels per step ("<precision>_two_kernels"; the same as the first leg
+    where a step is two kernels anyway, e.g. with parent drag)."""
     maker, default = WORKLOADS[name]
     precision = precision or default
     out = {"workload": name, "doc": " ".join(maker.__doc__.split()), "precision": precision}
-    for prec in [precision] + (["fp32"] if precision != "fp32" else []):
-        sim = maker(prec)
+    legs = [(precision, False)] + ([("fp32", False)] if precision != "fp32" else []) + [(precision, True)]
+    for prec, tk in legs:
+        sim = maker(prec, two_kernels=tk)
+        if tk:
+            prec = prec + "_two_kernels"
         F.run(sim, warmup)
         sec, _, _ = timed_run(sim, steps)
         fms, ums = kernel_durations(sim)

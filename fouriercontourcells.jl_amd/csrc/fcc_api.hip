@@ -775,15 +775,15 @@ struct Sim final : fcc_sim {
         return FCC_OK;
     }
 
-    // Whether a step runs as one kernel: single simulations on the two-lane
-    // FP32 / mixed kernels (the mixed one without Bending) with contact and a
-    // Verlet skin, without parent drag (a child reads its parent's θ before
-    // the update), bonds (their kernel adds to the centre forces between the
-    // forces and the update) or debug buffers.
+    // Whether a step runs as one kernel: single simulations on the two- or
+    // four-lane FP32 / mixed kernels (the mixed ones without Bending) with
+    // contact and a Verlet skin, without parent drag (a child reads its
+    // parent's θ before the update), bonds (their kernel adds to the centre
+    // forces between the forces and the update) or debug buffers.
     bool one_kernel() const {
         if constexpr (sizeof(T) != 4) return false;
-        return !member && !two_kernels && fast_ok() && nth == 40 && N >= four_below && has_contact &&
-               skin > 0.0 && !has_pdrag && !has_bonds && !debug && (!mfma_T || mfma_T == 3);
+        return !member && !two_kernels && fast_ok() && nth == 40 && has_contact && skin > 0.0 && !has_pdrag &&
+               !has_bonds && !debug && (!mfma_T || mfma_T == 3);
     }
 
     // The other buffer of each state pair, allocated at the first one-kernel step.
@@ -822,22 +822,29 @@ struct Sim final : fcc_sim {
     // use_status 3: the step's status protocol (step_open, reading the
     // buffers of parity `par`); 4: the same without the freeze check (timing
     // series, whose state is restored afterwards).
+    // Below four_below bodies the four-lane kernels (four waves of 16 bodies
+    // per workgroup), else the two-lane ones (two waves of 32).
     int launch_one_kernel(int use_status, hipEvent_t eb, hipEvent_t ee) {
         const unsigned g = blocks(N, 64);
+        const bool four = N < four_below;
         if constexpr (sizeof(T) == 4) {
             if (mfma_T) {
                 const bool bf = cfg.precision == FCC_MIXED_BF16;
-#define FCC_STEP_MF(KV, NCV)                                                                              \
-    if (bf) FCC_LAUNCH((k_step_mfma<KV, 3, NCV, true>), g, 128, 0, stream, eb, ee, p, d, asyn, bpull, use_status, par); \
-    else FCC_LAUNCH((k_step_mfma<KV, 3, NCV, false>), g, 128, 0, stream, eb, ee, p, d, asyn, bpull, use_status, par);
+#define FCC_STEP_MF2(KV, NCV, BFV)                                                                        \
+    if (four) FCC_LAUNCH((k_step_mfma4<KV, 3, NCV, BFV>), g, 256, 0, stream, eb, ee, p, d, asyn4, bpull4, use_status, par); \
+    else FCC_LAUNCH((k_step_mfma<KV, 3, NCV, BFV>), g, 128, 0, stream, eb, ee, p, d, asyn, bpull, use_status, par);
+#define FCC_STEP_MF(KV, NCV) if (bf) { FCC_STEP_MF2(KV, NCV, true) } else { FCC_STEP_MF2(KV, NCV, false) }
                 if (K == 5) {
                     if (p.ncls <= 1) { FCC_STEP_MF(5, 1) } else { FCC_STEP_MF(5, 2) }
                 } else {
                     if (p.ncls <= 1) { FCC_STEP_MF(7, 1) } else { FCC_STEP_MF(7, 2) }
                 }
 #undef FCC_STEP_MF
+#undef FCC_STEP_MF2
             } else {
-#define FCC_STEP_F(KV, NCV) FCC_LAUNCH((k_step_fast2<T, KV, NCV>), g, 128, 0, stream, eb, ee, p, d, use_status, par);
+#define FCC_STEP_F(KV, NCV)                                                                               \
+    if (four) FCC_LAUNCH((k_step_fast4<T, KV, NCV>), g, 256, 0, stream, eb, ee, p, d, use_status, par);     \
+    else FCC_LAUNCH((k_step_fast2<T, KV, NCV>), g, 128, 0, stream, eb, ee, p, d, use_status, par);
                 if (K == 5) {
                     if (p.ncls <= 1) { FCC_STEP_F(5, 1) } else { FCC_STEP_F(5, 2) }
                 } else {

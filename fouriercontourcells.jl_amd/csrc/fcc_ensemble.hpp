@@ -173,27 +173,34 @@ struct Ensemble final : fcc_ensemble {
         if (a.nth != 40 || a.has_pdrag || a.debug || (a.cfg.flags & FCC_FLAG_TWO_KERNELS)) return false;
         if (v == 2 && a.mfma_T != 3) return false;
         for (const Sim<T> *s : m)
-            if (s->N < s->four_below || !s->has_contact || s->skin <= 0.0 || s->has_bonds) return false;
+            if (!s->has_contact || s->skin <= 0.0 || s->has_bonds) return false;
         return true;
     }
 
-    // The step kernel of a one-kernel ensemble step (tiles of 64 bodies).
+    // The step kernels of a one-kernel ensemble step (tiles of 64 bodies): the
+    // four-lane one over the gf4 tiles of the replicas below four_below
+    // bodies, the two-lane one over the gf tiles after them.
     int launch_step_kernel(hipEvent_t eb = nullptr, hipEvent_t ee = nullptr) {
         if constexpr (sizeof(T) == 4) {
             const Sim<T> &a = *m[0];
+            hipEvent_t e4 = gf ? nullptr : ee, e2 = gf4 ? nullptr : eb;
             if (var == 2) {
                 const bool bf = a.cfg.precision == FCC_MIXED_BF16;
-#define FCC_SME(KV, NCV)                                                                                      \
-    if (bf) FCC_LAUNCH((k_step_mfma_ens<KV, 3, NCV, true>), gf, 128, 0, stream, eb, ee, dP, dD, tf, a.asyn, a.bpull); \
-    else FCC_LAUNCH((k_step_mfma_ens<KV, 3, NCV, false>), gf, 128, 0, stream, eb, ee, dP, dD, tf, a.asyn, a.bpull);
+#define FCC_SME2(KV, NCV, BFV)                                                                                \
+    if (gf4) FCC_LAUNCH((k_step_mfma4_ens<KV, 3, NCV, BFV>), gf4, 256, 0, stream, eb, e4, dP, dD, tf, a.asyn4, a.bpull4); \
+    if (gf) FCC_LAUNCH((k_step_mfma_ens<KV, 3, NCV, BFV>), gf, 128, 0, stream, e2, ee, dP, dD, tf + gf4, a.asyn, a.bpull);
+#define FCC_SME(KV, NCV) if (bf) { FCC_SME2(KV, NCV, true) } else { FCC_SME2(KV, NCV, false) }
                 if (a.K == 5) {
                     if (ncls <= 1) { FCC_SME(5, 1) } else { FCC_SME(5, 2) }
                 } else {
                     if (ncls <= 1) { FCC_SME(7, 1) } else { FCC_SME(7, 2) }
                 }
 #undef FCC_SME
+#undef FCC_SME2
             } else {
-#define FCC_SFE(KV, NCV) FCC_LAUNCH((k_step_fast2_ens<T, KV, NCV>), gf, 128, 0, stream, eb, ee, dP, dD, tf);
+#define FCC_SFE(KV, NCV)                                                                                      \
+    if (gf4) FCC_LAUNCH((k_step_fast4_ens<T, KV, NCV>), gf4, 256, 0, stream, eb, e4, dP, dD, tf);             \
+    if (gf) FCC_LAUNCH((k_step_fast2_ens<T, KV, NCV>), gf, 128, 0, stream, e2, ee, dP, dD, tf + gf4);
                 if (a.K == 5) {
                     if (ncls <= 1) { FCC_SFE(5, 1) } else { FCC_SFE(5, 2) }
                 } else {
@@ -383,11 +390,12 @@ struct Ensemble final : fcc_ensemble {
             // Bodies per forces workgroup: 16 for the four-lane kernels (replicas
             // below FOUR_LANES_BELOW bodies at nθ = 40, the kernel each would get
             // alone: MFMA or FP32 / FP64), 32 for the two-lane kernels, one for
-            // contours of more than 64 nodes, 64 otherwise.
+            // contours of more than 64 nodes, 64 otherwise; 64 for both lane
+            // counts in one-kernel steps.
             const bool fast40 = (var == 1 && m[0]->nth == 40) || var == 2;
             for (size_t r = 0; r < R; r++) (fast40 && sizes[r] < m[r]->four_below ? small : large)[r] = sizes[r];
             const int tile = one ? 64 : (var == 3 ? 1 : (var == 2 || fast40 ? 32 : 64));
-            std::vector<Tile> a = tile_table(small, 16, lc.xcd_order != 0);
+            std::vector<Tile> a = tile_table(small, one ? 64 : 16, lc.xcd_order != 0);
             const size_t n4 = a.size();
             const std::vector<Tile> a2 = tile_table(large, tile, lc.xcd_order != 0);
             a.insert(a.end(), a2.begin(), a2.end());

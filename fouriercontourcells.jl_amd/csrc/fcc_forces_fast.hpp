@@ -434,15 +434,15 @@ __device__ __forceinline__ void quarter_turn(T &c, T &s, int r, bool inv) {
 // cache, and its pullback sums take the inverse turn before the groups are
 // added.
 //
-// NW waves per workgroup (NW = 2 only with two lanes per body), wave w on the
-// bodies of tile wg·NW + w; with NW = 2 the workgroup is a one-kernel step
-// (the pullback rows and centre force / torque go to the waves' LDS slices,
-// then wave 0 updates the 64 bodies, fused_update).
+// NW waves per workgroup (1, or NW = LPB: 64 bodies), wave w on the bodies of
+// tile wg·NW + w; with NW = LPB the workgroup is a one-kernel step (the
+// pullback rows and centre force / torque go to the waves' LDS slices, then
+// wave 0 updates the 64 bodies, fused_update).
 template <typename T, int K, int NCLS, int LPB, int NW, typename PP, typename DP>
 __device__ __forceinline__ void k_forces_fast2_impl(const PP &p, const DP &d, int use_status, int debug,
                                                      int64_t wg, int q = 0) {
     static_assert(LPB == 2 || LPB == 4, "two or four lanes per body");
-    static_assert(NW == 1 || (NW == 2 && LPB == 2), "two waves per workgroup with two lanes per body");
+    static_assert(NW == 1 || NW == LPB, "one wave, or 64 bodies per workgroup");
     constexpr int NTH = 40, NB = 64 / LPB, NH2 = NTH / LPB;
     constexpr int NC = 2 * K + 1;
     constexpr int NCL = NCLS > 0 ? NCLS : 1;
@@ -460,7 +460,7 @@ __device__ __forceinline__ void k_forces_fast2_impl(const PP &p, const DP &d, in
     const int64_t blk = wg * NW + w;
     const int64_t i = (int64_t)blk * NB + b;
     int64_t clk = 0;  // one-kernel step: its clock
-    if constexpr (NW == 2) {
+    if constexpr (NW > 1) {
         if (use_status == 2) {  // ensemble step: k_gate decided and ticked the clock
             if (d.st->update != 0) return;
             clk = d.st->clock - 1;
@@ -733,7 +733,7 @@ __device__ __forceinline__ void k_forces_fast2_impl(const PP &p, const DP &d, in
         d.gth[ii] = tt;
         d.Ebody[ii] = E;
     }
-    if constexpr (NW == 2) {
+    if constexpr (NW > 1) {
         // the same row, force and torque into the wave's LDS slice (pass 2
         // has read its contour slots): body b at rows[b·W], the force and
         // torque after the NB rows
@@ -773,6 +773,22 @@ __global__ __launch_bounds__(128) void k_step_fast2_ens(const Params<T> *ps, con
     const FCC_CONST Params<T> &p = as_const(ps)[tl.rep];
     const Ptrs<T> d = step_buffers(load_uniform(ds, tl.rep));
     k_forces_fast2_impl<T, K, NCLS, 2, 2>(p, d, 2, 0, tl.blk);
+}
+
+// The one-kernel step with four lanes per body (systems below
+// FOUR_LANES_BELOW bodies): four waves of 16 bodies per workgroup.
+template <typename T, int K, int NCLS>
+__global__ __launch_bounds__(256) void k_step_fast4(const Params<T> p, const Ptrs<T> d, int use_status, int q) {
+    k_forces_fast2_impl<T, K, NCLS, 4, 4>(p, d, use_status, 0, xcd_tile(blockIdx.x, gridDim.x), q);
+}
+
+template <typename T, int K, int NCLS>
+__global__ __launch_bounds__(256) void k_step_fast4_ens(const Params<T> *ps, const Ptrs<T> *ds, const Tile *tiles) {
+    const Tile tl = load_uniform(tiles, blockIdx.x);
+    if (tl.rep < 0) return;
+    const FCC_CONST Params<T> &p = as_const(ps)[tl.rep];
+    const Ptrs<T> d = step_buffers(load_uniform(ds, tl.rep));
+    k_forces_fast2_impl<T, K, NCLS, 4, 4>(p, d, 2, 0, tl.blk);
 }
 
 template <typename T, int K, int NCLS, int LPB>

@@ -105,25 +105,44 @@ __device__ __forceinline__ void synth4(const float *lab_row, bool live, int g, i
 // Operand tables (built on the host, 16-bit values):
 //  asyn[(t * 2 + part) * 64 + lane][4]   synthesis A (S rows), part 0 hi / 1 lo
 //  bpull[(t * 2 + part) * 64 + lane][4]  pullback B (S rows of tile t), part 0 hi / 1 lo
-template <int K, int T, int NCLS, bool BF, typename PP, typename DP>
+//
+// NW waves per workgroup (1 or 4), wave w on the 16 bodies of tile wg·NW + w.
+// With NW = 4 the workgroup is a one-kernel step of 64 bodies: the waves
+// leave their bodies' pullback rows and centre force / torque in their LDS
+// slices, and wave 0 updates the 64 bodies (fused_update).
+template <int K, int T, int NCLS, bool BF, int NW, typename PP, typename DP>
 __device__ __forceinline__ void k_forces_mfma4_impl(const PP &p, const DP &d, const uint16_t *asyn,
                                                      const uint16_t *bpull, int use_status, int debug,
-                                                     int64_t blk) {
+                                                     int64_t wg, int q = 0) {
     typedef Mfma16<BF> MT;
     typedef typename MT::v_t v_t;
     constexpr int NC = 2 * K + 1;
     constexpr int W = row_width<NC, float>();
     constexpr int TT = tiles4<T>();
-    __shared__ float sR[NTH_M][16];  // R of the wave's 16 bodies for the contact pass
+    static_assert(NW == 1 || NW == 4, "one wave, or 64 bodies per workgroup");
+    static_assert(16 * W + 3 * 16 <= NTH_M * 16, "pullback rows fit the wave's LDS slice");
+    __shared__ float sRw[NW][NTH_M][16];  // R of each wave's 16 bodies for the contact pass
     __shared__ float scs[2][NTH_M];
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63, w = NW == 1 ? 0 : (int)(threadIdx.x >> 6);
+    float(*sR)[16] = sRw[w];
     const int b = lane & 15, g = lane >> 4;
-    for (int k = lane; k < NTH_M; k += 64) {
+    for (int k = threadIdx.x; k < NTH_M; k += 64 * NW) {
         scs[0][k] = d.cs[2 * k];
         scs[1][k] = d.cs[2 * k + 1];
     }
+    const int64_t blk = wg * NW + w;
     const int64_t i = (int64_t)blk * 16 + b;
-    if (forces_frozen(use_status, d.st, blk, lane)) return;  // uniform over the grid
+    int64_t clk = 0;  // one-kernel step: its clock
+    if constexpr (NW > 1) {
+        if (use_status == 2) {  // ensemble step: k_gate decided and ticked the clock
+            if (d.st->update != 0) return;
+            clk = d.st->clock - 1;
+        } else if (step_open(d.st, q, use_status == 4, blk, lane, clk)) {
+            return;  // uniform over the grid
+        }
+    } else {
+        if (forces_frozen(use_status, d.st, blk, lane)) return;  // uniform over the grid
+    }
     const bool live = i < p.N;
     const int64_t ii = live ? i : 0;
     const float r0 = live ? d.R0[ii] : 1.f;
@@ -410,13 +429,54 @@ __device__ __forceinline__ void k_forces_mfma4_impl(const PP &p, const DP &d, co
         d.gth[ii] = tt;
         d.Ebody[ii] = E;
     }
+    if constexpr (NW > 1) {
+        // the same rows into the wave's LDS slice (the contact pass has read
+        // its contour slots): body r of the wave at rows[r·W], its force and
+        // torque after the 16 rows
+        float *rows = &sR[0][0];
+        if (n < NC) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) rows[(4 * g + r) * W + n] = gacc[r] * (1.f / PULL_SCALE);
+        }
+        if (g == 0) {
+            rows[16 * W + 3 * b] = fx;
+            rows[16 * W + 3 * b + 1] = fy;
+            rows[16 * W + 3 * b + 2] = tt;
+        }
+        __syncthreads();
+        if (w != 0) return;
+        fused_update<float, K, true>(p, d, &sRw[0][0][0], NTH_M * 16, 16, wg * 64, lane, q, clk, use_status == 2);
+    }
 }
 
 template <int K, int T, int NCLS, bool BF>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_forces_mfma4(const Params<float> p, const Ptrs<float> d,
                                                      const uint16_t *asyn, const uint16_t *bpull, int use_status,
                                                      int debug) {
-    k_forces_mfma4_impl<K, T, NCLS, BF>(p, d, asyn, bpull, use_status, debug, xcd_tile(blockIdx.x, gridDim.x));
+    k_forces_mfma4_impl<K, T, NCLS, BF, 1>(p, d, asyn, bpull, use_status, debug, xcd_tile(blockIdx.x, gridDim.x));
+}
+
+// One-kernel step of the mixed path with four lanes per body (four waves, 64
+// bodies per workgroup) reading the buffers of parity q; use_status 3, or 4
+// for timing series.
+template <int K, int T, int NCLS, bool BF>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_step_mfma4(const Params<float> p, const Ptrs<float> d,
+                                                     const uint16_t *asyn, const uint16_t *bpull, int use_status,
+                                                     int q) {
+    k_forces_mfma4_impl<K, T, NCLS, BF, 4>(p, d, asyn, bpull, use_status, 0, xcd_tile(blockIdx.x, gridDim.x), q);
+}
+
+// Ensemble form: tiles of 64 bodies, each replica's buffers by the parity
+// k_gate left in its status (step_buffers).
+template <int K, int T, int NCLS, bool BF>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_step_mfma4_ens(const Params<float> *ps, const Ptrs<float> *ds,
+                                                         const Tile *tiles, const uint16_t *asyn,
+                                                         const uint16_t *bpull) {
+    const Tile tl = load_uniform(tiles, blockIdx.x);
+    if (tl.rep < 0) return;
+    const FCC_CONST Params<float> &p = as_const(ps)[tl.rep];
+    const Ptrs<float> d = step_buffers(load_uniform(ds, tl.rep));
+    k_forces_mfma4_impl<K, T, NCLS, BF, 4>(p, d, asyn, bpull, 2, 0, tl.blk);
 }
 
 template <int K, int T, int NCLS, bool BF>
@@ -427,7 +487,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     if (tl.rep < 0) return;
     const FCC_CONST Params<float> &p = as_const(ps)[tl.rep];
     const FCC_CONST Ptrs<float> &d = as_const(ds)[tl.rep];
-    k_forces_mfma4_impl<K, T, NCLS, BF>(p, d, asyn, bpull, use_status, debug, tl.blk);
+    k_forces_mfma4_impl<K, T, NCLS, BF, 1>(p, d, asyn, bpull, use_status, debug, tl.blk);
 }
 
 }  // namespace fcc

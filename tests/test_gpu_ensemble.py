@@ -199,14 +199,16 @@ def test_replicas_must_share_modes_and_precision():
         F.Ensemble([], precision="fp32")
 
 
-def test_timed_run_equals_run():
-    say("timed ensemble run")
+@pytest.mark.parametrize("two_kernels", [False, True])
+def test_timed_run_equals_run(two_kernels):
+    # one-kernel steps time the step kernel as "forces" (update 0)
+    say("timed ensemble run", two_kernels)
     reps = [fast_replica(seed=s, nx=12, ny=12) for s in (7, 8)]
     e1 = F.Ensemble(reps, precision="fp32")
-    e2 = F.Ensemble(reps, precision="fp32")
+    e2 = F.Ensemble(reps, precision="fp32", two_kernels=two_kernels)
     e1.run(70)
     f, u = e2.run_timed(70)
-    assert f > 0 and u > 0
+    assert f > 0 and (u > 0 if two_kernels else u == 0)
     for a, b in zip(e1.members, e2.members):
         assert_same(a, b)
 

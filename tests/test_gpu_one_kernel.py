@@ -1,13 +1,14 @@
-"""One-kernel steps (k_step_mfma, k_step_fast2): the forces and the update of
-a step in one kernel, whose workgroups write the new positions, lab rows and
-radius bounds into the other buffer of each pair while other workgroups still
-read the current state as neighbours. Every result must equal bitwise that of
-the two-kernel step (FCC_FLAG_TWO_KERNELS: the forces kernel, then the update
+"""One-kernel steps (k_step_mfma, k_step_fast2 with two lanes per body,
+k_step_mfma4, k_step_fast4 with four): the forces and the update of a step in
+one kernel, whose workgroups write the new positions, lab rows and radius
+bounds into the other buffer of each pair while other workgroups still read
+the current state as neighbours. Every result must equal bitwise that of the
+two-kernel step (FCC_FLAG_TWO_KERNELS: the forces kernel, then the update
 kernel), whose arithmetic it shares: in graph mode and eagerly, over rebuilds,
 frozen chunks, odd step counts, checkpoints, observers and state reads. The
 two-kernel step itself is pinned to the FP64 oracle by test_gpu_parity.py and
-test_gpu_precision.py; ensembles (two-kernel) against one-kernel single runs
-are in test_gpu_ensemble.py and test_gpu_config4.py."""
+test_gpu_precision.py; ensembles against single runs are also in
+test_gpu_ensemble.py and test_gpu_config4.py."""
 import sys
 
 import numpy as np
@@ -55,9 +56,12 @@ def assert_same(a, b, what):
 
 @pytest.mark.parametrize("precision", ["fp32", "mixed_fp16", "mixed_bf16"])
 @pytest.mark.parametrize("maxmode", [6, 8])
-def test_one_kernel_equals_two_kernels(precision, maxmode):
-    say("one kernel == two kernels", precision, maxmode)
-    a, b = monolayer(precision, False, maxmode=maxmode), monolayer(precision, True, maxmode=maxmode)
+@pytest.mark.parametrize("lanes", [2, 4])
+def test_one_kernel_equals_two_kernels(precision, maxmode, lanes):
+    say("one kernel == two kernels", precision, maxmode, lanes)
+    tl = lanes == 2
+    a = monolayer(precision, False, maxmode=maxmode, two_lanes=tl)
+    b = monolayer(precision, True, maxmode=maxmode, two_lanes=tl)
     # odd counts: the state ends in either buffer of the pairs
     for n in (137, 1, 64):
         F.run(a, n)
@@ -68,7 +72,7 @@ def test_one_kernel_equals_two_kernels(precision, maxmode):
     assert_same(a, b, "eager steps")
     assert a.counters().builds > 5
     # a checkpoint taken from either buffer parity resumes alike
-    c = monolayer(precision, False, maxmode=maxmode).restore(a.checkpoint())
+    c = monolayer(precision, False, maxmode=maxmode, two_lanes=tl).restore(a.checkpoint())
     for s in (a, b, c):
         F.run(s, 41)
     assert_same(a, b, "after checkpoint")
@@ -83,9 +87,10 @@ def test_one_kernel_equals_two_kernels(precision, maxmode):
 
 
 @pytest.mark.parametrize("precision", ["fp32", "mixed_fp16"])
-def test_one_kernel_graph_equals_eager_with_callbacks(precision):
-    say("one kernel graph == eager", precision)
-    a, b = monolayer(precision, False), monolayer(precision, False)
+@pytest.mark.parametrize("lanes", [2, 4])
+def test_one_kernel_graph_equals_eager_with_callbacks(precision, lanes):
+    say("one kernel graph == eager", precision, lanes)
+    a, b = (monolayer(precision, False, two_lanes=lanes == 2) for _ in range(2))
     F.step(a, 90)
     seen = []
     F.run(b, 90, every=7, callback=lambda s: seen.append(s.step_count))
@@ -97,16 +102,17 @@ def test_one_kernel_graph_equals_eager_with_callbacks(precision):
     assert_same(a, b, "after dt change")
 
 
-def test_one_kernel_two_coupling_classes_and_bending():
+@pytest.mark.parametrize("lanes", [2, 4])
+def test_one_kernel_two_coupling_classes_and_bending(lanes):
     # two coupling classes (the NCLS = 2 kernels) and Bending in the plain
     # kernel; types 1 and 2 alternate
-    say("one kernel, two classes, bending")
+    say("one kernel, two classes, bending", lanes)
     types = (F.BodyType(v0=0.5, Dr=0.2), F.BodyType(v0=0.2, Dr=0.4, Dt=0.001))
     energies = (F.Tension(0.01), F.OnTypes(F.Bending(0.002), 2),
                 F.Overlap([[1.0, 0.5], [0.5, 1.0]]))
 
     def mk(prec, tk):
-        s = monolayer(prec, tk, types=types, energies=energies)
+        s = monolayer(prec, tk, types=types, energies=energies, two_lanes=lanes == 2)
         bo = s.bodies
         bo.type = (np.arange(len(bo.x)) % 2 + 1).astype(np.int32)
         s.bodies = bo
@@ -119,11 +125,12 @@ def test_one_kernel_two_coupling_classes_and_bending():
         assert_same(a, b, prec)
 
 
-def test_one_kernel_observer_and_frames():
+@pytest.mark.parametrize("lanes", [2, 4])
+def test_one_kernel_observer_and_frames(lanes):
     # device observations between runs, a frame requested before a run (its
     # contours are those of the state at the request), contours after
-    say("one kernel observer, frames")
-    a, b = monolayer("mixed_fp16", False), monolayer("mixed_fp16", True)
+    say("one kernel observer, frames", lanes)
+    a, b = (monolayer("mixed_fp16", tk, two_lanes=lanes == 2) for tk in (False, True))
     for s in (a, b):
         F.run(s, 11)
         F.observer_start(s, nbins=64, prange=(3.4, 4.4), max_samples=16)
@@ -232,3 +239,29 @@ def test_one_kernel_ensemble_autotune_and_observer():
         oa, ob = F.observations(a), F.observations(b)
         assert np.array_equal(oa.msd, ob.msd) and np.array_equal(oa.hist, ob.hist)
     ens.close()
+
+
+def own_lanes(rp, precision, two_kernels):
+    return F.Simulation(rp.bodies, rp.box, types=rp.types, energies=rp.energies, dt=rp.dt, seed=rp.seed,
+                        skin=rp.skin, precision=precision, two_kernels=two_kernels)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "mixed_fp16"])
+def test_one_kernel_ensemble_with_both_lane_counts(precision):
+    # replicas on their own lane choice: the small ones on the four-lane step
+    # kernel, one of 132496 bodies on the two-lane one, both launched per
+    # step over their tiles; each equals itself alone with two kernels
+    say("one-kernel ensemble, four- and two-lane replicas", precision)
+    reps = [fast_replica(11, 10), fast_replica(12, 364, v0=0.5, skin=0.2), fast_replica(13, 8)]
+    ens = F.Ensemble(reps, precision=precision)
+    solo = [own_lanes(rp, precision, True) for rp in reps]
+    for n in (45, 3):
+        ens.run(n)
+        for s in solo:
+            F.run(s, n)
+        for a, b in zip(ens.members, solo):
+            assert_same(a, b, f"ensemble run {n}")
+    assert ens.members[0].counters().builds > 3
+    ens.close()
+    for s in solo:
+        s.close()

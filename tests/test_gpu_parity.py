@@ -159,11 +159,11 @@ def test_regression_fixture_on_device_fp64(hip):
         np.testing.assert_allclose(bod.coef[1:], ref[5:15], atol=1e-10, rtol=0)
 
 
-def soft_fluid(precision="fp64", v0=2.0, skin=0.05, seed=1, n=8, ntheta=40):
+def soft_fluid(precision="fp64", v0=2.0, skin=0.05, seed=1, n=8, ntheta=40, two_kernels=False):
     b, box = F.hexagonal_monolayer(n, n, packing=0.95)
     return F.Simulation(b, box, types=(F.BodyType(v0=v0, Dr=0.5, Dt=0.01),),
                         energies=(F.Tension(0.01), F.Overlap(1.0)), dt=0.005, skin=skin, seed=seed,
-                        precision=precision, ntheta=ntheta)
+                        precision=precision, ntheta=ntheta, two_kernels=two_kernels)
 
 
 def state(s):
@@ -215,15 +215,17 @@ def test_body_terms_only_graph(hip):
     assert same(F.step(mk(), 50), F.run(mk(), 50))
 
 
-def test_timed_run_equals_run(hip):
-    # the instrumented graph run (bench timing) changes nothing in the results
+@pytest.mark.parametrize("two_kernels", [False, True])
+def test_timed_run_equals_run(hip, two_kernels):
+    # the instrumented graph run (bench timing) changes nothing in the results;
+    # a one-kernel step is timed as "forces" (update 0)
     import ctypes
-    say("timed run == run")
-    a, b = soft_fluid("fp32"), soft_fluid("fp32")
+    say("timed run == run", two_kernels)
+    a, b = soft_fluid("fp32"), soft_fluid("fp32", two_kernels=two_kernels)
     F.run(a, 150)
     f, u = ctypes.c_double(), ctypes.c_double()
     assert hip.run_timed(b._h, 150, ctypes.byref(f), ctypes.byref(u)) == 0
-    assert f.value > 0 and u.value > 0
+    assert f.value > 0 and (u.value > 0 if two_kernels else u.value == 0)
     assert same(a, b)
 
 
@@ -232,12 +234,15 @@ def test_kernel_timing_leaves_state_unchanged(hip, precision):
     # fcc_time_kernels saves and restores the state around its launches
     import ctypes
     say("kernel timing", precision)
-    a, b = soft_fluid(precision), soft_fluid(precision)
+    a, b = soft_fluid(precision), soft_fluid(precision, two_kernels=True)
     F.run(a, 60)
     F.run(b, 60)
     f, u = ctypes.c_double(), ctypes.c_double()
     assert hip.time_kernels(b._h, 7, ctypes.byref(f), ctypes.byref(u)) == 0
     assert f.value > 0 and u.value > 0
+    # and a simulation on one-kernel steps
+    assert hip.time_kernels(a._h, 7, ctypes.byref(f), ctypes.byref(u)) == 0
+    assert f.value > 0 and np.isfinite(u.value)
     F.run(a, 60)
     F.run(b, 60)
     assert same(a, b)
