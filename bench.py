@@ -173,9 +173,10 @@ def sync(sim):
 def timed_run(sim, steps):
     """Graph-mode run of `steps` steps, timed on the host between two device
     synchronisations (torch.cuda.synchronize waits for every stream of the
-    device, the simulation's own included); the eager last step of each
-    chunk, between HIP events, gives the average kernel durations over the
-    same region (fcc_run_timed)."""
+    device, the simulation's own included); HIP events around each chunk's
+    graph launches (one-kernel steps) or around the kernels of each chunk's
+    eagerly launched last step (two-kernel steps) give the average kernel
+    durations over the same region (fcc_run_timed)."""
     import torch
     f, u = ctypes.c_double(), ctypes.c_double()
     sync(sim)
@@ -552,9 +553,10 @@ def main():
                                   independent=not args.no_independent)
 
     cpu = None if args.no_cpu else cpu_baseline(args.n, args.cpu_seconds)
-    timing_note = ("HIP events on the simulation's stream around the last step of each graph chunk of the "
-                   "timed run, launched eagerly with the graph's kernels and arguments (fcc_run_timed; roctx "
-                   "range 'timed'); flops of the mean of its first and last state")
+    timing_note = ("HIP events on the simulation's stream around the graph launches of each chunk of the "
+                   "timed run, the chunk's time over its steps (one-kernel steps; two-kernel steps: around the "
+                   "kernels of each chunk's last step, launched eagerly with the graph's kernels and arguments), "
+                   "fcc_run_timed, roctx range 'timed'; flops of the mean of its first and last state")
     end_note = ("captured graph of 50 back-to-back launches at the end state of the timed run, right after it, "
                 "with ~400 untimed launches first (fcc_time_kernels; roctx range 'kernel_timing'); the forces "
                 "series re-evaluates one state, whose data stay in L2 between launches")

@@ -190,6 +190,10 @@ static bool row_phys4(int t, int m, int T, int &q, int &k) {
 
 struct fcc_sim {
     bool borrowed = false;  // owned by an ensemble
+    // The device status is the one the last fcc_run left, known on the host
+    // (run() sets it; every entry point that may change the status or the
+    // state clears it): the next run needs no start snapshot.
+    bool mirror_ok = false;
     virtual ~fcc_sim() {}
     virtual int set_bodies(const fcc_bodies *b) = 0;
     virtual int get_bodies(fcc_bodies *b) = 0;
@@ -273,7 +277,8 @@ struct Sim final : fcc_sim {
     // run), each with its completion event, and the timing events of each
     // chunk's eagerly launched last step (fcc_run_timed).
     static constexpr int NQ = 2;
-    DevStatus *snap = nullptr;
+    DevStatus *snap = nullptr, *snap_d = nullptr;  // pinned status snapshots, and their device address
+    DevStatus mirror{};                            // the status a run left (mirror_ok)
     hipEvent_t sev[NQ + 1] = {};
     hipEvent_t tev[NQ][4] = {};
     double interval = 16.0;
@@ -800,8 +805,8 @@ struct Sim final : fcc_sim {
     // Before one-kernel steps from the current buffers (and after eager ones):
     // a skin trip of the last one-kernel step becomes the rebuild flag, and
     // the step clock slot of parity `par` takes the clock (k_step_slots).
-    int sync_slots() {
-        hipLaunchKernelGGL(k_step_slots, 1, 1, 0, stream, d.st, par);
+    int sync_slots(DevStatus *out = nullptr) {
+        hipLaunchKernelGGL(k_step_slots, 1, 1, 0, stream, d.st, par, out);
         HIPCHK(hipGetLastError());
         return FCC_OK;
     }
@@ -1095,6 +1100,7 @@ struct Sim final : fcc_sim {
     int ensure_pipeline() {
         if (snap) return FCC_OK;
         HIPCHK(hipHostMalloc((void **)&snap, sizeof(DevStatus) * (NQ + 1), hipHostMallocDefault));
+        HIPCHK(hipHostGetDevicePointer((void **)&snap_d, snap, 0));
         for (auto &e : sev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         for (int q = 0; q < NQ; q++)
             for (auto &e : tev[q]) HIPCHK(hipEventCreate(&e));
@@ -1102,21 +1108,26 @@ struct Sim final : fcc_sim {
     }
 
     // k steps, then a snapshot of the status into pinned slot `slot`.
+    // Timed runs: one-kernel steps between two events around the chunk's
+    // graph launches (inside a graph the step kernels follow each other
+    // without a gap, so the interval over k is the step kernel's duration);
+    // two-kernel steps launch the chunk's last step eagerly, the same kernels
+    // and arguments as the graph, with events around each kernel.
     int issue(int64_t k, int slot) {
         int rc;
         if (!want_timed) {
             if ((rc = replay(k))) return rc;
+        } else if (gone) {
+            HIPCHK(hipEventRecord(tev[slot][0], stream));
+            if ((rc = replay(k))) return rc;
+            HIPCHK(hipEventRecord(tev[slot][1], stream));
         } else {
-            // the chunk's last step is launched eagerly between events:
-            // the same kernels and arguments as the graph
             if ((rc = replay(k - 1))) return rc;
             p.skin_check = (has_contact && skin > 0.0) ? 1 : 0;
-            // (a one-kernel step has no update kernel to time)
-            if ((rc = launch_step(tev[slot][0], tev[slot][1], gone ? nullptr : tev[slot][2],
-                                  gone ? nullptr : tev[slot][3])))
-                return rc;
+            if ((rc = launch_step(tev[slot][0], tev[slot][1], tev[slot][2], tev[slot][3]))) return rc;
         }
-        HIPCHK(hipMemcpyAsync(snap + slot, d.st, sizeof(DevStatus), hipMemcpyDeviceToHost, stream));
+        hipLaunchKernelGGL(k_snapshot, 1, 1, 0, stream, d.st, snap_d + slot);
+        HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(sev[slot], stream));
         return FCC_OK;
     }
@@ -1142,20 +1153,32 @@ struct Sim final : fcc_sim {
     // ext/FourierContourCellsCUDAExt.jl:65-71; it does not change results.
     int run(int64_t n) override {
         if (n <= 0) return FCC_OK;
+        const bool mirrored = mirror_ok && !member;
+        mirror_ok = false;  // set again by a run that ends with the status known
         if (has_contact && skin <= 0.0) return step(n);  // graph_capable (src/simulation.jl:206-207)
         int rc;
         if ((rc = ensure_pipeline())) return rc;
         if (has_contact && !valid && (rc = rebuild())) return rc;
-        if (one_kernel() && (rc = sync_slots())) return rc;
         // The status at the start of the run is one more snapshot (slot NQ),
-        // read while the first chunks run. A rebuild flag raised by the last
-        // step before the run freezes them, and the lists are rebuilt before
-        // the run goes on (update_neighbors!(sim), src/simulation.jl:132-142).
-        HIPCHK(hipMemcpyAsync(snap + NQ, d.st, sizeof(DevStatus), hipMemcpyDeviceToHost, stream));
-        HIPCHK(hipEventRecord(sev[NQ], stream));
-        bool started = false;
-        DevStatus st;
-        int64_t clock = 0, done = 0, planned = 0, issued = 0, read = 0;
+        // read while the first chunks run (for one-kernel steps written by the
+        // slot synchronisation). A rebuild flag raised by the last step before
+        // the run freezes them, and the lists are rebuilt before the run goes
+        // on (update_neighbors!(sim), src/simulation.jl:132-142). Straight
+        // after a run that left no rebuild flag and no pending skin trip (and
+        // with nothing in between that changes the status or the state), the
+        // status is that run's last snapshot: no start snapshot.
+        bool started = mirrored;
+        DevStatus st, last = mirror;
+        if (!mirrored) {
+            if (one_kernel()) {
+                if ((rc = sync_slots(snap_d + NQ))) return rc;
+            } else {
+                hipLaunchKernelGGL(k_snapshot, 1, 1, 0, stream, d.st, snap_d + NQ);
+                HIPCHK(hipGetLastError());
+            }
+            HIPCHK(hipEventRecord(sev[NQ], stream));
+        }
+        int64_t clock = mirrored ? mirror.clock : 0, done = 0, planned = 0, issued = 0, read = 0;
         int64_t kq[NQ] = {};
         // parity of the buffers holding the state the device has reached
         // (the host's buffers run ahead by the queued one-kernel steps)
@@ -1172,6 +1195,7 @@ struct Sim final : fcc_sim {
         auto drain = [&]() -> int {
             HIPCHK(hipMemcpyAsync(&st, d.st, sizeof st, hipMemcpyDeviceToHost, stream));
             HIPCHK(hipStreamSynchronize(stream));
+            last = st;
             if (gone) pact ^= (int)((st.clock - clock) & 1);
             account(st.clock - clock);
             clock = st.clock;
@@ -1190,7 +1214,9 @@ struct Sim final : fcc_sim {
                     gbuilds = builds;
                     since_rebuild = 0;
                 }
-                const int64_t k = std::min(chunk_len(), n - planned);
+                // (a remainder of less than half a chunk joins the chunk)
+                const int64_t cl = chunk_len();
+                const int64_t k = n - planned < cl + cl / 2 ? n - planned : cl;
                 const int slot = (int)(issued % NQ);
                 if ((rc = issue(k, slot))) return rc;
                 kq[slot] = k;
@@ -1200,6 +1226,7 @@ struct Sim final : fcc_sim {
             if (!started) {
                 HIPCHK(hipEventSynchronize(sev[NQ]));
                 clock = snap[NQ].clock;
+                last = snap[NQ];
                 started = true;
                 if (has_contact && snap[NQ].rebuild != 0) {
                     if ((rc = drain()) || (rc = rebuild())) return rc;
@@ -1210,6 +1237,7 @@ struct Sim final : fcc_sim {
             const int slot = (int)(read % NQ);
             HIPCHK(hipEventSynchronize(sev[slot]));
             const DevStatus s = snap[slot];
+            last = s;
             read++;
             const int64_t executed = s.clock - clock;
             clock = s.clock;
@@ -1217,13 +1245,13 @@ struct Sim final : fcc_sim {
             // a rebuild is due: the flag, or a skin trip of the last one-kernel step
             const bool due = s.rebuild != 0 || (gone && s.tstep[pact] == s.clock - 1);
             trip_left = gone && s.rebuild == 0 && s.tstep[pact] == s.clock - 1;
-            if (want_timed && executed == kq[slot]) {  // the chunk's last step ran (not frozen)
+            if (want_timed && executed == kq[slot]) {  // every step of the chunk ran (none frozen)
                 float a = 0.f, b = 0.f;
                 HIPCHK(hipEventElapsedTime(&a, tev[slot][0], tev[slot][1]));
                 if (!gone) HIPCHK(hipEventElapsedTime(&b, tev[slot][2], tev[slot][3]));
                 acc_f += a;
                 acc_u += b;
-                acc_n++;
+                acc_n += gone ? kq[slot] : 1;  // steps of the chunk, or its eager last step
             }
             account(executed);
             if (due && (rc = drain())) return rc;
@@ -1238,6 +1266,9 @@ struct Sim final : fcc_sim {
         }
         set_parity(pact);
         if (trip_left) return sync_slots();  // a trip of the last step becomes the rebuild flag
+        // (the last snapshot is the status the device is left with)
+        mirror = last;
+        mirror_ok = last.rebuild == 0 && last.clock == clock && read == issued;
         return FCC_OK;
     }
 
@@ -2013,15 +2044,19 @@ int fcc_destroy(fcc_sim *s) {
 
 #define FCC_HANDLE(s) \
     if (!(s)) return fail(FCC_ERR_ARGUMENT, "null simulation handle")
+// entry points that may change the device status or the state
+#define FCC_HANDLE_MUT(s) \
+    FCC_HANDLE(s);        \
+    (s)->mirror_ok = false
 
-int fcc_set_bodies(fcc_sim *s, const fcc_bodies *b) { FCC_HANDLE(s); return s->set_bodies(b); }
+int fcc_set_bodies(fcc_sim *s, const fcc_bodies *b) { FCC_HANDLE_MUT(s); return s->set_bodies(b); }
 int fcc_get_bodies(fcc_sim *s, fcc_bodies *b) { FCC_HANDLE(s); return s->get_bodies(b); }
-int fcc_set_dt(fcc_sim *s, double dt) { FCC_HANDLE(s); return s->set_dt(dt); }
-int fcc_set_bonds(fcc_sim *s, int64_t n, const int32_t *pairs) { FCC_HANDLE(s); return s->set_bonds(n, pairs); }
-int fcc_step(fcc_sim *s, int64_t n) { FCC_HANDLE(s); return s->step(n); }
+int fcc_set_dt(fcc_sim *s, double dt) { FCC_HANDLE_MUT(s); return s->set_dt(dt); }
+int fcc_set_bonds(fcc_sim *s, int64_t n, const int32_t *pairs) { FCC_HANDLE_MUT(s); return s->set_bonds(n, pairs); }
+int fcc_step(fcc_sim *s, int64_t n) { FCC_HANDLE_MUT(s); return s->step(n); }
 int fcc_run(fcc_sim *s, int64_t n) { FCC_HANDLE(s); return s->run(n); }
-int fcc_update_neighbors(fcc_sim *s, int32_t force) { FCC_HANDLE(s); return s->update_neighbors(force); }
-int fcc_compute_gradients(fcc_sim *s, int32_t frame) { FCC_HANDLE(s); return s->compute_gradients(frame); }
+int fcc_update_neighbors(fcc_sim *s, int32_t force) { FCC_HANDLE_MUT(s); return s->update_neighbors(force); }
+int fcc_compute_gradients(fcc_sim *s, int32_t frame) { FCC_HANDLE_MUT(s); return s->compute_gradients(frame); }
 int fcc_get_gradients(fcc_sim *s, double *gx, double *gy, double *gth, double *glab, double *gcoef, double *E) {
     FCC_HANDLE(s);
     return s->get_gradients(gx, gy, gth, glab, gcoef, E);
@@ -2044,14 +2079,14 @@ int fcc_get_neighbors(fcc_sim *s, int32_t *cap, int32_t *count, int32_t *idx) {
     FCC_HANDLE(s);
     return s->get_neighbors(cap, count, idx);
 }
-int fcc_set_neighbor_capacity(fcc_sim *s, int32_t cap) { FCC_HANDLE(s); return s->set_neighbor_capacity(cap); }
+int fcc_set_neighbor_capacity(fcc_sim *s, int32_t cap) { FCC_HANDLE_MUT(s); return s->set_neighbor_capacity(cap); }
 int fcc_get_counters(fcc_sim *s, int64_t *step, double *t, int64_t *clock, int64_t *builds) {
     FCC_HANDLE(s);
     return s->get_counters(step, t, clock, builds);
 }
 
 int fcc_time_step_kernels(fcc_sim *s, int64_t steps, double *fms, double *ums) {
-    FCC_HANDLE(s);
+    FCC_HANDLE_MUT(s);
     if (steps < 0) return fail(FCC_ERR_ARGUMENT, "steps must be non-negative");
     return s->time_step_kernels(steps, fms, ums);
 }
@@ -2063,7 +2098,7 @@ int fcc_time_kernels(fcc_sim *s, int64_t reps, double *fms, double *ums) {
 }
 
 int fcc_time_rebuild(fcc_sim *s, int64_t reps, double *stage_ms) {
-    FCC_HANDLE(s);
+    FCC_HANDLE_MUT(s);
     if (reps < 1 || !stage_ms) return fail(FCC_ERR_ARGUMENT, "need reps >= 1 and an output array");
     return s->time_rebuild(reps, stage_ms);
 }
@@ -2084,7 +2119,7 @@ static int check_observer(const fcc_observer_config *c) {
 }
 
 int fcc_observer_start(fcc_sim *s, const fcc_observer_config *c) {
-    FCC_HANDLE(s);
+    FCC_HANDLE_MUT(s);
     int rc = check_observer(c);
     return rc ? rc : s->observer_start(c);
 }
@@ -2101,7 +2136,7 @@ int fcc_checkpoint_save(fcc_sim *s, void *buf, int64_t bytes, int64_t *needed) {
     return s->checkpoint(buf, bytes, needed);
 }
 int fcc_checkpoint_load(fcc_sim *s, const void *buf, int64_t bytes) {
-    FCC_HANDLE(s);
+    FCC_HANDLE_MUT(s);
     return s->restore(buf, bytes);
 }
 int fcc_run_observed(fcc_sim *s, int64_t n, int64_t every) {

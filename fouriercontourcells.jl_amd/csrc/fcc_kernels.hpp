@@ -1116,13 +1116,19 @@ __global__ void k_clear_status(DevStatus *st) {
 
 // Before one-kernel steps reading the buffers of parity q: a skin trip of the
 // last one-kernel step (marked at tstep[q] with clock - 1) raises the rebuild
-// flag, the markers are cleared and the step clock slot takes the clock.
-__global__ void k_step_slots(DevStatus *st, int q) {
+// flag, the markers are cleared and the step clock slot takes the clock. With
+// `out` (pinned host memory), the resulting status is also copied there.
+__global__ void k_step_slots(DevStatus *st, int q, DevStatus *out) {
     const int64_t c = st->clock;
     if (st->tstep[q] == c - 1) st->rebuild = 1;
     st->tstep[0] = st->tstep[1] = INT64_MIN;
     st->cslot[q] = c;
+    if (out) *out = *st;
 }
+
+// A copy of the status into pinned host memory (the snapshot the host reads
+// after the event recorded behind it).
+__global__ void k_snapshot(const DevStatus *st, DevStatus *out) { *out = *st; }
 
 // Step budget of a simulation: freeze once `clock` reaches `target`.
 __global__ void k_set_target(DevStatus *st, int64_t target) {

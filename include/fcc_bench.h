@@ -23,11 +23,15 @@ extern "C" {
 int fcc_time_step_kernels(fcc_sim *sim, int64_t steps, double *forces_ms,
                           double *update_ms);
 
-/* fcc_run (graph mode) where the last step of every chunk is launched eagerly
- * (same kernels and arguments as the captured step) between HIP events on the
- * simulation's stream; returns the average forces and update kernel
- * durations over those steps, in milliseconds (one-kernel steps: the step
- * kernel's time and 0). Results equal fcc_run's. */
+/* fcc_run (graph mode) with HIP events on the simulation's stream; returns
+ * average kernel durations in milliseconds. One-kernel steps: events around
+ * each chunk's graph launches, forces_ms = the chunk's time over its steps
+ * (the step kernel's duration: inside a graph the kernels follow each other
+ * without a gap), update_ms 0. Two-kernel steps: the last step of every
+ * chunk is launched eagerly (same kernels and arguments as the captured
+ * step) with events around each kernel, forces_ms and update_ms the averages
+ * over those steps. Chunks with frozen steps are left out. Results equal
+ * fcc_run's. */
 int fcc_run_timed(fcc_sim *sim, int64_t n, double *forces_ms, double *update_ms);
 
 /* Average duration of the forces kernel and of the update kernel(s) of a

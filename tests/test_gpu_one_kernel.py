@@ -265,3 +265,44 @@ def test_one_kernel_ensemble_with_both_lane_counts(precision):
     ens.close()
     for s in solo:
         s.close()
+
+
+@pytest.mark.parametrize("two_kernels", [False, True])
+@pytest.mark.parametrize("lanes", [2, 4])
+def test_short_runs_between_other_calls(two_kernels, lanes):
+    # consecutive runs start from the status the run before left (no start
+    # snapshot) unless a call in between may have changed it: short graph
+    # runs with reads, kernel timing, forced list builds, a dt change,
+    # checkpoint round trips, timed runs and eager steps in between, against
+    # the same sequence of eager steps
+    import ctypes
+    say("short runs between other calls", two_kernels, lanes)
+    a, b = (monolayer("mixed_fp16", two_kernels, two_lanes=lanes == 2, v0=1.2, skin=0.05) for _ in range(2))
+    f, u = ctypes.c_double(), ctypes.c_double()
+    for i, n in enumerate((1, 2, 3, 5, 8, 13, 1, 1, 21, 34, 2, 55, 4)):
+        F.run(a, n)
+        F.step(b, n)
+        if i == 1:
+            a.counters(), b.counters()
+            a.bodies, b.bodies
+        elif i == 3:
+            assert a.engine.time_kernels(a._h, 3, ctypes.byref(f), ctypes.byref(u)) == 0
+        elif i == 4:
+            F.update_neighbors(a, True)
+            F.update_neighbors(b, True)
+        elif i == 6:
+            a.dt = b.dt = 0.004
+        elif i == 7:
+            a.restore(a.checkpoint())
+            b.restore(b.checkpoint())
+        elif i == 8:
+            assert a.engine.run_timed(a._h, 6, ctypes.byref(f), ctypes.byref(u)) == 0
+            F.step(b, 6)
+        elif i == 10:
+            F.step(a, 3)
+            F.step(b, 3)
+        elif i == 11:
+            F.compute_gradients(a)
+            F.compute_gradients(b)
+        assert_same(a, b, f"after run {i} ({n} steps)")
+    assert a.counters().builds > 4
