@@ -19,7 +19,9 @@ fcc_time_kernels) and the update kernel of a two-kernel step, the list-build
 stages, the FP64 CPU baseline (the oracle, OpenMP over bodies) on a bounded
 sample of the same workload, and the batched-ensemble workload (64 replicas
 of uneven size in one launch per kernel and step) next to the same replicas
-run as independent simulations. The timed run and the kernel timing are
+run as independent simulations, and the two N = 65536 workloads (`workloads`:
+a polydisperse FP32 monolayer, cells with nuclei in mixed precision and plain
+FP32) over the same steps. The timed run and the kernel timing are
 roctx ranges ("timed", "kernel_timing") for rocprofv3 --marker-trace. Fields
 read from committed profiles name their file (`traffic_source`,
 `issue_source`).
@@ -127,6 +129,7 @@ def bench_workload(name, precision, steps, warmup):
         if tk:
             prec = prec + "_two_kernels"
         F.run(sim, warmup)
+        clock_warmup(sim)
         sec, _, _ = timed_run(sim, steps)
         fms, ums = kernel_durations(sim)
         v = sim.N * steps / sec
@@ -445,6 +448,8 @@ def main():
                     help="only the ensemble leg (its own JSON line)")
     ap.add_argument("--no-two-kernels", action="store_true",
                     help="without the leg that runs the workload with two kernels per step")
+    ap.add_argument("--no-workloads", action="store_true",
+                    help="without the polydisperse and nuclei workloads (N=65536) of the full line")
     ap.add_argument("--no-independent", action="store_true",
                     help="ensemble leg without the independent-simulations comparison")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -552,6 +557,11 @@ def main():
         ensemble = bench_ensemble(args.ensemble_steps, min(args.warmup, 200), max(1, args.ensemble_steps // 4),
                                   independent=not args.no_independent)
 
+    # the other workloads, each with its paths, over the same steps
+    workloads = None
+    if not args.no_workloads:
+        workloads = {name: bench_workload(name, None, args.steps, args.warmup) for name in sorted(WORKLOADS)}
+
     cpu = None if args.no_cpu else cpu_baseline(args.n, args.cpu_seconds)
     timing_note = ("HIP events on the simulation's stream around the graph launches of each chunk of the "
                    "timed run, the chunk's time over its steps (one-kernel steps; two-kernel steps: around the "
@@ -639,6 +649,7 @@ def main():
         "roofline_binning": binning,
         "cpu_baseline": cpu,
         "ensemble": ensemble,
+        "workloads": workloads,
     }
     print(json.dumps(line), flush=True)
 
