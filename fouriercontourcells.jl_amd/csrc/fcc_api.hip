@@ -631,10 +631,7 @@ struct Sim final : fcc_sim {
         if ((rc = alloc(&d.cidx, cidx.size())) || (rc = alloc(&d.ccount, N))) return rc;
         HIPCHK(hipMemcpy(d.cidx, cidx.data(), sizeof(int32_t) * cidx.size(), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(d.ccount, cnt.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice));
-        if (has_pdrag) {
-            if ((rc = alloc(&d.disp, 3 * N))) return rc;
-            HIPCHK(hipMemsetAsync(d.disp, 0, sizeof(T) * 3 * N, stream));
-        }
+        if (has_pdrag && (rc = alloc(&d.th0, N))) return rc;
         return set_bodies(b);
     }
 
@@ -756,12 +753,10 @@ struct Sim final : fcc_sim {
 
     int launch_update(hipEvent_t eb = nullptr, hipEvent_t ee = nullptr) {
         const unsigned g = blocks(N, 256);
-        const int ud = has_pdrag ? 1 : 0, dbg = debug ? 1 : 0;
-        if (has_pdrag) FCC_LAUNCH((k_displacement<T>), g, 256, 0, stream, eb, nullptr, p, d);
-        hipEvent_t eu = has_pdrag ? nullptr : eb;
+        const int pd = has_pdrag ? 1 : 0, dbg = debug ? 1 : 0;
 #define FCC_UPD(KMV, EX)                                                                                   \
-    if (mixed) FCC_LAUNCH((k_update<T, KMV, EX, true>), g, 256, 0, stream, eu, ee, p, d, ud, dbg);          \
-    else FCC_LAUNCH((k_update<T, KMV, EX, false>), g, 256, 0, stream, eu, ee, p, d, ud, dbg);
+    if (mixed) FCC_LAUNCH((k_update<T, KMV, EX, true>), g, 256, 0, stream, eb, ee, p, d, pd, dbg);          \
+    else FCC_LAUNCH((k_update<T, KMV, EX, false>), g, 256, 0, stream, eb, ee, p, d, pd, dbg);
         if (K == 5) {
             FCC_UPD(5, true)
         } else if (K == 7) {

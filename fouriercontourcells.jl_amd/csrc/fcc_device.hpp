@@ -162,7 +162,11 @@ struct Ptrs {
     T *labn, *rmaxn, *rminn;
     int32_t *cidx, *ccount;
     int32_t ccap;
-    T *disp;
+    // parent drag: θ at the start of the step, stored by the forces kernels
+    // (which do not write θ), so that a child's update reads its parent's
+    // orientation while the parent's own update overwrites θ (null without
+    // parent drag)
+    T *th0;
     int32_t *bidx, *bcount;  // bond partners, slot-major (bidx[q*N + i]); null without bonds
     const T *S;   // [3][nth][ldc]: C, D, DD rows
     const T *cs;  // [nth][2]: cos φk, sin φk

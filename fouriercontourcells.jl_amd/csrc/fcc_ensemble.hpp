@@ -322,13 +322,11 @@ struct Ensemble final : fcc_ensemble {
 
     int launch_update(hipEvent_t eb = nullptr, hipEvent_t ee = nullptr) {
         const Sim<T> &a = *m[0];
-        const int ud = a.has_pdrag ? 1 : 0, dbg = a.debug ? 1 : 0;
+        const int pd = a.has_pdrag ? 1 : 0, dbg = a.debug ? 1 : 0;
         const int ub = lc.update_block;
-        if (ud) FCC_LAUNCH((k_displacement_ens<T>), gu, ub, 0, stream, eb, nullptr, dP, dD, tu);
-        hipEvent_t eu = ud ? nullptr : eb;
 #define FCC_UE(KMV, EX)                                                                                       \
-    if (a.mixed) FCC_LAUNCH((k_update_ens<T, KMV, EX, true>), gu, ub, 0, stream, eu, ee, dP, dD, tu, ud, dbg); \
-    else FCC_LAUNCH((k_update_ens<T, KMV, EX, false>), gu, ub, 0, stream, eu, ee, dP, dD, tu, ud, dbg);
+    if (a.mixed) FCC_LAUNCH((k_update_ens<T, KMV, EX, true>), gu, ub, 0, stream, eb, ee, dP, dD, tu, pd, dbg); \
+    else FCC_LAUNCH((k_update_ens<T, KMV, EX, false>), gu, ub, 0, stream, eb, ee, dP, dD, tu, pd, dbg);
         if (a.K == 5) {
             FCC_UE(5, true)
         } else if (a.K == 7) {

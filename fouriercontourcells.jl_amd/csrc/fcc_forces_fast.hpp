@@ -378,6 +378,7 @@ __device__ __forceinline__ void k_forces_fast_impl(const PP &p, const DP &d, int
     d.gy[i] = fy;
     d.gth[i] = tt;
     d.Ebody[i] = E;
+    if (d.th0) d.th0[i] = d.th[i];  // parent drag: θ at the start of the step
 }
 
 template <typename T, int NTH_, int K, int NCLS>
@@ -732,6 +733,7 @@ __device__ __forceinline__ void k_forces_fast2_impl(const PP &p, const DP &d, in
         d.gy[ii] = fy;
         d.gth[ii] = tt;
         d.Ebody[ii] = E;
+        if (d.th0) d.th0[ii] = d.th[ii];  // parent drag: θ at the start of the step
     }
     if constexpr (NW > 1) {
         // the same row, force and torque into the wave's LDS slice (pass 2

@@ -448,6 +448,7 @@ __device__ __forceinline__ void k_forces_mfma_impl(const PP &p, const DP &d,
         d.gy[ii] = fy;
         d.gth[ii] = tt;
         d.Ebody[ii] = E;
+        if (d.th0) d.th0[ii] = d.th[ii];  // parent drag: θ at the start of the step
     }
     if constexpr (NW == 2) {
         // the same rows into the wave's LDS slice: body rr of the wave at

@@ -428,6 +428,7 @@ __device__ __forceinline__ void k_forces_mfma4_impl(const PP &p, const DP &d, co
         d.gy[ii] = fy;
         d.gth[ii] = tt;
         d.Ebody[ii] = E;
+        if (d.th0) d.th0[ii] = d.th[ii];  // parent drag: θ at the start of the step
     }
     if constexpr (NW > 1) {
         // the same rows into the wave's LDS slice (the contact pass has read

@@ -243,6 +243,7 @@ __device__ __forceinline__ void k_forces_wide_impl(const PP &p, const DP &d, int
         d.gy[i] = acc[NCM + 1];
         d.gth[i] = acc[NCM + 2];
         d.Ebody[i] = E;
+        if (d.th0) d.th0[i] = d.th[i];  // parent drag: θ at the start of the step
     }
 }
 

@@ -219,6 +219,7 @@ __device__ __forceinline__ void k_forces_impl(const PP &p, const DP &d, int use_
     d.gy[i] = fy;
     d.gth[i] = tt;
     d.Ebody[i] = E;
+    if (d.th0) d.th0[i] = d.th[i];  // parent drag: θ at the start of the step
 }
 
 template <typename T, int NTH_, int KM>
@@ -463,9 +464,10 @@ __global__ __launch_bounds__(256) void k_coef_from_lab(const Params<T> p, const 
     store_row<T, KM>(c, d.coef + i * p.ldc, p.K);
 }
 
-// substrate_displacement (src/dynamics.jl:84-99) of body c.
+// substrate_displacement (src/dynamics.jl:84-99) of body c with its
+// orientation thc.
 template <typename T, typename PP, typename DP>
-__device__ __forceinline__ void substrate_disp(const PP &p, const DP &d, int64_t c, uint64_t draw, T &ux,
+__device__ __forceinline__ void substrate_disp(const PP &p, const DP &d, int64_t c, T thc, uint64_t draw, T &ux,
                                                T &uy) {
     const int tc = d.type[c];
     T fx = d.gx[c], fy = d.gy[c];
@@ -478,7 +480,7 @@ __device__ __forceinline__ void substrate_disp(const PP &p, const DP &d, int64_t
         }
     }
     T s, co;
-    fsincos(d.th[c], &s, &co);
+    fsincos(thc, &s, &co);
     const T st = sqrt(T(2) * p.Dt[tc] * p.dt);
     T x1 = T(0), x2 = T(0);
     if (p.Dt[tc] != T(0)) philox_pair<T>(p.key0, p.key1, (uint32_t)(c + 1), draw, 0, x1, x2);
@@ -486,48 +488,32 @@ __device__ __forceinline__ void substrate_disp(const PP &p, const DP &d, int64_t
     uy = p.dt * (p.v0[tc] * s - p.mur[tc] * fy) + st * x2;
 }
 
-// displacement_kernel! (src/dynamics.jl:128-147), used when some body type
-// has parent drag: a child needs its parent's θ before the update.
-template <typename T, int UB, typename PP, typename DP>
-__device__ __forceinline__ void k_displacement_impl(const PP &p, const DP &d, int64_t blk) {
-    const int64_t i = (int64_t)blk * (UB ? UB : (int)blockDim.x) + threadIdx.x;
-    if (i >= p.N || d.st->update != 0) return;
-    const uint64_t draw = (uint64_t)(d.st->clock - 1);
-    const int ti = d.type[i];
-    T d0, d1;
+// displacement_kernel! (src/dynamics.jl:128-147) of body i (type ti,
+// orientation thi, centre force (gx, gy)) when some body type has parent
+// drag: a child adds its parent's substrate displacement, whose orientation
+// is the parent's at the start of the step (d.th0, stored by the forces
+// kernel); a parent's includes the forces on its children with parent drag.
+// d2 is the rotational noise increment.
+template <typename T, typename PP, typename DP>
+__device__ __forceinline__ void pdrag_disp(const PP &p, const DP &d, int64_t i, int ti, T thi, T gx, T gy,
+                                           uint64_t draw, T &d0, T &d1, T &d2) {
     if (p.pdrag[ti]) {
+        const int64_t pr = d.parent[i];
         T ux, uy;
-        substrate_disp<T>(p, d, d.parent[i], draw, ux, uy);
+        substrate_disp<T>(p, d, pr, d.th0[pr], draw, ux, uy);
         T s, co;
-        fsincos(d.th[i], &s, &co);
+        fsincos(thi, &s, &co);
         const T st = sqrt(T(2) * p.Dt[ti] * p.dt);
         T x1 = T(0), x2 = T(0);
         if (p.Dt[ti] != T(0)) philox_pair<T>(p.key0, p.key1, (uint32_t)(i + 1), draw, 0, x1, x2);
-        d0 = ux + p.dt * (p.v0[ti] * co - p.mur[ti] * d.gx[i]) + st * x1;
-        d1 = uy + p.dt * (p.v0[ti] * s - p.mur[ti] * d.gy[i]) + st * x2;
+        d0 = ux + p.dt * (p.v0[ti] * co - p.mur[ti] * gx) + st * x1;
+        d1 = uy + p.dt * (p.v0[ti] * s - p.mur[ti] * gy) + st * x2;
     } else {
-        substrate_disp<T>(p, d, i, draw, d0, d1);
+        substrate_disp<T>(p, d, i, thi, draw, d0, d1);
     }
     T x3 = T(0), x4;
     if (p.Dr[ti] != T(0)) philox_pair<T>(p.key0, p.key1, (uint32_t)(i + 1), draw, 1, x3, x4);
-    d.disp[3 * i] = d0;
-    d.disp[3 * i + 1] = d1;
-    d.disp[3 * i + 2] = sqrt(T(2) * p.Dr[ti] * p.dt) * x3;
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void k_displacement(const Params<T> p, const Ptrs<T> d) {
-    k_displacement_impl<T, 256>(p, d, blockIdx.x);
-}
-
-// Ensemble launch: any workgroup size up to 256 (the tile size of the table).
-template <typename T>
-__global__ __launch_bounds__(256) void k_displacement_ens(const Params<T> *ps, const Ptrs<T> *ds, const Tile *tiles) {
-    const Tile tl = load_uniform(tiles, blockIdx.x);
-    if (tl.rep < 0) return;
-    const FCC_CONST Params<T> &p = as_const(ps)[tl.rep];
-    const FCC_CONST Ptrs<T> &d = as_const(ds)[tl.rep];
-    k_displacement_impl<T, 0>(p, d, tl.blk);
+    d2 = sqrt(T(2) * p.Dr[ti] * p.dt) * x3;
 }
 
 // Pieces of the Euler–Maruyama step. They and the helpers they call
@@ -709,7 +695,7 @@ template <typename T, int KM>
 struct BodyIn {
     int ti;
     T th, r0, gth;
-    T f0, f1, f2;  // centre force (gx, gy), or the displacement (f2 unused without it)
+    T f0, f1;  // centre force (gx, gy)
     double x, y, x0, y0;
     T rmax0;
     T gl[2 * KM + 1], c[2 * KM + 1];
@@ -719,20 +705,15 @@ struct BodyIn {
 // round trip covers the body's whole state (behind the noise draws, which
 // branch on the type's diffusion constants, each load would wait apart).
 template <typename T, int KM, bool EXACT, typename PP, typename DP>
-__device__ __forceinline__ void load_body(const PP &p, const DP &d, int64_t i, int K, int use_disp,
-                                          BodyIn<T, KM> &b) {
+__device__ __forceinline__ void load_body(const PP &p, const DP &d, int64_t i, int K, BodyIn<T, KM> &b) {
     b.ti = d.type[i];
     b.th = d.th[i];
     b.r0 = d.R0[i];
     b.gth = d.gth[i];
     b.x = d.x[i];
     b.y = d.y[i];
-    const T *const q0 = use_disp ? d.disp + 3 * i : d.gx + i;
-    const T *const q1 = use_disp ? d.disp + 3 * i + 1 : d.gy + i;
-    const T *const q2 = use_disp ? d.disp + 3 * i + 2 : d.gx + i;
-    b.f0 = *q0;
-    b.f1 = *q1;
-    b.f2 = *q2;
+    b.f0 = d.gx[i];
+    b.f1 = d.gy[i];
     load_coef<T, KM, EXACT>(b.gl, d.glab + i * p.ldc, K);
     load_coef<T, KM, EXACT>(b.c, d.lab + i * p.ldc, K);
     b.x0 = d.x0[i];  // the skin reference is allocated with the neighbour list
@@ -743,21 +724,21 @@ __device__ __forceinline__ void load_body(const PP &p, const DP &d, int64_t i, i
 // The update of body i from its loaded inputs: displacement and noise, θ and
 // shape step, wrap, lab frame, skin check and stores. sin θ, cos θ and the
 // harmonics of θ are computed once for the displacement, the body-frame
-// coefficients and the body-frame gradient. Returns whether a body of the
+// coefficients and the body-frame gradient. With `pdrag` (some body type has
+// parent drag) the displacement is that of displacement_kernel!
+// (pdrag_disp). Returns whether a body of the
 // wave tripped the skin check (which also sets *trip unless it is null);
 // with `budget` body 0 takes the step budget into `stop`.
 template <typename T, int KM, bool EXACT, bool F64S, typename PP, typename DP>
-__device__ __forceinline__ bool finish_body(const PP &p, const DP &d, int64_t i, int K, int use_disp, int debug,
+__device__ __forceinline__ bool finish_body(const PP &p, const DP &d, int64_t i, int K, int pdrag, int debug,
                                             uint64_t draw, BodyIn<T, KM> &b, int32_t *trip, bool budget) {
     constexpr int NCM = 2 * KM + 1;
     const TypeParams<T> tp = type_params<T>(p, b.ti);
     T cn[KM + 1], sn[KM + 1];
     harmonics<T, KM>(b.th, K, cn, sn);
     T d0, d1, d2;
-    if (use_disp) {
-        d0 = b.f0;
-        d1 = b.f1;
-        d2 = b.f2;
+    if (pdrag) {
+        pdrag_disp<T>(p, d, i, b.ti, b.th, b.f0, b.f1, draw, d0, d1, d2);
     } else {
         T s, co;
         if constexpr (sizeof(T) == 4) {
@@ -779,36 +760,36 @@ __device__ __forceinline__ bool finish_body(const PP &p, const DP &d, int64_t i,
 }
 
 // Fused Euler–Maruyama update (src/dynamics.jl:180-203) with the
-// displacement and Philox noise of src/dynamics.jl:84-99,128-147 (bodies
-// without parent drag), the body-frame gradient, the shape constraint (FP64
+// displacement and Philox noise of src/dynamics.jl:84-99,128-147 (with
+// `pdrag`, the displacement_kernel! form with parent drag), the body-frame gradient, the shape constraint (FP64
 // when F64S), position wrapping in FP64, the lab frame of the new state and
 // the Verlet skin check (src/neighbors.jl:198-205); one body per lane.
 template <typename T, int KM, bool EXACT, bool F64S, int UB, typename PP, typename DP>
-__device__ __forceinline__ void k_update_impl(const PP &p, const DP &d, int use_disp, int debug, int64_t blk) {
+__device__ __forceinline__ void k_update_impl(const PP &p, const DP &d, int pdrag, int debug, int64_t blk) {
     const int64_t i = (int64_t)blk * (UB ? UB : (int)blockDim.x) + threadIdx.x;
     if (i >= p.N) return;
     const int K = EXACT ? KM : p.K;
     BodyIn<T, KM> b;
-    load_body<T, KM, EXACT>(p, d, i, K, use_disp, b);
+    load_body<T, KM, EXACT>(p, d, i, K, b);
     if (d.st->update != 0) return;
     const uint64_t draw = (uint64_t)(d.st->clock - 1);
-    (void)finish_body<T, KM, EXACT, F64S>(p, d, i, K, use_disp, debug, draw, b, &d.st->rebuild, true);
+    (void)finish_body<T, KM, EXACT, F64S>(p, d, i, K, pdrag, debug, draw, b, &d.st->rebuild, true);
 }
 
 template <typename T, int KM, bool EXACT, bool F64S>
-__global__ __launch_bounds__(256) void k_update(const Params<T> p, const Ptrs<T> d, int use_disp, int debug) {
-    k_update_impl<T, KM, EXACT, F64S, 256>(p, d, use_disp, debug, blockIdx.x);
+__global__ __launch_bounds__(256) void k_update(const Params<T> p, const Ptrs<T> d, int pdrag, int debug) {
+    k_update_impl<T, KM, EXACT, F64S, 256>(p, d, pdrag, debug, blockIdx.x);
 }
 
 // Ensemble launch: any workgroup size up to 256 (the tile size of the table).
 template <typename T, int KM, bool EXACT, bool F64S>
 __global__ __launch_bounds__(256) void k_update_ens(const Params<T> *ps, const Ptrs<T> *ds, const Tile *tiles,
-                                                    int use_disp, int debug) {
+                                                    int pdrag, int debug) {
     const Tile tl = load_uniform(tiles, blockIdx.x);
     if (tl.rep < 0) return;
     const FCC_CONST Params<T> &p = as_const(ps)[tl.rep];
     const FCC_CONST Ptrs<T> &d = as_const(ds)[tl.rep];
-    k_update_impl<T, KM, EXACT, F64S, 0>(p, d, use_disp, debug, tl.blk);
+    k_update_impl<T, KM, EXACT, F64S, 0>(p, d, pdrag, debug, tl.blk);
 }
 
 // Update part of a one-kernel step (single simulations without parent drag,
@@ -853,7 +834,6 @@ __device__ __forceinline__ void fused_update(const PP &p, const DP &d, const T *
         b.f0 = f[0];
         b.f1 = f[1];
         b.gth = f[2];
-        b.f2 = b.f0;
         DP dn = d;
         dn.x = d.xn;
         dn.y = d.yn;

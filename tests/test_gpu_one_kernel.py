@@ -150,14 +150,18 @@ def test_one_kernel_observer_and_frames(lanes):
 
 
 @pytest.mark.parametrize("precision", ["mixed_fp16", "fp32"])
-def test_one_kernel_at_full_size(precision):
-    # the bench workload (N = 262144, default lane choice: two lanes, one kernel)
-    say("one kernel full size", precision)
+@pytest.mark.parametrize("n,two_lanes", [(512, False), (256, False), (256, True)])
+def test_one_kernel_at_full_size(precision, n, two_lanes):
+    # the bench workload (N = 262144, default lane choice: two lanes, one
+    # kernel) and N = 65536 (four lanes by default, or two), each with many
+    # workgroups resident per CU
+    say("one kernel large", precision, n, two_lanes)
     def mk(tk):
-        b, box = F.hexagonal_monolayer(512, 512, packing=1.0, maxmode=8, seed=1234)
+        b, box = F.hexagonal_monolayer(n, n, packing=1.0, maxmode=8, seed=1234)
         return F.Simulation(b, box, types=(F.BodyType(v0=0.1, Dr=0.01, muab=0.1),),
                             energies=(F.Tension(0.01), F.Overlap(1.0)), dt=0.005, ntheta=40,
-                            precision=precision, seed=1234, skin=0.3, two_kernels=tk)
+                            precision=precision, seed=1234, skin=0.3, two_kernels=tk,
+                            two_lanes=two_lanes)
     a = mk(False)
     F.run(a, 21)
     sa = state(a)
