@@ -88,7 +88,10 @@ struct Ensemble final : fcc_ensemble {
     int var = -1, tiles_var = -1, ncls = 1;  // forces kernel: 3 wide, 2 MFMA, 1 fast, 0 generic
     bool one = false, tiles_one = false;     // one kernel per step (k_step_*_ens)
     bool bonds = false;                      // a replica has bond terms on bonds: bond kernel after forces
-    hipGraphExec_t gexec = nullptr, gexec_n = nullptr;  // one step; GRAPH_STEPS steps
+    // captured steps: gx[b] holds 2^b steps (1 to 32), so k steps take
+    // k / 32 + popcount(k mod 32) graph launches
+    static constexpr int NGE = 6;
+    hipGraphExec_t gx[NGE] = {};
     bool graph_ok = false;
     std::vector<double> interval;  // mean steps between rebuilds, per replica
     std::vector<int64_t> since;
@@ -107,8 +110,8 @@ struct Ensemble final : fcc_ensemble {
     ~Ensemble() override {
         if (stream) (void)hipStreamSynchronize(stream);
         if (side) (void)hipStreamSynchronize(side);
-        if (gexec) (void)hipGraphExecDestroy(gexec);
-        if (gexec_n) (void)hipGraphExecDestroy(gexec_n);
+        for (hipGraphExec_t g : gx)
+            if (g) (void)hipGraphExecDestroy(g);
         for (int q = 0; q < NB; q++) {
             if (sev[q]) (void)hipEventDestroy(sev[q]);
             for (hipEvent_t e : tev[q])
@@ -423,13 +426,13 @@ struct Ensemble final : fcc_ensemble {
         }
         if (!graph_ok) {
             std::lock_guard<std::mutex> lk(g_capture_lock);
-            for (hipGraphExec_t *g : {&gexec, &gexec_n})
-                if (*g) {
-                    HIPCHK(hipGraphExecDestroy(*g));
-                    *g = nullptr;
+            for (hipGraphExec_t &g : gx)
+                if (g) {
+                    HIPCHK(hipGraphExecDestroy(g));
+                    g = nullptr;
                 }
-            int rc = capture_steps(1, &gexec);
-            if (!rc) rc = capture_steps(GRAPH_STEPS, &gexec_n);
+            int rc = FCC_OK;
+            for (int b = 0; b < NGE && !rc; b++) rc = capture_steps(1 << b, &gx[b]);
             if (rc) return rc;
             graph_ok = true;
         }
@@ -458,10 +461,13 @@ struct Ensemble final : fcc_ensemble {
         return FCC_OK;
     }
 
-    // k graph steps: k / GRAPH_STEPS launches of the long graph, the rest one by one.
+    // k graph steps: k / 32 launches of the longest graph, then one graph per
+    // set bit of k mod 32.
     int replay(int64_t k) {
-        for (int64_t q = 0; q < k / GRAPH_STEPS; q++) HIPCHK(hipGraphLaunch(gexec_n, stream));
-        for (int64_t q = 0; q < k % GRAPH_STEPS; q++) HIPCHK(hipGraphLaunch(gexec, stream));
+        constexpr int64_t LONG = 1 << (NGE - 1);
+        for (int64_t q = 0; q < k / LONG; q++) HIPCHK(hipGraphLaunch(gx[NGE - 1], stream));
+        for (int b = NGE - 2; b >= 0; b--)
+            if ((k % LONG) & (1 << b)) HIPCHK(hipGraphLaunch(gx[b], stream));
         return FCC_OK;
     }
 
@@ -491,13 +497,20 @@ struct Ensemble final : fcc_ensemble {
         return FCC_OK;
     }
 
+    // Steps per chunk: an eighth of the shortest rebuild-interval estimate
+    // (a tripped replica stays frozen until the chunk after its trip has
+    // been read), 8 to 32 steps, rounded down to a power of two (one graph
+    // launch per chunk).
     int64_t chunk_len() const {
         if (lc.chunk > 0) return lc.chunk;
         double mi = 1e300;
         for (size_t r = 0; r < m.size(); r++)
             if (m[r]->has_contact) mi = std::min(mi, interval[r]);
         if (mi >= 1e300) return 64;
-        return std::min<int64_t>(4 * GRAPH_STEPS, std::max<int64_t>(GRAPH_STEPS, (int64_t)std::llround(mi / 8)));
+        const int64_t c = std::min<int64_t>(4 * GRAPH_STEPS, std::max<int64_t>(GRAPH_STEPS, (int64_t)std::llround(mi / 8)));
+        int64_t p2 = GRAPH_STEPS;
+        while (2 * p2 <= c) p2 *= 2;
+        return p2;
     }
 
     int ensure_async() {
@@ -770,10 +783,9 @@ struct Ensemble final : fcc_ensemble {
                 if ((one && u > 0) || (ki == 0 && !one)) continue;
                 if ((rc = copy_regions(regions, snap, false))) break;
                 hipError_t e = hipSuccess;
-                for (int q = 0; q < 2 && e == hipSuccess; q++) e = hipGraphLaunch(gexec, stream);
+                for (int q = 0; q < 2 && e == hipSuccess; q++) e = hipGraphLaunch(gx[0], stream);
                 if (e == hipSuccess) e = hipEventRecord(e0, stream);
-                for (int64_t q = 0; q < steps / GRAPH_STEPS && e == hipSuccess; q++) e = hipGraphLaunch(gexec_n, stream);
-                for (int64_t q = 0; q < steps % GRAPH_STEPS && e == hipSuccess; q++) e = hipGraphLaunch(gexec, stream);
+                if (e == hipSuccess && (rc = replay(steps))) break;
                 if (e == hipSuccess) e = hipEventRecord(e1, stream);
                 if (e == hipSuccess) e = hipEventSynchronize(e1);
                 float t = 0.f;
