@@ -441,6 +441,7 @@ struct Sim final : fcc_sim {
             if (q < c->n_types && bt.parent_drag) has_pdrag = true;
         }
         p.dt = (T)c->dt;
+        noise_amplitudes();
         p.key0 = (uint32_t)c->seed;
         p.key1 = (uint32_t)(c->seed >> 32);
         p.skin = (T)skin;
@@ -1366,8 +1367,19 @@ struct Sim final : fcc_sim {
     int set_dt(double dt) override {
         cfg.dt = dt;
         p.dt = (T)dt;
+        noise_amplitudes();
         gen++;  // dt is baked into the captured graph
         return FCC_OK;
+    }
+
+    // √(2·D·dt) of every type in T: (2·D)·dt rounded to T, then the
+    // correctly rounded square root (the bits the kernels formed per body).
+    void noise_amplitudes() {
+        for (int q = 0; q < FCC_MAX_TYPES; q++) {
+            const T a = T(2) * p.Dt[q], b = T(2) * p.Dr[q];
+            p.sDt[q] = std::sqrt((T)(a * p.dt));
+            p.sDr[q] = std::sqrt((T)(b * p.dt));
+        }
     }
 
     int rows_out(double *out, const T *src) {

@@ -117,6 +117,9 @@ struct Params {
     // body types
     T v0[FCC_MAX_TYPES], Dr[FCC_MAX_TYPES], Dt[FCC_MAX_TYPES], mur[FCC_MAX_TYPES],
         muth[FCC_MAX_TYPES], muab[FCC_MAX_TYPES];
+    // noise amplitudes √(2·Dt·dt), √(2·Dr·dt) per type, formed on the host in
+    // T as the kernels would ((2·D)·dt, then the correctly rounded root)
+    T sDt[FCC_MAX_TYPES], sDr[FCC_MAX_TYPES];
     int pdrag[FCC_MAX_TYPES];
     T dt;
     uint32_t key0, key1;

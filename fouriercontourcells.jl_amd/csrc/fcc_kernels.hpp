@@ -317,6 +317,14 @@ __device__ __forceinline__ void enforce_shape(T (&c)[2 * KM + 1], int K) {
     c[0] = (T)sqrt(U(1) - q / U(2));
 }
 
+// Square root of a term of the radius bound: the hardware root in FP32
+// (within one ulp of the correctly rounded one, which the FP32 sum of the
+// terms rounds as much anyway; rmax and rmin enter the list cutoff, with the
+// skin, and the contact tests, next to an FP32-evaluated radius), the
+// correctly rounded one in FP64.
+__device__ __forceinline__ float bound_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double bound_sqrt(double x) { return sqrt(x); }
+
 // lab_frame! (src/geometry.jl:73-92) on registers.
 template <typename T, int KM>
 __device__ __forceinline__ T lab_frame(const T (&c)[2 * KM + 1], T th, T r0, int K, T (&l)[2 * KM + 1],
@@ -334,7 +342,7 @@ __device__ __forceinline__ T lab_frame(const T (&c)[2 * KM + 1], T th, T r0, int
             T a = c[m], b = c[KM + m];
             l[m] = a * cn[m] - b * sn[m];
             l[KM + m] = a * sn[m] + b * cn[m];
-            bound += sqrt(a * a + b * b);
+            bound += bound_sqrt(a * a + b * b);
         }
     }
     rmin = r0 * (c[0] - (bound - c[0]));
@@ -481,9 +489,9 @@ __device__ __forceinline__ void substrate_disp(const PP &p, const DP &d, int64_t
     }
     T s, co;
     fsincos(thc, &s, &co);
-    const T st = sqrt(T(2) * p.Dt[tc] * p.dt);
+    const T st = p.sDt[tc];
     T x1 = T(0), x2 = T(0);
-    if (p.Dt[tc] != T(0)) philox_pair<T>(p.key0, p.key1, (uint32_t)(c + 1), draw, 0, x1, x2);
+    if (st != T(0)) philox_pair<T>(p.key0, p.key1, (uint32_t)(c + 1), draw, 0, x1, x2);
     ux = p.dt * (p.v0[tc] * co - p.mur[tc] * fx) + st * x1;
     uy = p.dt * (p.v0[tc] * s - p.mur[tc] * fy) + st * x2;
 }
@@ -503,17 +511,17 @@ __device__ __forceinline__ void pdrag_disp(const PP &p, const DP &d, int64_t i, 
         substrate_disp<T>(p, d, pr, d.th0[pr], draw, ux, uy);
         T s, co;
         fsincos(thi, &s, &co);
-        const T st = sqrt(T(2) * p.Dt[ti] * p.dt);
+        const T st = p.sDt[ti];
         T x1 = T(0), x2 = T(0);
-        if (p.Dt[ti] != T(0)) philox_pair<T>(p.key0, p.key1, (uint32_t)(i + 1), draw, 0, x1, x2);
+        if (st != T(0)) philox_pair<T>(p.key0, p.key1, (uint32_t)(i + 1), draw, 0, x1, x2);
         d0 = ux + p.dt * (p.v0[ti] * co - p.mur[ti] * gx) + st * x1;
         d1 = uy + p.dt * (p.v0[ti] * s - p.mur[ti] * gy) + st * x2;
     } else {
         substrate_disp<T>(p, d, i, thi, draw, d0, d1);
     }
     T x3 = T(0), x4;
-    if (p.Dr[ti] != T(0)) philox_pair<T>(p.key0, p.key1, (uint32_t)(i + 1), draw, 1, x3, x4);
-    d2 = sqrt(T(2) * p.Dr[ti] * p.dt) * x3;
+    if (p.sDr[ti] != T(0)) philox_pair<T>(p.key0, p.key1, (uint32_t)(i + 1), draw, 1, x3, x4);
+    d2 = p.sDr[ti] * x3;
 }
 
 // Pieces of the Euler–Maruyama step. They and the helpers they call
@@ -527,7 +535,7 @@ __device__ __forceinline__ void pdrag_disp(const PP &p, const DP &d, int64_t i, 
 // would wait on the body's type).
 template <typename T>
 struct TypeParams {
-    T v0, mur, Dt, Dr, muth, muab;
+    T v0, mur, sDt, sDr, muth, muab;  // sDt, sDr: √(2·Dt·dt), √(2·Dr·dt)
 };
 
 // (readfirstlane keeps the compiler from folding the chain back into a load
@@ -560,8 +568,8 @@ __device__ __forceinline__ TypeParams<T> type_params(const PP &p, int ti) {
     TypeParams<T> tp;
     tp.v0 = type_param<T>(p.v0, ti);
     tp.mur = type_param<T>(p.mur, ti);
-    tp.Dt = type_param<T>(p.Dt, ti);
-    tp.Dr = type_param<T>(p.Dr, ti);
+    tp.sDt = type_param<T>(p.sDt, ti);
+    tp.sDr = type_param<T>(p.sDr, ti);
     tp.muth = type_param<T>(p.muth, ti);
     tp.muab = type_param<T>(p.muab, ti);
     return tp;
@@ -576,8 +584,8 @@ __device__ __forceinline__ void body_displacement(const PP &p, int64_t i, const 
                                                   T gx, T gy, uint64_t draw, T &d0, T &d1) {
 #pragma clang fp contract(off)
     T x1 = T(0), x2 = T(0);
-    if (tp.Dt != T(0)) philox_pair<T>(p.key0, p.key1, (uint32_t)(i + 1), draw, 0, x1, x2);
-    const T st = sqrt(T(2) * tp.Dt * p.dt);
+    if (tp.sDt != T(0)) philox_pair<T>(p.key0, p.key1, (uint32_t)(i + 1), draw, 0, x1, x2);
+    const T st = tp.sDt;
     d0 = p.dt * (tp.v0 * co - tp.mur * gx) + st * x1;
     d1 = p.dt * (tp.v0 * s - tp.mur * gy) + st * x2;
 }
@@ -587,8 +595,8 @@ template <typename T, typename PP>
 __device__ __forceinline__ T rotation_noise(const PP &p, int64_t i, const TypeParams<T> &tp, uint64_t draw) {
 #pragma clang fp contract(off)
     T x3 = T(0), x4;
-    if (tp.Dr != T(0)) philox_pair<T>(p.key0, p.key1, (uint32_t)(i + 1), draw, 1, x3, x4);
-    return sqrt(T(2) * tp.Dr * p.dt) * x3;
+    if (tp.sDr != T(0)) philox_pair<T>(p.key0, p.key1, (uint32_t)(i + 1), draw, 1, x3, x4);
+    return tp.sDr * x3;
 }
 
 // Body-frame gradient, orientation and shape update with the shape
