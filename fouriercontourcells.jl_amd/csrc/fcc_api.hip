@@ -441,6 +441,7 @@ struct Sim final : fcc_sim {
             if (q < c->n_types && bt.parent_drag) has_pdrag = true;
         }
         p.dt = (T)c->dt;
+        p.one_type = c->n_types == 1 ? 1 : 0;
         noise_amplitudes();
         p.key0 = (uint32_t)c->seed;
         p.key1 = (uint32_t)(c->seed >> 32);

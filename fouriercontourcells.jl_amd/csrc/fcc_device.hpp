@@ -117,6 +117,7 @@ struct Params {
     // body types
     T v0[FCC_MAX_TYPES], Dr[FCC_MAX_TYPES], Dt[FCC_MAX_TYPES], mur[FCC_MAX_TYPES],
         muth[FCC_MAX_TYPES], muab[FCC_MAX_TYPES];
+    int one_type;  // a single body type: the update takes its parameters without a select
     // noise amplitudes √(2·Dt·dt), √(2·Dr·dt) per type, formed on the host in
     // T as the kernels would ((2·D)·dt, then the correctly rounded root)
     T sDt[FCC_MAX_TYPES], sDr[FCC_MAX_TYPES];

@@ -566,6 +566,15 @@ __device__ __forceinline__ T type_param(const A &a, int ti) {
 template <typename T, typename PP>
 __device__ __forceinline__ TypeParams<T> type_params(const PP &p, int ti) {
     TypeParams<T> tp;
+    if (p.one_type) {  // uniform: every body is of type 0
+        tp.v0 = p.v0[0];
+        tp.mur = p.mur[0];
+        tp.sDt = p.sDt[0];
+        tp.sDr = p.sDr[0];
+        tp.muth = p.muth[0];
+        tp.muab = p.muab[0];
+        return tp;
+    }
     tp.v0 = type_param<T>(p.v0, ti);
     tp.mur = type_param<T>(p.mur, ti);
     tp.sDt = type_param<T>(p.sDt, ti);
