@@ -1374,7 +1374,8 @@ struct Sim final : fcc_sim {
     }
 
     // √(2·D·dt) of every type in T: (2·D)·dt rounded to T, then the
-    // correctly rounded square root (the bits the kernels formed per body).
+    // correctly rounded square root, the bits a per-body evaluation on the
+    // device gives (IEEE multiply and square root in both places).
     void noise_amplitudes() {
         for (int q = 0; q < FCC_MAX_TYPES; q++) {
             const T a = T(2) * p.Dt[q], b = T(2) * p.Dr[q];
